@@ -1,0 +1,285 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker / CPU baseline.  The product path
+(hybrid-sparse-pde-solver_amd) never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class Params(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("restart_size", C.c_int),
+                ("drop_tol", C.c_double), ("amg_levels", C.c_int), ("amg_strong", C.c_double),
+                ("amg_smooth_pre", C.c_int), ("amg_smooth_post", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("restarts", C.c_int),
+                ("state", C.c_int), ("escalations", C.c_int), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double), ("setup_time_s", C.c_double),
+                ("solve_time_s", C.c_double), ("flop_count", C.c_longlong),
+                ("mem_bytes", C.c_longlong), ("energy_joules", C.c_double)]
+
+
+def params(tol=1e-8, max_iter=2000, restart_size=50, drop_tol=1e-4, amg_levels=10,
+           amg_strong=0.25, pre=2, post=2):
+    return Params(tol, max_iter, restart_size, drop_tol, amg_levels, amg_strong, pre, post)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        sig = {
+            "orc_csr_poisson": (vp, [C.c_int]),
+            "orc_csr_convdiff": (vp, [C.c_int, C.c_double, C.c_double, C.c_double]),
+            "orc_csr_aniso": (vp, [C.c_int, C.c_double]),
+            "orc_csr_from_coo": (vp, [C.c_int, C.c_int, C.c_int, ip, ip, dp]),
+            "orc_csr_from_csr": (vp, [C.c_int, C.c_int, ip, ip, dp]),
+            "orc_csr_free": (None, [vp]),
+            "orc_csr_info": (None, [vp, ip, ip, ip]),
+            "orc_csr_export": (None, [vp, ip, ip, dp]),
+            "orc_spmv": (None, [vp, dp, dp]),
+            "orc_uniform": (None, [C.c_ulonglong, C.c_int, dp]),
+            "orc_solve": (C.c_int, [vp, dp, dp, C.c_int, C.c_int, C.POINTER(Params), C.POINTER(Stats)]),
+            "orc_precond_apply": (C.c_int, [vp, C.c_int, C.POINTER(Params), dp, dp]),
+            "orc_adaptive": (C.c_int, [vp, dp, dp, C.POINTER(Params), C.c_int, C.POINTER(Stats)]),
+            "orc_features": (C.c_int, [vp, dp]),
+            "orc_amg_setup": (vp, [vp, C.POINTER(Params)]),
+            "orc_amg_free": (None, [vp]),
+            "orc_amg_levels": (C.c_int, [vp]),
+            "orc_amg_level_matrix": (vp, [vp, C.c_int, C.c_int]),
+            "orc_amg_aggregates": (C.c_int, [vp, C.c_int, ip]),
+            "orc_amg_apply": (None, [vp, dp, dp]),
+            "orc_aggregate": (C.c_int, [vp, C.c_double, ip]),
+            "orc_gepp_solve": (C.c_int, [vp, dp, dp]),
+            "orc_transpose": (vp, [vp]),
+            "orc_matmat": (vp, [vp, vp]),
+            "orc_ilu_factor": (vp, [vp, C.c_double, C.c_int]),
+            "orc_last_error": (C.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Csr:
+    """Oracle-side CSR (int32 indices, FP64 values) owned by liboracle."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_csr_free(self.h)
+            self.h = None
+
+    @staticmethod
+    def poisson(n):
+        return Csr(lib().orc_csr_poisson(n))
+
+    @staticmethod
+    def convdiff(n, eps, bx, by):
+        return Csr(lib().orc_csr_convdiff(n, eps, bx, by))
+
+    @staticmethod
+    def aniso(n, eps):
+        return Csr(lib().orc_csr_aniso(n, eps))
+
+    @staticmethod
+    def from_coo(rows, cols, r, c, v):
+        r = np.ascontiguousarray(r, np.int32)
+        c = np.ascontiguousarray(c, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        return Csr(lib().orc_csr_from_coo(rows, cols, len(v), _ip(r), _ip(c), _dp(v)))
+
+    @staticmethod
+    def from_csr(rows, cols, rp, ci, v):
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        return Csr(lib().orc_csr_from_csr(rows, cols, _ip(rp), _ip(ci), _dp(v)))
+
+    def shape_nnz(self):
+        r, c, z = C.c_int(), C.c_int(), C.c_int()
+        lib().orc_csr_info(self.h, C.byref(r), C.byref(c), C.byref(z))
+        return r.value, c.value, z.value
+
+    @property
+    def rows(self):
+        return self.shape_nnz()[0]
+
+    def arrays(self):
+        r, c, z = self.shape_nnz()
+        rp = np.zeros(r + 1, np.int32)
+        ci = np.zeros(z, np.int32)
+        v = np.zeros(z, np.float64)
+        lib().orc_csr_export(self.h, _ip(rp), _ip(ci), _dp(v))
+        return rp, ci, v
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.rows, np.float64)
+        lib().orc_spmv(self.h, _dp(x), _dp(y))
+        return y
+
+    def transpose(self):
+        return Csr(lib().orc_transpose(self.h))
+
+    def matmat(self, other):
+        return Csr(lib().orc_matmat(self.h, other.h))
+
+
+def uniform(seed, n):
+    out = np.zeros(n, np.float64)
+    lib().orc_uniform(seed, n, _dp(out))
+    return out
+
+
+SOLVERS = {"cg": 0, "fgmres": 1}
+PRECONDS = {"none": 0, "jacobi": 1, "ilu": 2, "amg": 3}
+
+
+def solve(A, b, solver="cg", precond="jacobi", x0=None, **kw):
+    p = params(**kw)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
+    st = Stats()
+    rc = lib().orc_solve(A.h, _dp(b), _dp(x), SOLVERS[solver], PRECONDS[precond], C.byref(p), C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return x, st
+
+
+def precond_apply(A, r, precond, **kw):
+    p = params(**kw)
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.zeros_like(r)
+    if lib().orc_precond_apply(A.h, PRECONDS[precond], C.byref(p), _dp(r), _dp(z)) < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return z
+
+
+def adaptive(A, b, advisor=0, **kw):
+    p = params(**kw)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b)
+    st = Stats()
+    rc = lib().orc_adaptive(A.h, _dp(b), _dp(x), C.byref(p), advisor, C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return x, st
+
+
+def features(A):
+    out = np.zeros(8, np.float64)
+    if lib().orc_features(A.h, _dp(out)) < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    keys = ["n", "nnz", "dd", "sym", "fro", "cond", "lmax", "spd"]
+    return dict(zip(keys, out.tolist()))
+
+
+class Amg:
+    def __init__(self, A, **kw):
+        p = params(**kw)
+        self.h = lib().orc_amg_setup(A.h, C.byref(p))
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_amg_free(self.h)
+            self.h = None
+
+    def levels(self):
+        return lib().orc_amg_levels(self.h)
+
+    def matrix(self, lv, which="A"):
+        return Csr(lib().orc_amg_level_matrix(self.h, lv, {"A": 0, "P": 1, "R": 2}[which]))
+
+    def aggregates(self, lv):
+        n = self.matrix(lv, "A").rows
+        out = np.zeros(n, np.int32)
+        lib().orc_amg_aggregates(self.h, lv, _ip(out))
+        return out
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        lib().orc_amg_apply(self.h, _dp(r), _dp(z))
+        return z
+
+
+def aggregate(A, theta):
+    out = np.zeros(A.rows, np.int32)
+    nagg = lib().orc_aggregate(A.h, theta, _ip(out))
+    return out, nagg
+
+
+def gepp_solve(A, rhs):
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    x = np.zeros_like(rhs)
+    lib().orc_gepp_solve(A.h, _dp(rhs), _dp(x))
+    return x
+
+
+def ilu_factor(A, drop_tol=0.0, fill=0):
+    return Csr(lib().orc_ilu_factor(A.h, drop_tol, fill))
+
+
+def read_mtx(path):
+    """MatrixMarket coordinate reader with symmetric expansion, the way
+    matrix_market_io.cpp:115-200 builds its COO before from_coo."""
+    with open(path) as f:
+        banner = f.readline().lower().split()
+        sym = banner[4] if len(banner) > 4 else "general"
+        line = f.readline()
+        while line.startswith("%") or not line.strip():
+            line = f.readline()
+        rows, cols, nz = (int(t) for t in line.split()[:3])
+        data = np.loadtxt(f, ndmin=2)
+    r = data[:, 0].astype(np.int64) - 1
+    c = data[:, 1].astype(np.int64) - 1
+    v = data[:, 2] if data.shape[1] > 2 else np.ones(len(r))
+    if sym in ("symmetric", "skew-symmetric"):
+        off = r != c
+        sgn = -1.0 if sym == "skew-symmetric" else 1.0
+        r, c, v = (np.concatenate([r, c[off]]), np.concatenate([c, r[off]]),
+                   np.concatenate([v, sgn * v[off]]))
+    return rows, cols, r.astype(np.int32), c.astype(np.int32), v
+
+
+def read_mtx_vector(path):
+    with open(path) as f:
+        f.readline()
+        line = f.readline()
+        while line.startswith("%") or not line.strip():
+            line = f.readline()
+        n = int(line.split()[0])
+        vals = np.loadtxt(f, ndmin=1)
+    return np.asarray(vals[:n], np.float64)
