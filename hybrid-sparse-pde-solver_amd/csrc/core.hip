@@ -1,0 +1,518 @@
+// =============================================================================
+// core.hip — context, device memory, CSR upload + row-block schedule, SpMV and
+// BLAS-1 entry points of the C ABI.
+//
+// Reference counterparts: SparseMatrix (sparse_matrix.hpp:34-170,
+// sparse_matrix.cpp:68-87), Vector BLAS-1 (vector.cpp:39-119), and the role of
+// the CUDA backend's DeviceSparseMatrix / cusparseSpMV / cublasD*
+// (cuda_backend.cpp:26-431) — here with everything resident on the device.
+// =============================================================================
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device_common.hpp"
+
+namespace hsps_dev {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ---- DevBuf ------------------------------------------------------------------
+DevBuf::DevBuf(Ctx* c, size_t b) : ctx(c), bytes(b) {
+    if (b == 0) return;
+    c->set_current();
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        throw Error(HSPS_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e));
+    }
+    c->bytes_in_use += b;
+}
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+        reset();
+        ctx = o.ctx; p = o.p; bytes = o.bytes;
+        o.p = nullptr; o.bytes = 0;
+    }
+    return *this;
+}
+void DevBuf::reset() {
+    if (p) {
+        (void)hipFree(p);
+        if (ctx) ctx->bytes_in_use -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+}
+DevBuf::~DevBuf() { reset(); }
+
+// ---- row-block schedule ------------------------------------------------------
+std::vector<int> make_row_blocks(const int* rp, int rows, int max_rows, int max_nnz) {
+    std::vector<int> b;
+    b.reserve(rows / 64 + 2);
+    b.push_back(0);
+    int start = 0;
+    while (start < rows) {
+        int end = start;
+        const int base = rp[start];
+        if (rp[start + 1] - base > max_nnz) {
+            end = start + 1;  // long row: own block
+        } else {
+            while (end < rows && end - start < max_rows && rp[end + 1] - base <= max_nnz) ++end;
+        }
+        b.push_back(end);
+        start = end;
+    }
+    return b;
+}
+
+void Csr::build_schedule() {
+    std::vector<int> b = make_row_blocks(h_rp.data(), rows, kMaxBlockRows, kLdsNnz);
+    n_blocks = (int)b.size() - 1;
+    blocks = DevBuf(ctx, b.size() * sizeof(int));
+    HSPS_HIP(hipMemcpy(blocks.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+__global__ void k_f64_to_f32(int n, const double* __restrict__ x, float* __restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = (float)x[i];
+}
+
+void Csr::upload_values() {
+    ctx->set_current();
+    if (nnz > 0) {
+        HSPS_HIP(hipMemcpy(v64.p, h_v.data(), (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
+        if (v32.p) {
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0, nullptr, nnz, v64.as<double>(),
+                               v32.as<float>());
+            HSPS_CHECK_LAUNCH();
+        }
+    }
+    std::vector<double> d(rows, 0.0);
+    for (int i = 0; i < rows; ++i)
+        for (int k = h_rp[i]; k < h_rp[i + 1]; ++k)
+            if (h_ci[k] == i) { d[i] = h_v[k]; break; }
+    if (rows > 0) HSPS_HIP(hipMemcpy(diag64.p, d.data(), (size_t)rows * sizeof(double), hipMemcpyHostToDevice));
+    HSPS_HIP(hipDeviceSynchronize());
+}
+
+static void validate_csr(int rows, int cols, int nnz, const int* rp, const int* ci) {
+    if (rows < 0 || cols < 0 || nnz < 0) throw Error(HSPS_ERR_INVALID, "csr: negative dimension");
+    if (rp[0] != 0 || rp[rows] != nnz) throw Error(HSPS_ERR_INVALID, "csr: row_ptr[0] must be 0 and row_ptr[rows] == nnz");
+    for (int i = 0; i < rows; ++i)
+        if (rp[i + 1] < rp[i]) throw Error(HSPS_ERR_INVALID, "csr: row_ptr not monotone at row " + std::to_string(i));
+    for (int k = 0; k < nnz; ++k)
+        if (ci[k] < 0 || ci[k] >= cols) throw Error(HSPS_ERR_INVALID, "csr: column index out of range at entry " + std::to_string(k));
+}
+
+std::unique_ptr<Csr> make_csr(Ctx* ctx, int rows, int cols, std::vector<int> rp, std::vector<int> ci,
+                              std::vector<double> v, bool want_f32) {
+    validate_csr(rows, cols, (int)v.size(), rp.data(), ci.data());
+    auto A = std::make_unique<Csr>();
+    A->ctx = ctx;
+    A->rows = rows;
+    A->cols = cols;
+    A->nnz = (int)v.size();
+    A->h_rp = std::move(rp);
+    A->h_ci = std::move(ci);
+    A->h_v = std::move(v);
+    A->rp = DevBuf(ctx, (size_t)(rows + 1) * sizeof(int));
+    A->ci = DevBuf(ctx, (size_t)A->nnz * sizeof(int));
+    A->v64 = DevBuf(ctx, (size_t)A->nnz * sizeof(double));
+    if (want_f32) A->v32 = DevBuf(ctx, (size_t)A->nnz * sizeof(float));
+    A->diag64 = DevBuf(ctx, (size_t)rows * sizeof(double));
+    ctx->set_current();
+    HSPS_HIP(hipMemcpy(A->rp.p, A->h_rp.data(), (size_t)(rows + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (A->nnz) HSPS_HIP(hipMemcpy(A->ci.p, A->h_ci.data(), (size_t)A->nnz * sizeof(int), hipMemcpyHostToDevice));
+    A->upload_values();
+    A->build_schedule();
+    return A;
+}
+
+// ---- SpMV launchers -------------------------------------------------------------
+void spmv_f64(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip) {
+    EpiStore<double> e;
+    e.flag = skip;
+    e.y = y;
+    launch_spmv(A, A.v64.as<double>(), x, e, s);
+}
+void spmv_f32(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip) {
+    EpiStore<float> e;
+    e.flag = skip;
+    e.y = y;
+    launch_spmv(A, A.v32.as<float>(), x, e, s);
+}
+void spmv_add_f64(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip) {
+    EpiAdd<double> e;
+    e.flag = skip;
+    e.y = y;
+    launch_spmv(A, A.v64.as<double>(), x, e, s);
+}
+void spmv_add_f32(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip) {
+    EpiAdd<float> e;
+    e.flag = skip;
+    e.y = y;
+    launch_spmv(A, A.v32.as<float>(), x, e, s);
+}
+void residual_f64(const Csr& A, const double* b, const double* x, double* r, hipStream_t s, double* d_nrm2,
+                  double* partials, unsigned* counter, const int* skip) {
+    if (d_nrm2) {
+        EpiResidual<double, true> e;
+        e.flag = skip; e.b = b; e.r = r; e.out = d_nrm2; e.partials = partials; e.counter = counter;
+        launch_spmv(A, A.v64.as<double>(), x, e, s);
+    } else {
+        EpiResidual<double, false> e;
+        e.flag = skip; e.b = b; e.r = r;
+        launch_spmv(A, A.v64.as<double>(), x, e, s);
+    }
+}
+void residual_f32(const Csr& A, const float* b, const float* x, float* r, hipStream_t s, const int* skip) {
+    EpiResidual<float, false> e;
+    e.flag = skip; e.b = b; e.r = r;
+    launch_spmv(A, A.v32.as<float>(), x, e, s);
+}
+void smooth_f64(const Csr& A, const double* dinv, double omega, const double* rhs, const double* xi, double* xo,
+                hipStream_t s, const int* skip) {
+    EpiSmooth<double> e;
+    e.flag = skip; e.dinv = dinv; e.rhs = rhs; e.xi = xi; e.xo = xo; e.omega = omega;
+    launch_spmv(A, A.v64.as<double>(), xi, e, s);
+}
+void smooth_f32(const Csr& A, const float* dinv, float omega, const float* rhs, const float* xi, float* xo,
+                hipStream_t s, const int* skip) {
+    EpiSmooth<float> e;
+    e.flag = skip; e.dinv = dinv; e.rhs = rhs; e.xi = xi; e.xo = xo; e.omega = omega;
+    launch_spmv(A, A.v32.as<float>(), xi, e, s);
+}
+
+// ---- BLAS-1 kernels ------------------------------------------------------------
+int reduction_blocks(int n) { return std::min(grid_for(n, 8), kMaxRedBlocks); }
+
+__global__ void __launch_bounds__(kBlock)
+k_dot(int n, const double* __restrict__ x, const double* __restrict__ y, double* out, double* partials,
+      unsigned* counter) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[0] += x[i] * y[i];
+    double tot[1];
+    if (grid_sum<1>(v, partials, counter, tot)) *out = tot[0];
+}
+void dot_f64_dev(int n, const double* x, const double* y, double* d_out, double* partials, unsigned* counter,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_dot, dim3(reduction_blocks(n)), dim3(kBlock), 0, s, n, x, y, d_out, partials, counter);
+    HSPS_CHECK_LAUNCH();
+}
+
+__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y, int mode) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (mode == 0) y[i] += a * x[i];          // axpy
+        else if (mode == 1) y[i] = a * x[i] + b * y[i];  // axpby
+        else y[i] *= a;                                    // scale
+    }
+}
+void axpy_f64(int n, double a, const double* x, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, 0.0, y, 0);
+    HSPS_CHECK_LAUNCH();
+}
+void axpby_f64(int n, double a, const double* x, double b, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, b, y, 1);
+    HSPS_CHECK_LAUNCH();
+}
+void scale_f64(int n, double a, double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, nullptr, 0.0, x, 2);
+    HSPS_CHECK_LAUNCH();
+}
+
+__global__ void k_cvt(int n, const double* __restrict__ x, float* __restrict__ y, const int* skip) {
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = (float)x[i];
+}
+void copy_f64_to_f32(int n, const double* x, float* y, hipStream_t s, const int* skip) {
+    hipLaunchKernelGGL(k_cvt, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, y, skip);
+    HSPS_CHECK_LAUNCH();
+}
+template <class T>
+__global__ void k_zero(int n, T* __restrict__ x, const int* skip) {
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = T(0);
+}
+void fill_zero_f64(int n, double* x, hipStream_t s, const int* skip) {
+    hipLaunchKernelGGL(k_zero<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, skip);
+    HSPS_CHECK_LAUNCH();
+}
+void fill_zero_f32(int n, float* x, hipStream_t s, const int* skip) {
+    hipLaunchKernelGGL(k_zero<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, skip);
+    HSPS_CHECK_LAUNCH();
+}
+
+}  // namespace hsps_dev
+
+using namespace hsps_dev;
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+const char* hsps_last_error(void) { return g_last_error.c_str(); }
+
+void hsps_params_default(hsps_params* p) {
+    if (!p) return;
+    p->tol = 1e-8;
+    p->max_iter = 2000;
+    p->restart_size = 50;
+    p->drop_tol = 1e-4;
+    p->fill_per_row = 0;
+    p->amg_levels = 10;
+    p->amg_strong = 0.25;
+    p->amg_smooth_pre = 2;
+    p->amg_smooth_post = 2;
+    p->amg_omega = 0.67;
+    p->jacobi_omega = 1.0;
+    p->precision = HSPS_FP64;
+    p->reliable_delta = 0.1;
+    p->check_every = 32;
+}
+
+int hsps_ctx_create(int device, hsps_ctx* out) {
+    return guarded([&] {
+        if (!out) throw Error(HSPS_ERR_INVALID, "hsps_ctx_create: out is NULL");
+        int n = 0;
+        HSPS_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n)
+            throw Error(HSPS_ERR_INVALID, "hsps_ctx_create: device " + std::to_string(device) + " not present (" +
+                                              std::to_string(n) + " devices)");
+        auto* c = new hsps_ctx_s;
+        c->device = device;
+        try {
+            HSPS_HIP(hipSetDevice(device));
+            HSPS_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+            c->stream = c->own_stream;
+            c->red_partials = DevBuf(c, sizeof(double) * kMaxRedBlocks * 4);
+            c->red_counter = DevBuf(c, 256);
+            c->red_out = DevBuf(c, 256);
+            HSPS_HIP(hipMemset(c->red_counter.p, 0, 256));
+            HSPS_HIP(hipHostMalloc((void**)&c->pinned, 64 * sizeof(double), hipHostMallocDefault));
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+        return HSPS_OK;
+    });
+}
+
+int hsps_ctx_destroy(hsps_ctx ctx) {
+    return guarded([&] {
+        if (!ctx) return HSPS_OK;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->red_partials.reset();
+        ctx->red_counter.reset();
+        ctx->red_out.reset();
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return HSPS_OK;
+    });
+}
+
+int hsps_ctx_set_stream(hsps_ctx ctx, void* stream) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+        return HSPS_OK;
+    });
+}
+void* hsps_ctx_stream(hsps_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+int hsps_ctx_sync(hsps_ctx ctx) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        HSPS_HIP(hipStreamSynchronize(ctx->stream));
+        return HSPS_OK;
+    });
+}
+size_t hsps_ctx_bytes_in_use(hsps_ctx ctx) { return ctx ? ctx->bytes_in_use : 0; }
+
+int hsps_malloc(hsps_ctx ctx, size_t bytes, void** d_ptr) {
+    return guarded([&] {
+        if (!ctx || !d_ptr) throw Error(HSPS_ERR_INVALID, "hsps_malloc: null argument");
+        ctx->set_current();
+        *d_ptr = nullptr;
+        if (bytes == 0) return HSPS_OK;
+        hipError_t e = hipMalloc(d_ptr, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(HSPS_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        return HSPS_OK;
+    });
+}
+int hsps_free(hsps_ctx ctx, void* d_ptr) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        if (d_ptr) {
+            ctx->set_current();
+            HSPS_HIP(hipStreamSynchronize(ctx->stream));
+            HSPS_HIP(hipFree(d_ptr));
+        }
+        return HSPS_OK;
+    });
+}
+int hsps_memcpy_h2d(hsps_ctx ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        if (bytes == 0) return HSPS_OK;
+        ctx->set_current();
+        HSPS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HSPS_HIP(hipStreamSynchronize(ctx->stream));
+        return HSPS_OK;
+    });
+}
+int hsps_memcpy_d2h(hsps_ctx ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        if (bytes == 0) return HSPS_OK;
+        ctx->set_current();
+        HSPS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HSPS_HIP(hipStreamSynchronize(ctx->stream));
+        return HSPS_OK;
+    });
+}
+int hsps_memcpy_d2d(hsps_ctx ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        if (bytes == 0) return HSPS_OK;
+        ctx->set_current();
+        HSPS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return HSPS_OK;
+    });
+}
+int hsps_memset(hsps_ctx ctx, void* dst, int value, size_t bytes) {
+    return guarded([&] {
+        if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        if (bytes == 0) return HSPS_OK;
+        ctx->set_current();
+        HSPS_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        return HSPS_OK;
+    });
+}
+
+int hsps_csr_create(hsps_ctx ctx, int rows, int cols, int nnz, const int* rp, const int* ci, const double* v,
+                    hsps_csr* out) {
+    return guarded([&] {
+        if (!ctx || !out || !rp || (nnz > 0 && (!ci || !v)))
+            throw Error(HSPS_ERR_INVALID, "hsps_csr_create: null argument");
+        validate_csr(rows, cols, nnz, rp, ci);
+        auto A = make_csr(ctx, rows, cols, std::vector<int>(rp, rp + rows + 1), std::vector<int>(ci, ci + nnz),
+                          std::vector<double>(v, v + nnz));
+        auto* h = new hsps_csr_s;
+        static_cast<Csr&>(*h) = std::move(*A);
+        *out = h;
+        return HSPS_OK;
+    });
+}
+
+int hsps_csr_destroy(hsps_csr A) {
+    return guarded([&] {
+        if (A) {
+            (void)hipSetDevice(A->ctx->device);
+            (void)hipStreamSynchronize(A->ctx->stream);
+            delete A;
+        }
+        return HSPS_OK;
+    });
+}
+
+int hsps_csr_info(hsps_csr A, int* rows, int* cols, int* nnz) {
+    return guarded([&] {
+        if (!A) throw Error(HSPS_ERR_INVALID, "null matrix");
+        if (rows) *rows = A->rows;
+        if (cols) *cols = A->cols;
+        if (nnz) *nnz = A->nnz;
+        return HSPS_OK;
+    });
+}
+
+int hsps_csr_update_values(hsps_csr A, const double* v) {
+    return guarded([&] {
+        if (!A || (!v && A->nnz)) throw Error(HSPS_ERR_INVALID, "null argument");
+        HSPS_HIP(hipStreamSynchronize(A->ctx->stream));
+        std::copy(v, v + A->nnz, A->h_v.begin());
+        A->upload_values();
+        return HSPS_OK;
+    });
+}
+
+int hsps_spmv_f64(hsps_csr A, const double* x, double* y) {
+    return guarded([&] {
+        if (!A || (A->rows && !y) || (A->cols && !x)) throw Error(HSPS_ERR_INVALID, "hsps_spmv_f64: null argument");
+        A->ctx->set_current();
+        spmv_f64(*A, x, y, A->ctx->stream);
+        return HSPS_OK;
+    });
+}
+int hsps_spmv_f32(hsps_csr A, const float* x, float* y) {
+    return guarded([&] {
+        if (!A || (A->rows && !y) || (A->cols && !x)) throw Error(HSPS_ERR_INVALID, "hsps_spmv_f32: null argument");
+        if (!A->v32.p && A->nnz) throw Error(HSPS_ERR_UNSUPPORTED, "matrix has no FP32 copy");
+        A->ctx->set_current();
+        spmv_f32(*A, x, y, A->ctx->stream);
+        return HSPS_OK;
+    });
+}
+int hsps_residual_f64(hsps_csr A, const double* b, const double* x, double* r) {
+    return guarded([&] {
+        if (!A || !b || !x || !r) throw Error(HSPS_ERR_INVALID, "hsps_residual_f64: null argument");
+        if (A->rows != A->cols) throw Error(HSPS_ERR_SIZE, "residual needs a square matrix");
+        A->ctx->set_current();
+        residual_f64(*A, b, x, r, A->ctx->stream);
+        return HSPS_OK;
+    });
+}
+
+static double ctx_dot(hsps_ctx ctx, int n, const double* x, const double* y) {
+    ctx->set_current();
+    dot_f64_dev(n, x, y, ctx->red_out.as<double>(), ctx->red_partials.as<double>(), ctx->red_counter.as<unsigned>(),
+                ctx->stream);
+    HSPS_HIP(hipMemcpyAsync(ctx->pinned, ctx->red_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HSPS_HIP(hipStreamSynchronize(ctx->stream));
+    return ctx->pinned[0];
+}
+
+int hsps_dot_f64(hsps_ctx ctx, int n, const double* x, const double* y, double* out) {
+    return guarded([&] {
+        if (!ctx || !out || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_dot_f64: bad argument");
+        *out = n == 0 ? 0.0 : ctx_dot(ctx, n, x, y);
+        return HSPS_OK;
+    });
+}
+int hsps_norm2_f64(hsps_ctx ctx, int n, const double* x, double* out) {
+    return guarded([&] {
+        if (!ctx || !out || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_norm2_f64: bad argument");
+        *out = n == 0 ? 0.0 : std::sqrt(ctx_dot(ctx, n, x, x));
+        return HSPS_OK;
+    });
+}
+int hsps_axpy_f64(hsps_ctx ctx, int n, double a, const double* x, double* y) {
+    return guarded([&] {
+        if (!ctx || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_axpy_f64: bad argument");
+        if (n) { ctx->set_current(); axpy_f64(n, a, x, y, ctx->stream); }
+        return HSPS_OK;
+    });
+}
+int hsps_axpby_f64(hsps_ctx ctx, int n, double a, const double* x, double b, double* y) {
+    return guarded([&] {
+        if (!ctx || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_axpby_f64: bad argument");
+        if (n) { ctx->set_current(); axpby_f64(n, a, x, b, y, ctx->stream); }
+        return HSPS_OK;
+    });
+}
+int hsps_scale_f64(hsps_ctx ctx, int n, double a, double* x) {
+    return guarded([&] {
+        if (!ctx || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_scale_f64: bad argument");
+        if (n) { ctx->set_current(); scale_f64(n, a, x, ctx->stream); }
+        return HSPS_OK;
+    });
+}
+
+}  // extern "C"

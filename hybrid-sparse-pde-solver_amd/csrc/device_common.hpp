@@ -1,0 +1,220 @@
+// =============================================================================
+// device_common.hpp — gfx950 device building blocks: wave64 / workgroup /
+// grid reductions with a deterministic last-workgroup finish, and the CSR
+// row-block SpMV template with pluggable epilogues.
+// =============================================================================
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hsps_internal.hpp"
+
+namespace hsps_dev {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum NV values over the workgroup; the result is valid in thread 0.
+// `lds` must hold NV * (kBlock/64) doubles.  Ends with a barrier-free result
+// in thread 0; callers that reuse `lds` must barrier first.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
+    constexpr int W = kBlock / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k * W + wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) s += lds[k * W + w];
+            v[k] = s;
+        }
+    }
+}
+
+// Grid-wide sum of NV doubles.  Every workgroup writes its partial; the last
+// one to arrive (agent-scope acq_rel ticket) sums all partials in a fixed
+// order — deterministic for a given grid — and returns true in its thread 0
+// with the totals in tot[].  The ticket counter is reset for the next use.
+// partials: NV * gridDim.x doubles; counter: one zeroed unsigned.
+template <int NV>
+__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter,
+                                         double (&tot)[NV]) {
+    __shared__ double red[NV * (kBlock / 64)];
+    __shared__ int is_last;
+    block_sum<NV>(v, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
+        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] += __hip_atomic_load(&partials[k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();  // red[] reuse
+    block_sum<NV>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) tot[k] = acc[k];
+        *counter = 0u;
+        return true;
+    }
+    return false;
+}
+
+template <class T> struct AccOf;
+template <> struct AccOf<double> { using type = double; };
+template <> struct AccOf<float> { using type = float; };
+
+// CSR row-block SpMV.  Workgroup b owns rows [blocks[b], blocks[b+1]).
+//  * staged path (block nnz <= kLdsNnz): the workgroup streams val/col
+//    contiguously, writes val*x[col] products to LDS, then sums each row from
+//    LDS with tpr lanes per row (tpr = pow2 <= 256/rows, <= 64).
+//  * long-row path (one row with nnz > kLdsNnz): the whole workgroup strides
+//    the row and reduces.
+// Epi supplies: skip(), row(row, sum, red[]), NV, finish(tot[]) and the
+// reduction buffers partials/counter when NV > 0.
+template <typename V, typename T, class Epi>
+__global__ void __launch_bounds__(kBlock)
+k_spmv(const int* __restrict__ blocks, const int* __restrict__ rp, const int* __restrict__ ci,
+       const V* __restrict__ val, const T* __restrict__ x, Epi epi) {
+    using Acc = typename AccOf<T>::type;
+    if (epi.skip()) return;
+    __shared__ Acc prod[kLdsNnz];
+    const int tid = threadIdx.x;
+    const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    double red[Epi::NV > 0 ? Epi::NV : 1];
+#pragma unroll
+    for (int k = 0; k < (Epi::NV > 0 ? Epi::NV : 1); ++k) red[k] = 0.0;
+    if (k1 - k0 > kLdsNnz) {
+        Acc s = 0;
+        for (int k = k0 + tid; k < k1; k += kBlock) s += (Acc)val[k] * (Acc)x[ci[k]];
+        double v1[1] = {(double)s};
+        __shared__ double lred[kBlock / 64];
+        block_sum<1>(v1, lred);
+        if (tid == 0) epi.row(r0, (Acc)v1[0], red);
+    } else {
+        for (int k = k0 + tid; k < k1; k += kBlock) prod[k - k0] = (Acc)val[k] * (Acc)x[ci[k]];
+        __syncthreads();
+        const int nrows = r1 - r0;
+        int tpr = 1;
+        while (tpr < 64 && tpr * 2 * nrows <= kBlock) tpr *= 2;
+        const int g = tid / tpr, l = tid - g * tpr, groups = kBlock / tpr;
+        for (int row = r0 + g; row < r1; row += groups) {
+            const int a = rp[row] - k0, b = rp[row + 1] - k0;
+            Acc s = 0;
+            for (int k = a + l; k < b; k += tpr) s += prod[k];
+            for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (l == 0) epi.row(row, s, red);
+        }
+    }
+    if constexpr (Epi::NV > 0) {
+        double tot[Epi::NV];
+        if (grid_sum<Epi::NV>(red, epi.partials, epi.counter, tot)) epi.finish(tot);
+    }
+}
+
+// ---- basic epilogues ------------------------------------------------------------
+struct SkipFlag {
+    const int* flag = nullptr;
+    __device__ bool skip() const { return flag && *flag; }
+};
+
+template <typename T>
+struct EpiStore : SkipFlag {
+    static constexpr int NV = 0;
+    T* y;
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    template <class A> __device__ void row(int i, A s, double*) const { y[i] = (T)s; }
+    __device__ void finish(const double*) const {}
+};
+
+template <typename T>
+struct EpiAdd : SkipFlag {
+    static constexpr int NV = 0;
+    T* y;
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    template <class A> __device__ void row(int i, A s, double*) const { y[i] = (T)((A)y[i] + s); }
+    __device__ void finish(const double*) const {}
+};
+
+// r = b - A x; optional ||r||^2 -> *out
+template <typename T, bool NORM>
+struct EpiResidual : SkipFlag {
+    static constexpr int NV = NORM ? 1 : 0;
+    const T* b;
+    T* r;
+    double* out = nullptr;
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    template <class A> __device__ void row(int i, A s, double* red) const {
+        const T v = (T)((A)b[i] - s);
+        r[i] = v;
+        if constexpr (NORM) red[0] += (double)v * (double)v;
+    }
+    __device__ void finish(const double* tot) const {
+        if constexpr (NORM) *out = tot[0];
+    }
+};
+
+// xo = xi + omega * dinv * (rhs - A xi)   (amg.cpp:242-254, one sweep)
+template <typename T>
+struct EpiSmooth : SkipFlag {
+    static constexpr int NV = 0;
+    const T* dinv;
+    const T* rhs;
+    const T* xi;
+    T* xo;
+    T omega;
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    template <class A> __device__ void row(int i, A s, double*) const {
+        xo[i] = (T)((A)xi[i] + (A)omega * (A)dinv[i] * ((A)rhs[i] - s));
+    }
+    __device__ void finish(const double*) const {}
+};
+
+template <typename V, typename T, class Epi>
+inline void launch_spmv(const Csr& A, const V* val, const T* x, const Epi& epi, hipStream_t s) {
+    if (A.n_blocks == 0) return;
+    hipLaunchKernelGGL((k_spmv<V, T, Epi>), dim3(A.n_blocks), dim3(kBlock), 0, s, A.blocks.as<int>(),
+                       A.rp.as<int>(), A.ci.as<int>(), val, x, epi);
+    HSPS_CHECK_LAUNCH();
+}
+
+// Grid size for element-wise / reduction kernels over n entries.
+inline int grid_for(int n, int per_thread = 4) {
+    long g = ((long)n + (long)kBlock * per_thread - 1) / ((long)kBlock * per_thread);
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return (int)g;
+}
+
+}  // namespace hsps_dev

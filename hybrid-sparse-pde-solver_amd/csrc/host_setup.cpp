@@ -1,0 +1,349 @@
+// =============================================================================
+// host_setup.cpp — host-side setup stages (see host_setup.hpp).
+// =============================================================================
+#include "host_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+namespace hsps_host {
+
+static constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+std::vector<int> aggregate(const HostCsr& A, double theta, int* n_coarse) {
+    const int n = A.rows;
+    std::vector<double> row_max(n, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double m = 0.0;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] != i) m = std::max(m, std::abs(A.v[k]));
+        row_max[i] = m;
+    }
+    // Sequential greedy sweep in row order: the decisions depend on it.
+    std::vector<int> agg(n, -1);
+    int nagg = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        agg[i] = nagg;
+        const double thr = theta * row_max[i];
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int j = A.ci[k];
+            if (j != i && agg[j] < 0 && std::abs(A.v[k]) >= thr) agg[j] = nagg;
+        }
+        ++nagg;
+    }
+    if (n_coarse) *n_coarse = nagg;
+    return agg;
+}
+
+std::vector<double> inv_diagonal(const HostCsr& A) {
+    std::vector<double> d(A.rows, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < A.rows; ++i) {
+        double a = 0.0;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] == i) { a = A.v[k]; break; }
+        d[i] = std::abs(a) > kEps ? 1.0 / a : 0.0;
+    }
+    return d;
+}
+
+HostCsr smoothed_prolongation(const HostCsr& A, const std::vector<int>& agg, int nc, double omega) {
+    const int n = A.rows;
+    // omega / a_ii, 0 when |a_ii| <= eps (amg.cpp:120-127)
+    std::vector<double> w(n, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] == i) { w[i] = std::abs(A.v[k]) > kEps ? omega / A.v[k] : 0.0; break; }
+    // Row i touches at most (row nnz) distinct aggregates: a tiny sorted list.
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+        auto& row = rows[i];
+        row.emplace_back(agg[i], 1.0);
+        const double wi = w[i];
+        if (!(std::abs(wi) < kEps)) {
+            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const int j = A.ci[k];
+                if (j == i) continue;
+                const int c = agg[j];
+                const double t = wi * A.v[k];
+                size_t q = 0;
+                while (q < row.size() && row[q].first != c) ++q;
+                if (q == row.size()) row.emplace_back(c, 0.0);
+                row[q].second -= t;
+            }
+        }
+        std::sort(row.begin(), row.end());
+        row.erase(std::remove_if(row.begin(), row.end(), [](const auto& e) { return !(std::abs(e.second) > 1e-14); }),
+                  row.end());
+    }
+    HostCsr P;
+    P.rows = n;
+    P.cols = nc;
+    P.rp.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) P.rp[i + 1] = P.rp[i] + (int)rows[i].size();
+    P.ci.resize(P.rp[n]);
+    P.v.resize(P.rp[n]);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        int p = P.rp[i];
+        for (auto& e : rows[i]) { P.ci[p] = e.first; P.v[p] = e.second; ++p; }
+    }
+    return P;
+}
+
+HostCsr transpose(const HostCsr& A) {
+    HostCsr T;
+    T.rows = A.cols;
+    T.cols = A.rows;
+    T.rp.assign(A.cols + 1, 0);
+    for (int k = 0; k < A.nnz(); ++k) T.rp[A.ci[k] + 1]++;
+    for (int j = 0; j < A.cols; ++j) T.rp[j + 1] += T.rp[j];
+    T.ci.resize(A.nnz());
+    T.v.resize(A.nnz());
+    std::vector<int> pos(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.rows; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int p = pos[A.ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[k];
+        }
+    return T;
+}
+
+HostCsr spgemm(const HostCsr& A, const HostCsr& B) {
+    if (A.cols != B.rows) throw std::runtime_error("mat_mat_product: inner dimensions mismatch");
+    const int m = A.rows, n = B.cols;
+    std::vector<int> cnt(m + 1, 0);
+    std::vector<std::vector<int>> cols(m);
+    std::vector<std::vector<double>> vals(m);
+#pragma omp parallel
+    {
+        std::vector<double> acc(n, 0.0);
+        std::vector<char> seen(n, 0);
+        std::vector<int> touched;
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < m; ++i) {
+            for (int pa = A.rp[i]; pa < A.rp[i + 1]; ++pa) {
+                const int j = A.ci[pa];
+                const double av = A.v[pa];
+                for (int pb = B.rp[j]; pb < B.rp[j + 1]; ++pb) {
+                    const int l = B.ci[pb];
+                    if (!seen[l]) { seen[l] = 1; touched.push_back(l); }
+                    acc[l] = std::fma(av, B.v[pb], acc[l]);
+                }
+            }
+            std::sort(touched.begin(), touched.end());
+            auto& cc = cols[i];
+            auto& vv = vals[i];
+            cc.reserve(touched.size());
+            vv.reserve(touched.size());
+            for (int l : touched) {
+                if (std::abs(acc[l]) > kEps * kEps) { cc.push_back(l); vv.push_back(acc[l]); }
+                acc[l] = 0.0;
+                seen[l] = 0;
+            }
+            touched.clear();
+        }
+    }
+    HostCsr C;
+    C.rows = m;
+    C.cols = n;
+    C.rp.assign(m + 1, 0);
+    for (int i = 0; i < m; ++i) C.rp[i + 1] = C.rp[i] + (int)cols[i].size();
+    C.ci.resize(C.rp[m]);
+    C.v.resize(C.rp[m]);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < m; ++i) {
+        std::copy(cols[i].begin(), cols[i].end(), C.ci.begin() + C.rp[i]);
+        std::copy(vals[i].begin(), vals[i].end(), C.v.begin() + C.rp[i]);
+    }
+    return C;
+}
+
+HostCsr ilu0(const HostCsr& A) {
+    HostCsr F = A;
+    const int n = A.rows;
+    std::vector<int> diag(n, -1), pos(n, -1);
+    for (int i = 0; i < n; ++i)
+        for (int k = F.rp[i]; k < F.rp[i + 1]; ++k)
+            if (F.ci[k] == i) { diag[i] = k; break; }
+    for (int i = 1; i < n; ++i) {
+        for (int p = F.rp[i]; p < F.rp[i + 1]; ++p) pos[F.ci[p]] = p;
+        for (int p = F.rp[i]; p < F.rp[i + 1]; ++p) {
+            const int k = F.ci[p];
+            if (k >= i) break;
+            const double dk = diag[k] >= 0 ? F.v[diag[k]] : 0.0;
+            if (std::abs(dk) < kEps) throw std::runtime_error("ILU(0): near-zero pivot at row " + std::to_string(k));
+            F.v[p] /= dk;
+            const double lik = F.v[p];
+            for (int q = F.rp[k]; q < F.rp[k + 1]; ++q) {
+                const int j = F.ci[q];
+                if (j <= k) continue;
+                const int pij = pos[j];
+                if (pij >= 0) F.v[pij] -= lik * F.v[q];
+            }
+        }
+        for (int p = F.rp[i]; p < F.rp[i + 1]; ++p) pos[F.ci[p]] = -1;
+    }
+    return F;
+}
+
+HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
+    const int n = A.rows;
+    std::vector<std::vector<std::pair<int, double>>> L(n), U(n);
+    std::vector<double> w(n, 0.0), udiag(n, 0.0);
+    std::vector<char> act(n, 0);
+    for (int i = 0; i < n; ++i) {
+        double rn = 0.0;
+        for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+            w[A.ci[p]] = A.v[p];
+            act[A.ci[p]] = 1;
+            rn += A.v[p] * A.v[p];
+        }
+        const double tau = tau_rel * std::sqrt(rn);
+        // Pivot columns k < i in increasing order (ilu.cpp:121-146).
+        for (int k = 0; k < i; ++k) {
+            if (!act[k]) continue;
+            if (std::abs(w[k]) < tau) { w[k] = 0.0; act[k] = 0; continue; }
+            const double ukk = udiag[k];
+            if (std::abs(ukk) < kEps) { act[k] = 0; continue; }
+            const double lik = w[k] / ukk;
+            w[k] = lik;
+            for (const auto& e : U[k]) {
+                const int j = e.first;
+                if (j <= k) continue;
+                if (!act[j] && std::abs(lik * e.second) >= tau) act[j] = 1;
+                w[j] -= lik * e.second;
+            }
+        }
+        std::vector<std::pair<int, double>> Lr, Ur;
+        for (int j = 0; j < i; ++j) {
+            if (act[j] && std::abs(w[j]) >= tau) Lr.emplace_back(j, w[j]);
+            w[j] = 0.0;
+            act[j] = 0;
+        }
+        double d = w[i];
+        if (std::abs(d) < kEps) d = (d >= 0 ? 1.0 : -1.0) * kEps * 1e6;
+        Ur.emplace_back(i, d);
+        w[i] = 0.0;
+        act[i] = 0;
+        for (int j = i + 1; j < n; ++j) {
+            if (act[j] && std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
+            w[j] = 0.0;
+            act[j] = 0;
+        }
+        const int keep = A.rp[i + 1] - A.rp[i] + fill_p;
+        auto limit = [keep](std::vector<std::pair<int, double>>& row) {
+            if (keep <= 0 || (int)row.size() <= keep) return;
+            std::partial_sort(row.begin(), row.begin() + keep, row.end(),
+                              [](const auto& a, const auto& b) { return std::abs(a.second) > std::abs(b.second); });
+            row.resize(keep);
+            std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        };
+        limit(Lr);
+        limit(Ur);
+        udiag[i] = 0.0;
+        for (const auto& e : Ur)
+            if (e.first == i) { udiag[i] = e.second; break; }
+        L[i] = std::move(Lr);
+        U[i] = std::move(Ur);
+    }
+    HostCsr F;
+    F.rows = F.cols = n;
+    F.rp.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        for (const auto& e : L[i]) { F.ci.push_back(e.first); F.v.push_back(e.second); }
+        for (const auto& e : U[i]) { F.ci.push_back(e.first); F.v.push_back(e.second); }
+        F.rp[i + 1] = (int)F.ci.size();
+    }
+    return F;
+}
+
+std::vector<int> tri_levels(const HostCsr& F, bool lower, int* n_levels) {
+    const int n = F.rows;
+    std::vector<int> lev(n, 0);
+    int nl = 0;
+    if (lower) {
+        for (int i = 0; i < n; ++i) {
+            int l = 0;
+            for (int k = F.rp[i]; k < F.rp[i + 1]; ++k) {
+                const int j = F.ci[k];
+                if (j < i) l = std::max(l, lev[j] + 1);
+            }
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+    } else {
+        for (int i = n - 1; i >= 0; --i) {
+            int l = 0;
+            for (int k = F.rp[i]; k < F.rp[i + 1]; ++k) {
+                const int j = F.ci[k];
+                if (j > i) l = std::max(l, lev[j] + 1);
+            }
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+    }
+    if (n_levels) *n_levels = n ? nl : 0;
+    return lev;
+}
+
+std::vector<double> coarse_operator(const HostCsr& A) {
+    const int n = A.rows;
+    // Row-major dense copy; row swaps tracked so the same elimination can be
+    // replayed on each unit right-hand side.
+    std::vector<double> M((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) M[(size_t)i * n + A.ci[k]] = A.v[k];
+    std::vector<int> piv(n);
+    std::vector<char> skip(n, 0);
+    std::vector<double> factors((size_t)n * n, 0.0);  // factor[r][col] used in elimination
+    for (int col = 0; col < n; ++col) {
+        int p = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::abs(M[(size_t)r * n + col]) > std::abs(M[(size_t)p * n + col])) p = r;
+        piv[col] = p;
+        if (p != col)
+            for (int c = 0; c < n; ++c) std::swap(M[(size_t)col * n + c], M[(size_t)p * n + c]);
+        // Factors of earlier columns stay where they were recorded: the
+        // replay below interleaves swaps and eliminations in the same order.
+        if (std::abs(M[(size_t)col * n + col]) < 1e-14) { skip[col] = 1; continue; }
+        const double ip = 1.0 / M[(size_t)col * n + col];
+#pragma omp parallel for schedule(static)
+        for (int r = col + 1; r < n; ++r) {
+            const double f = M[(size_t)r * n + col] * ip;
+            factors[(size_t)r * n + col] = f;
+            for (int c = col; c < n; ++c) M[(size_t)r * n + c] -= f * M[(size_t)col * n + c];
+        }
+    }
+    // Replay on unit vectors: column e of G = solve(e_e).
+    std::vector<double> G((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int e = 0; e < n; ++e) {
+        std::vector<double> b(n, 0.0), x(n, 0.0);
+        b[e] = 1.0;
+        for (int col = 0; col < n; ++col) {
+            std::swap(b[col], b[piv[col]]);
+            if (skip[col]) continue;
+            for (int r = col + 1; r < n; ++r) b[r] -= factors[(size_t)r * n + col] * b[col];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double s = b[i];
+            for (int j = i + 1; j < n; ++j) s -= M[(size_t)i * n + j] * x[j];
+            const double d = M[(size_t)i * n + i];
+            x[i] = std::abs(d) > 1e-14 ? s / d : 0.0;
+        }
+        for (int i = 0; i < n; ++i) G[(size_t)i * n + e] = x[i];
+    }
+    return G;
+}
+
+}  // namespace hsps_host

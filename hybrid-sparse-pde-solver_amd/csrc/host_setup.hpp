@@ -1,0 +1,54 @@
+// =============================================================================
+// host_setup.hpp — host-side setup stages of the solve path (compiled by the
+// host compiler with OpenMP): SA-AMG aggregation / prolongation / Galerkin,
+// ILU(0)/ILUT factorisation, triangular level schedules and the coarsest-
+// level dense operator.  Decisions (aggregates, ILU patterns, pivots) follow
+// the reference exactly; see each function's citation.
+// =============================================================================
+#pragma once
+
+#include <vector>
+
+namespace hsps_host {
+
+struct HostCsr {
+    int rows = 0, cols = 0;
+    std::vector<int> rp{0}, ci;
+    std::vector<double> v;
+    int nnz() const { return (int)v.size(); }
+};
+
+// Greedy aggregation (amg.cpp:30-67): strength |a_ij| >= theta * max_{l!=i}|a_il|.
+std::vector<int> aggregate(const HostCsr& A, double theta, int* n_coarse);
+
+// Jacobi-smoothed tentative prolongation, one sweep, diagonal skipped,
+// |v| <= 1e-14 dropped (amg.cpp:73-178), built sparsely from the aggregate map.
+HostCsr smoothed_prolongation(const HostCsr& A, const std::vector<int>& agg, int n_coarse, double omega);
+
+// Transpose (sparse_matrix.cpp:138-165).
+HostCsr transpose(const HostCsr& A);
+
+// C = A*B, row-wise accumulation in (A-row, B-row) order with fused
+// multiply-add, ascending columns, |c| <= eps^2 dropped
+// (sparse_matrix.cpp:171-237).
+HostCsr spgemm(const HostCsr& A, const HostCsr& B);
+
+// 1/a_ii (0 where |a_ii| <= eps), amg.cpp:189-196 / :224-228.
+std::vector<double> inv_diagonal(const HostCsr& A);
+
+// ILU(0) on A's pattern (ilu.cpp:41-89) / ILUT(tau, p) (ilu.cpp:94-220).
+// Throws std::runtime_error on a near-zero pivot with the reference's message.
+HostCsr ilu0(const HostCsr& A);
+HostCsr ilut(const HostCsr& A, double tau, int fill_p);
+
+// Dependency levels of the unit-lower (lower=true) or upper part of an L\U
+// CSR.  Returns level of each row; *n_levels receives the count.
+std::vector<int> tri_levels(const HostCsr& LU, bool lower, int* n_levels);
+
+// Dense operator G (row-major n x n) of the reference's coarsest solve
+// (amg.cpp:260-291): Gaussian elimination with partial pivoting on [A | rhs],
+// pivots < 1e-14 skipped.  The solve is linear in rhs for a fixed A, so G is
+// formed once (columns = solves of unit vectors) and applied as z = G rhs.
+std::vector<double> coarse_operator(const HostCsr& A);
+
+}  // namespace hsps_host

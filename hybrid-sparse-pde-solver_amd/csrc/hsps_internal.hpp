@@ -1,0 +1,147 @@
+// =============================================================================
+// hsps_internal.hpp — host-side internals of the gfx950 solve path: handle
+// structs, error plumbing, workspace helpers.  Not part of the C ABI.
+// =============================================================================
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hsps_dev.h"
+
+namespace hsps_dev {
+
+// ---- error plumbing: exceptions inside, status codes at the boundary --------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define HSPS_HIP(call)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw ::hsps_dev::Error(HSPS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define HSPS_CHECK_LAUNCH() HSPS_HIP(hipGetLastError())
+
+// Runs body; maps exceptions to status codes and the thread-local message.
+template <class F>
+int guarded(F&& body) {
+    try {
+        return body();
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return HSPS_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return HSPS_ERR_INVALID;
+    }
+}
+
+// ---- device buffer with RAII, tracked by the owning context -------------------
+struct Ctx;
+struct DevBuf {
+    Ctx* ctx = nullptr;
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(Ctx* c, size_t b);
+    ~DevBuf();
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept;
+    void reset();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    size_t bytes_in_use = 0;
+    // scratch for standalone BLAS-1 reductions (stream ordered)
+    DevBuf red_partials, red_counter, red_out;
+    double* pinned = nullptr;  // host pinned readback area (64 doubles)
+    void set_current() const { HSPS_HIP(hipSetDevice(device)); }
+};
+
+// SpMV schedule constants (gfx950: 64-wide waves, 160 KB LDS per CU).
+constexpr int kBlock = 256;       // threads per workgroup (4 waves)
+constexpr int kLdsNnz = 2048;     // products staged in LDS per row block
+constexpr int kMaxBlockRows = 256;
+constexpr int kMaxRedBlocks = 2048;  // cap for grid-stride reduction kernels
+
+// Device CSR: int32 pattern, FP64 values + FP32 copy, diagonal, row blocks.
+struct Csr {
+    Ctx* ctx = nullptr;
+    int rows = 0, cols = 0, nnz = 0;
+    std::vector<int> h_rp, h_ci;
+    std::vector<double> h_v;
+    DevBuf rp, ci, v64, v32, diag64, blocks;
+    int n_blocks = 0;
+    void upload_values();
+    void build_schedule();
+};
+
+// Row-block partition: consecutive rows, <= kMaxBlockRows rows and <=
+// kLdsNnz nonzeros per block; a longer row gets a block of its own.
+std::vector<int> make_row_blocks(const int* rp, int rows, int max_rows, int max_nnz);
+
+// Wrap a device CSR from host arrays (used by AMG levels / ILU factors).
+std::unique_ptr<Csr> make_csr(Ctx* ctx, int rows, int cols, std::vector<int> rp, std::vector<int> ci,
+                              std::vector<double> v, bool want_f32 = true);
+
+// ---- launchers implemented in the .hip files ---------------------------------
+// SpMV variants (spmv.hip)
+void spmv_f64(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip = nullptr);
+void spmv_f32(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip = nullptr);
+// r = b - A x (FP64), and optionally ||r||^2 into *d_out (device scalar).
+void residual_f64(const Csr& A, const double* b, const double* x, double* r, hipStream_t s,
+                  double* d_nrm2 = nullptr, double* partials = nullptr, unsigned* counter = nullptr,
+                  const int* skip = nullptr);
+// y += A x (FP64 or FP32), y = A x as the 'add' flag says
+void spmv_add_f64(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip);
+void spmv_add_f32(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip);
+// xo = xi + omega * dinv .* (rhs - A xi)
+void smooth_f64(const Csr& A, const double* dinv, double omega, const double* rhs, const double* xi,
+                double* xo, hipStream_t s, const int* skip);
+void smooth_f32(const Csr& A, const float* dinv, float omega, const float* rhs, const float* xi, float* xo,
+                hipStream_t s, const int* skip);
+void residual_f32(const Csr& A, const float* b, const float* x, float* r, hipStream_t s, const int* skip);
+
+// BLAS-1 (blas.hip)
+void dot_f64_dev(int n, const double* x, const double* y, double* d_out, double* partials, unsigned* counter,
+                 hipStream_t s);
+void axpy_f64(int n, double a, const double* x, double* y, hipStream_t s);
+void axpby_f64(int n, double a, const double* x, double b, double* y, hipStream_t s);
+void scale_f64(int n, double a, double* x, hipStream_t s);
+void copy_f64_to_f32(int n, const double* x, float* y, hipStream_t s, const int* skip = nullptr);
+void fill_zero_f64(int n, double* x, hipStream_t s, const int* skip = nullptr);
+void fill_zero_f32(int n, float* x, hipStream_t s, const int* skip = nullptr);
+int reduction_blocks(int n);
+
+// Preconditioner object (precond.hip)
+struct Precond;
+void precond_apply_f64(const Precond& M, const double* r, double* z, hipStream_t s, const int* skip);
+void precond_apply_f32(const Precond& M, const float* r, float* z, hipStream_t s, const int* skip);
+int precond_kind(const Precond& M);
+const double* precond_jacobi_dinv64(const Precond& M);
+const float* precond_jacobi_dinv32(const Precond& M);
+
+}  // namespace hsps_dev
+
+struct hsps_ctx_s : hsps_dev::Ctx {};
+struct hsps_csr_s : hsps_dev::Csr {};

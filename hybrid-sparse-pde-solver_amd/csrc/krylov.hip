@@ -1,0 +1,766 @@
+// =============================================================================
+// krylov.hip — device-resident PCG and FGMRES(m).
+//
+// PCG (cg_solver.cpp:37-191): per iteration three kernels —
+//   K1  q = A p fused with the p.q reduction; its last workgroup forms alpha
+//       (and flags the p.q breakdown, :109-120);
+//   K2  x += alpha p, r -= alpha q, z = D^-1 r fused with the r.r and r.z
+//       reductions; its last workgroup forms rel = |r|/|b|, rho, beta and the
+//       convergence / rho-breakdown / iteration-limit decisions;
+//   K3  p = z + beta p.
+// FGMRES(m) (fgmres_solver.cpp:36-241): right-preconditioned, modified
+// Gram-Schmidt with each axpy fused into the next dot, Givens rotations and
+// the least-squares residual kept on the device (FP64).
+//
+// Precision: HSPS_FP64 runs everything in FP64.  HSPS_MIXED runs the SpMV,
+// preconditioner and Krylov vectors in FP32 with FP64 reductions and
+// FP64 H/Givens; the FP64 true residual b - A x is recomputed
+//   * CG: by reliable updates — whenever |r| falls below delta * max|r|
+//     since the last one, and to confirm convergence;
+//   * FGMRES: at every restart (each cycle is one refinement step), and a
+//     solve only ends when a cycle starts below tol.
+// The host polls a device flag every check_every iterations, one chunk ahead,
+// so the GPU never waits on the host within a chunk.
+// =============================================================================
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <type_traits>
+
+#include "device_common.hpp"
+#include "precond.hpp"
+
+namespace hsps_dev {
+
+static constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+enum : int { FLAG_NONE = 0, FLAG_BREAK_PQ = 1, FLAG_BREAK_RHO = 2, FLAG_RELIABLE = 3, FLAG_CHECK = 4 };
+
+struct CgState {
+    double rho, rho_old, alpha, beta, rel, bnorm, tol, rmax, delta;
+    double rr0, bb, rr_true;
+    int iter, max_iter, halt, done, flag, restarts, reliable, pad;
+};
+
+struct GmState {
+    double beta2, beta, res, tol_abs, inv_h;
+    int jdone, cycle_done, m, pad;
+};
+
+// ---- small device helpers ---------------------------------------------------------
+template <class T> __device__ __forceinline__ T to_t(double v) { return (T)v; }
+
+// ---- CG kernels -----------------------------------------------------------------------
+template <class T>
+struct EpiCgPq {
+    static constexpr int NV = 1;
+    const T* p;
+    T* q;
+    CgState* st;
+    double* partials;
+    unsigned* counter;
+    __device__ bool skip() const { return st->halt != 0; }
+    template <class A> __device__ void row(int i, A s, double* red) const {
+        const T qi = (T)s;
+        q[i] = qi;
+        red[0] += (double)p[i] * (double)qi;
+    }
+    __device__ void finish(const double* tot) const {
+        const double pq = tot[0];
+        if (fabs(pq) < kEps * 1e6 * fabs(st->rho)) {  // cg_solver.cpp:109
+            st->flag = FLAG_BREAK_PQ;
+            st->iter += 1;
+            st->halt = 1;
+        } else {
+            st->alpha = st->rho / pq;
+        }
+    }
+};
+
+// PREC: 0 none (z = r), 1 Jacobi (z = dinv r, inline), 2 external (z computed
+// after this kernel).  XT: type of the x being updated (FP64 x, or the FP32
+// low-order accumulator in mixed mode).
+template <class T, class XT, int PREC, bool MIXED>
+__global__ void __launch_bounds__(kBlock)
+k_cg_update(int n, XT* __restrict__ x, const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ r,
+            const T* __restrict__ dinv, CgState* st, double* partials, unsigned* counter) {
+    if (st->halt) return;
+    const double alpha = st->alpha;
+    double red[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        x[i] = (XT)((double)x[i] + alpha * (double)p[i]);
+        const T ri = (T)((double)r[i] - alpha * (double)q[i]);
+        r[i] = ri;
+        const double rd = (double)ri;
+        red[0] += rd * rd;
+        if constexpr (PREC == 1) red[1] += rd * (double)(T)(dinv[i] * ri);
+        if constexpr (PREC == 0) red[1] += rd * rd;
+    }
+    double tot[2];
+    if (!grid_sum<2>(red, partials, counter, tot)) return;
+    const double rn = sqrt(tot[0]);
+    const double rel = rn / st->bnorm;
+    st->rel = rel;
+    st->iter += 1;
+    if (rel < st->tol) {  // cg_solver.cpp:143
+        if (MIXED) st->flag = FLAG_CHECK; else st->done = 1;
+        st->halt = 1;
+        return;
+    }
+    if (MIXED) {
+        st->rmax = fmax(st->rmax, rn);
+        if (rn < st->delta * st->rmax) {
+            st->flag = FLAG_RELIABLE;
+            st->halt = 1;
+            return;
+        }
+    }
+    if (PREC == 2) return;  // rho from k_cg_rz after the external apply
+    st->rho_old = st->rho;
+    st->rho = tot[1];
+    if (fabs(st->rho_old) < kEps * 1e8 * fabs(st->rho)) {  // cg_solver.cpp:157
+        st->flag = FLAG_BREAK_RHO;
+        st->halt = 1;
+        return;
+    }
+    st->beta = st->rho / st->rho_old;
+    if (st->iter >= st->max_iter) st->halt = 1;
+}
+
+// rho = r.z for an external preconditioner (z already applied).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_rz(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st, double* partials, unsigned* counter) {
+    if (st->halt) return;
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) red[0] += (double)r[i] * (double)z[i];
+    double tot[1];
+    if (!grid_sum<1>(red, partials, counter, tot)) return;
+    st->rho_old = st->rho;
+    st->rho = tot[0];
+    if (fabs(st->rho_old) < kEps * 1e8 * fabs(st->rho)) {
+        st->flag = FLAG_BREAK_RHO;
+        st->halt = 1;
+        return;
+    }
+    st->beta = st->rho / st->rho_old;
+    if (st->iter >= st->max_iter) st->halt = 1;
+}
+
+// p = z + beta p (cg_solver.cpp:171); with init=true: p = z and rho = r.z
+// (cg_solver.cpp:77-85) from a freshly formed r.
+template <class T, int PREC, bool INIT>
+__global__ void __launch_bounds__(kBlock)
+k_cg_p(int n, T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ dinv,
+       CgState* st, double* partials, unsigned* counter) {
+    if (!INIT && st->halt) return;
+    const double beta = INIT ? 0.0 : st->beta;
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        T zi;
+        if constexpr (PREC == 0) zi = r[i];
+        else if constexpr (PREC == 1) zi = (T)(dinv[i] * r[i]);
+        else zi = z[i];
+        if (INIT) {
+            p[i] = zi;
+            red[0] += (double)r[i] * (double)zi;
+        } else {
+            p[i] = (T)((double)zi + beta * (double)p[i]);
+        }
+    }
+    if constexpr (INIT) {
+        double tot[1];
+        if (grid_sum<1>(red, partials, counter, tot)) {
+            st->rho = tot[0];
+            st->rho_old = tot[0];
+        }
+    }
+}
+
+// Mixed: x64 += x_lo; x_lo = 0.
+__global__ void k_flush(int n, double* __restrict__ x, float* __restrict__ xlo) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += (double)xlo[i];
+        xlo[i] = 0.0f;
+    }
+}
+
+// Mixed: replace r by the FP64 true residual rounded to FP32, and rebuild
+// rho / beta from it.  The p direction is kept (reliable update).
+template <int PREC>
+__global__ void __launch_bounds__(kBlock)
+k_cg_reliable(int n, const double* __restrict__ r64, float* __restrict__ r, const float* __restrict__ dinv,
+              CgState* st, double* partials, unsigned* counter) {
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float ri = (float)r64[i];
+        r[i] = ri;
+        if constexpr (PREC == 1) red[0] += (double)ri * (double)(dinv[i] * ri);
+        if constexpr (PREC == 0) red[0] += (double)ri * (double)ri;
+    }
+    double tot[1];
+    if (!grid_sum<1>(red, partials, counter, tot)) return;
+    const double rn = sqrt(st->rr_true);
+    const double rel = rn / st->bnorm;
+    st->reliable += 1;
+    st->rmax = rn;
+    if (st->flag == FLAG_CHECK && rel < st->tol) {
+        st->rel = rel;
+        st->done = 1;
+        st->halt = 1;
+        st->flag = FLAG_NONE;
+        return;
+    }
+    st->flag = FLAG_NONE;
+    st->halt = 0;
+    if (PREC != 2) {
+        st->rho_old = st->rho;
+        st->rho = tot[0];
+        st->beta = st->rho / st->rho_old;
+        if (st->iter >= st->max_iter) st->halt = 1;
+    }
+}
+
+// ---- FGMRES kernels ---------------------------------------------------------------------
+// V0 = r / beta, or stop when beta < tol_abs (fgmres_solver.cpp:82-96).
+template <class T>
+__global__ void k_gm_start(int n, const double* __restrict__ r, T* __restrict__ v0, GmState* st, double* g) {
+    const double beta = sqrt(st->beta2);
+    const bool stop = beta < st->tol_abs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->beta = beta;
+        st->res = beta;
+        st->jdone = 0;
+        st->cycle_done = stop ? 1 : 0;
+        for (int k = 0; k <= st->m; ++k) g[k] = 0.0;
+        g[0] = beta;
+    }
+    if (stop) return;
+    const double s = 1.0 / beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v0[i] = (T)(r[i] * s);
+}
+
+// One modified Gram-Schmidt step, with the previous step's axpy fused in:
+//   w -= h_prev * v_prev  (when h_prev given);  H[i][j] = w . v_i
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_mgs(int n, T* __restrict__ w, const T* __restrict__ vprev, const double* hprev, const T* __restrict__ vi,
+      double* hout, const int* skip, double* partials, unsigned* counter) {
+    if (*skip) return;
+    const double h = hprev ? *hprev : 0.0;
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        T wi = w[i];
+        if (hprev) {
+            wi = (T)((double)wi - h * (double)vprev[i]);
+            w[i] = wi;
+        }
+        red[0] += (double)wi * (double)vi[i];
+    }
+    double tot[1];
+    if (grid_sum<1>(red, partials, counter, tot)) *hout = tot[0];
+}
+
+__device__ void gm_givens(double f, double g, double& cs, double& sn, double& r) {  // fgmres_solver.cpp:36-45
+    if (fabs(g) < kEps) { cs = 1.0; sn = 0.0; r = f; }
+    else if (fabs(f) < kEps) { cs = 0.0; sn = 1.0; r = g; }
+    else { const double t = sqrt(f * f + g * g); cs = f / t; sn = g / t; r = t; }
+}
+__device__ void gm_rot(double& dx, double& dy, double cs, double sn) {  // fgmres_solver.cpp:47-51
+    const double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
+
+// Last MGS step + norm; then (one thread) the Givens update of column j and
+// the least-squares residual estimate (fgmres_solver.cpp:122-148).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_mgs_last(int n, int j, T* __restrict__ w, const T* __restrict__ vj, double* H, double* cs, double* sn, double* g,
+           GmState* st, double* partials, unsigned* counter) {
+    if (st->cycle_done) return;
+    const int m = st->m;
+    const double h = H[j * m + j];
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const T wi = (T)((double)w[i] - h * (double)vj[i]);
+        w[i] = wi;
+        red[0] += (double)wi * (double)wi;
+    }
+    double tot[1];
+    if (!grid_sum<1>(red, partials, counter, tot)) return;
+    const double hn = sqrt(tot[0]);
+    H[(j + 1) * m + j] = hn;
+    st->inv_h = hn > kEps ? 1.0 / hn : 0.0;  // lucky breakdown leaves V_{j+1} = 0
+    for (int i = 0; i < j; ++i) gm_rot(H[i * m + j], H[(i + 1) * m + j], cs[i], sn[i]);
+    double rv;
+    gm_givens(H[j * m + j], H[(j + 1) * m + j], cs[j], sn[j], rv);
+    H[j * m + j] = rv;
+    H[(j + 1) * m + j] = 0.0;
+    gm_rot(g[j], g[j + 1], cs[j], sn[j]);
+    st->res = fabs(g[j + 1]);
+    st->jdone = j + 1;
+    if (st->res < st->tol_abs) st->cycle_done = 1;
+}
+
+template <class T>
+__global__ void k_gm_scale(int n, T* __restrict__ v, const GmState* st) {
+    if (st->cycle_done) return;
+    const double s = st->inv_h;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = (T)((double)v[i] * s);
+}
+
+// y = H(0:k,0:k)^-1 g (fgmres_solver.cpp:57-66), one thread.
+__global__ void k_gm_backsolve(const double* H, const double* g, double* y, const GmState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int k = st->jdone, m = st->m;
+    for (int i = k - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int c = i + 1; c < k; ++c) s -= H[i * m + c] * y[c];
+        y[i] = fabs(H[i * m + i]) > kEps ? s / H[i * m + i] : 0.0;
+    }
+}
+
+// x += sum_i y_i Z_i in the reference's order (fgmres_solver.cpp:156-157).
+template <class T>
+__global__ void k_gm_xupdate(int n, double* __restrict__ x, const T* __restrict__ Z, size_t ld, const double* y,
+                             const GmState* st) {
+    const int k = st->jdone;
+    if (k == 0) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = x[i];
+        for (int c = 0; c < k; ++c) a += y[c] * (double)Z[c * ld + i];
+        x[i] = a;
+    }
+}
+
+// ---- host side -------------------------------------------------------------------------
+struct Pinned {
+    void* p = nullptr;
+    explicit Pinned(size_t b) { HSPS_HIP(hipHostMalloc(&p, b, hipHostMallocDefault)); }
+    ~Pinned() { if (p) (void)hipHostFree(p); }
+};
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { HSPS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void finish_stats(hsps_stats* s, long long F, long long B) {
+    s->flop_count = F;
+    s->mem_bytes = B;
+    s->energy_joules = 2.0e-10 * (double)F + 5.0e-9 * (double)B;
+}
+
+// flop_model (energy_monitor.hpp:163-177)
+static long long spmv_bytes(long long rows, long long nnz) { return (rows + 1 + nnz) * 4 + nnz * 8 + rows * 16; }
+
+// True relative residual ||b - A x|| / ||b|| in FP64 (device), for stats.
+static double true_rel(const Csr& A, const double* b, const double* x, double* r64, double bnorm, double* d_out,
+                       double* partials, unsigned* counter, hipStream_t s) {
+    residual_f64(A, b, x, r64, s, d_out, partials, counter, nullptr);
+    double h = 0.0;
+    HSPS_HIP(hipMemcpyAsync(&h, d_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+    return bnorm > 0 ? std::sqrt(h) / bnorm : std::sqrt(h);
+}
+
+template <class T>
+static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p, hsps_stats* out) {
+    constexpr bool MIXED = std::is_same<T, float>::value;
+    Ctx* ctx = A.ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A.rows;
+    const double t0 = now_s();
+    hsps_stats st{};
+    const int kind = M ? M->kind : HSPS_PRECOND_NONE;
+    const int PREC = kind == HSPS_PRECOND_NONE ? 0 : kind == HSPS_PRECOND_JACOBI ? 1 : 2;
+    if (MIXED && !A.v32.p && A.nnz) throw Error(HSPS_ERR_UNSUPPORTED, "matrix has no FP32 copy");
+
+    const size_t nb = (size_t)std::max(n, 1);
+    DevBuf r(ctx, nb * sizeof(T)), pv(ctx, nb * sizeof(T)), q(ctx, nb * sizeof(T));
+    DevBuf z(ctx, PREC == 2 ? nb * sizeof(T) : 0);
+    DevBuf r64(ctx, nb * sizeof(double));
+    DevBuf xlo(ctx, MIXED ? nb * sizeof(float) : 0);
+    const int red_grid = std::max(reduction_blocks(n), A.n_blocks);
+    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(red_grid, 1));
+    DevBuf counter(ctx, 64);
+    DevBuf dst(ctx, sizeof(CgState));
+    DevBuf scal(ctx, 64);
+    Pinned pin(sizeof(CgState) * 2);
+    CgState* hst = static_cast<CgState*>(pin.p);
+    CgState* d = dst.as<CgState>();
+    double* part = partials.as<double>();
+    unsigned* cnt = counter.as<unsigned>();
+    HSPS_HIP(hipMemsetAsync(counter.p, 0, 64, s));
+
+    // r = b - A x, ||r||^2, ||b||^2  (cg_solver.cpp:52-57)
+    residual_f64(A, b, x, r64.as<double>(), s, &d->rr0, part, cnt, nullptr);
+    dot_f64_dev(n, b, b, &d->bb, part, cnt, s);
+    HSPS_HIP(hipMemcpyAsync(hst, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+    const double bnorm = std::sqrt(hst->bb), r0 = std::sqrt(hst->rr0);
+    st.initial_residual = r0;
+    auto finish = [&](int conv) {
+        st.converged = conv;
+        st.solve_time_s = now_s() - t0;
+        if (out) *out = st;
+        return conv;
+    };
+    if (bnorm < kEps) {  // cg_solver.cpp:60-66 (absolute residual reported)
+        st.final_residual = r0;
+        st.true_residual = r0;
+        return finish(1);
+    }
+    if (r0 / bnorm < p.tol) {
+        st.final_residual = st.true_residual = r0 / bnorm;
+        return finish(1);
+    }
+
+    CgState init{};
+    init.bnorm = bnorm;
+    init.tol = p.tol;
+    init.max_iter = p.max_iter;
+    init.rmax = r0;
+    init.delta = p.reliable_delta > 0 ? p.reliable_delta : 0.1;
+    init.rr0 = hst->rr0;
+    init.bb = hst->bb;
+    init.halt = p.max_iter <= 0 ? 1 : 0;
+    std::memcpy(hst, &init, sizeof init);
+    HSPS_HIP(hipMemcpyAsync(d, hst, sizeof(CgState), hipMemcpyHostToDevice, s));
+
+    const T* dinv = nullptr;
+    if (PREC == 1) dinv = MIXED ? (const T*)precond_jacobi_dinv32(*M) : (const T*)precond_jacobi_dinv64(*M);
+    T* rv = r.as<T>();
+    T* p_ = pv.as<T>();
+    T* q_ = q.as<T>();
+    T* z_ = z.as<T>();
+    const int g = reduction_blocks(n);
+    const Csr& Am = A;
+
+    // r (working precision), z = M r, p = z, rho = r.z  (cg_solver.cpp:77-85)
+    auto start_direction = [&](bool init_rho) {
+        if (MIXED) copy_f64_to_f32(n, r64.as<double>(), (float*)rv, s);
+        else HSPS_HIP(hipMemcpyAsync(rv, r64.p, nb * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (PREC == 2) {
+            if (MIXED) precond_apply_f32(*M, (const float*)rv, (float*)z_, s, nullptr);
+            else precond_apply_f64(*M, (const double*)rv, (double*)z_, s, nullptr);
+        }
+        (void)init_rho;
+        switch (PREC) {
+            case 0: hipLaunchKernelGGL((k_cg_p<T, 0, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+            case 1: hipLaunchKernelGGL((k_cg_p<T, 1, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+            default: hipLaunchKernelGGL((k_cg_p<T, 2, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+        }
+        HSPS_CHECK_LAUNCH();
+    };
+    start_direction(true);
+    if (MIXED) fill_zero_f32(n, xlo.as<float>(), s);
+
+    auto spmv_pq = [&]() {
+        EpiCgPq<T> e;
+        e.p = p_; e.q = q_; e.st = d; e.partials = part; e.counter = cnt;
+        if (MIXED) launch_spmv(Am, Am.v32.as<float>(), (const float*)p_, e, s);
+        else launch_spmv(Am, Am.v64.as<double>(), (const double*)p_, e, s);
+    };
+    auto update = [&]() {
+        if (MIXED) {
+            float* xl = xlo.as<float>();
+            switch (PREC) {
+                case 0: hipLaunchKernelGGL((k_cg_update<T, float, 0, true>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part, cnt); break;
+                case 1: hipLaunchKernelGGL((k_cg_update<T, float, 1, true>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part, cnt); break;
+                default: hipLaunchKernelGGL((k_cg_update<T, float, 2, true>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part, cnt); break;
+            }
+        } else {
+            switch (PREC) {
+                case 0: hipLaunchKernelGGL((k_cg_update<T, double, 0, false>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part, cnt); break;
+                case 1: hipLaunchKernelGGL((k_cg_update<T, double, 1, false>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part, cnt); break;
+                default: hipLaunchKernelGGL((k_cg_update<T, double, 2, false>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part, cnt); break;
+            }
+        }
+        HSPS_CHECK_LAUNCH();
+        if (PREC == 2) {
+            if (MIXED) precond_apply_f32(*M, (const float*)rv, (float*)z_, s, &d->halt);
+            else precond_apply_f64(*M, (const double*)rv, (double*)z_, s, &d->halt);
+            hipLaunchKernelGGL(k_cg_rz<T>, dim3(g), dim3(kBlock), 0, s, n, rv, z_, d, part, cnt);
+            HSPS_CHECK_LAUNCH();
+        }
+    };
+    auto pupdate = [&]() {
+        switch (PREC) {
+            case 0: hipLaunchKernelGGL((k_cg_p<T, 0, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+            case 1: hipLaunchKernelGGL((k_cg_p<T, 1, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+            default: hipLaunchKernelGGL((k_cg_p<T, 2, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+        }
+        HSPS_CHECK_LAUNCH();
+    };
+
+    const int chunk = std::max(1, p.check_every > 0 ? p.check_every : 32);
+    Event ev[2];
+    int k = 0;
+    bool pending = false;
+    long long F = 0, B = 0;
+    auto read_state = [&]() {
+        HSPS_HIP(hipStreamSynchronize(s));
+        HSPS_HIP(hipMemcpyAsync(hst, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        HSPS_HIP(hipStreamSynchronize(s));
+    };
+    for (;;) {
+        for (int c = 0; c < chunk; ++c) {
+            spmv_pq();
+            update();
+            pupdate();
+        }
+        CgState* slot = hst + (k & 1);
+        HSPS_HIP(hipMemcpyAsync(slot, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        HSPS_HIP(hipEventRecord(ev[k & 1].e, s));
+        bool halted = false;
+        if (pending) {
+            HSPS_HIP(hipEventSynchronize(ev[(k - 1) & 1].e));
+            halted = hst[(k - 1) & 1].halt != 0;
+        }
+        pending = true;
+        ++k;
+        if (!halted) continue;
+        read_state();  // everything after the halt was a no-op
+        pending = false;
+        k = 0;
+        CgState cur = hst[0];
+        if (cur.done || cur.iter >= cur.max_iter) {
+            if (!(cur.flag == FLAG_CHECK || cur.flag == FLAG_RELIABLE) || cur.done) break;
+        }
+        if (cur.flag == FLAG_BREAK_PQ || cur.flag == FLAG_BREAK_RHO) {
+            // restart from the current x (cg_solver.cpp:109-120, 157-167)
+            if (MIXED) hipLaunchKernelGGL(k_flush, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, xlo.as<float>());
+            residual_f64(A, b, x, r64.as<double>(), s, nullptr, nullptr, nullptr, nullptr);
+            start_direction(true);
+            cur.restarts += 1;
+            cur.flag = FLAG_NONE;
+            cur.halt = cur.iter >= cur.max_iter ? 1 : 0;
+            // write back flag/halt/restarts without disturbing rho (set by k_cg_p<INIT>)
+            HSPS_HIP(hipMemcpyAsync(&d->flag, &cur.flag, sizeof(int), hipMemcpyHostToDevice, s));
+            HSPS_HIP(hipMemcpyAsync(&d->halt, &cur.halt, sizeof(int), hipMemcpyHostToDevice, s));
+            HSPS_HIP(hipMemcpyAsync(&d->restarts, &cur.restarts, sizeof(int), hipMemcpyHostToDevice, s));
+            HSPS_HIP(hipStreamSynchronize(s));
+            if (cur.halt) break;
+            continue;
+        }
+        if (MIXED && (cur.flag == FLAG_RELIABLE || cur.flag == FLAG_CHECK)) {
+            hipLaunchKernelGGL(k_flush, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, xlo.as<float>());
+            residual_f64(A, b, x, r64.as<double>(), s, &d->rr_true, part, cnt, nullptr);
+            switch (PREC) {
+                case 0: hipLaunchKernelGGL(k_cg_reliable<0>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, d, part, cnt); break;
+                case 1: hipLaunchKernelGGL(k_cg_reliable<1>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, d, part, cnt); break;
+                default: hipLaunchKernelGGL(k_cg_reliable<2>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, d, part, cnt); break;
+            }
+            HSPS_CHECK_LAUNCH();
+            if (PREC == 2) {
+                precond_apply_f32(*M, (const float*)rv, (float*)z_, s, &d->halt);
+                hipLaunchKernelGGL(k_cg_rz<T>, dim3(g), dim3(kBlock), 0, s, n, rv, z_, d, part, cnt);
+                HSPS_CHECK_LAUNCH();
+            }
+            pupdate();
+            read_state();
+            if (hst[0].done || hst[0].halt) break;
+            continue;
+        }
+        break;  // halted for the iteration limit
+    }
+    if (MIXED) hipLaunchKernelGGL(k_flush, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, xlo.as<float>());
+    read_state();
+    const CgState fin = hst[0];
+    st.iterations = fin.iter;
+    st.restarts = fin.restarts;
+    st.reliable_updates = fin.reliable;
+    st.final_residual = fin.rel;
+    st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, cnt, s);
+    const int conv = MIXED ? (st.true_residual < p.tol && fin.done) : (fin.rel < p.tol);
+    const long long it = fin.iter;
+    const size_t vb = MIXED ? 4 : 8;
+    F = it * (2LL * A.nnz + 10LL * n);
+    B = it * (spmv_bytes(n, A.nnz) + 7LL * n * (long long)vb);
+    finish_stats(&st, F, B);
+    return finish(conv);
+}
+
+template <class T>
+static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p,
+                      hsps_stats* out) {
+    constexpr bool MIXED = std::is_same<T, float>::value;
+    Ctx* ctx = A.ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A.rows;
+    const double t0 = now_s();
+    hsps_stats st{};
+    if (MIXED && !A.v32.p && A.nnz) throw Error(HSPS_ERR_UNSUPPORTED, "matrix has no FP32 copy");
+    const int m = std::max(1, std::min(p.restart_size, n));
+    const bool has_m = M && M->kind != HSPS_PRECOND_NONE;
+    const size_t ld = ((size_t)std::max(n, 1) + 63) / 64 * 64;
+    DevBuf V(ctx, ld * (m + 1) * sizeof(T));
+    DevBuf Z(ctx, has_m ? ld * m * sizeof(T) : 0);
+    DevBuf r64(ctx, ld * sizeof(double));
+    const int red_grid = std::max(reduction_blocks(n), A.n_blocks);
+    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(red_grid, 1));
+    DevBuf counter(ctx, 64);
+    DevBuf dst(ctx, sizeof(GmState));
+    DevBuf Hb(ctx, sizeof(double) * (size_t)(m + 1) * m);
+    DevBuf csb(ctx, sizeof(double) * m), snb(ctx, sizeof(double) * m), gb(ctx, sizeof(double) * (m + 1)),
+        yb(ctx, sizeof(double) * m);
+    DevBuf scal(ctx, 64);
+    Pinned pin(sizeof(GmState) * 4 + 64);
+    GmState* hst = static_cast<GmState*>(pin.p);
+    GmState* d = dst.as<GmState>();
+    double* part = partials.as<double>();
+    unsigned* cnt = counter.as<unsigned>();
+    double* H = Hb.as<double>();
+    HSPS_HIP(hipMemsetAsync(counter.p, 0, 64, s));
+    HSPS_HIP(hipMemsetAsync(Hb.p, 0, Hb.bytes, s));
+    T* Vp = V.as<T>();
+    T* Zp = has_m ? Z.as<T>() : Vp;
+    auto Vk = [&](int k) { return Vp + (size_t)k * ld; };
+    auto Zk = [&](int k) { return Zp + (size_t)k * ld; };
+
+    // ||b|| (fgmres_solver.cpp:182-188)
+    dot_f64_dev(n, b, b, scal.as<double>(), part, cnt, s);
+    double bb = 0.0;
+    HSPS_HIP(hipMemcpyAsync(&bb, scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+    const double bnorm = std::sqrt(bb);
+    auto finish = [&](int conv) {
+        st.converged = conv;
+        st.solve_time_s = now_s() - t0;
+        if (out) *out = st;
+        return conv;
+    };
+    if (bnorm < kEps) {
+        st.final_residual = 0.0;
+        return finish(1);
+    }
+    const double tol_abs = p.tol * bnorm;
+    GmState init{};
+    init.tol_abs = tol_abs;
+    init.m = m;
+    HSPS_HIP(hipMemcpyAsync(d, &init, sizeof init, hipMemcpyHostToDevice, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+
+    const int g = reduction_blocks(n);
+    int total = 0, cycles = 0, conv = 0;
+    double rel = 1.0;
+    bool first = true;
+    const int steps_per_poll = 4;
+    Event ev[2];
+    while (total < p.max_iter) {
+        const int cm = std::min(m, p.max_iter - total);
+        residual_f64(A, b, x, r64.as<double>(), s, &d->beta2, part, cnt, nullptr);
+        hipLaunchKernelGGL(k_gm_start<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r64.as<double>(), Vk(0), d, gb.as<double>());
+        HSPS_CHECK_LAUNCH();
+        int k = 0;
+        bool pending = false;
+        for (int j = 0; j < cm; ++j) {
+            const int* skip = &d->cycle_done;
+            if (has_m) {
+                if (MIXED) precond_apply_f32(*M, (const float*)Vk(j), (float*)Zk(j), s, skip);
+                else precond_apply_f64(*M, (const double*)Vk(j), (double*)Zk(j), s, skip);
+            }
+            T* w = Vk(j + 1);
+            if (MIXED) spmv_f32(A, (const float*)Zk(j), (float*)w, s, skip);
+            else spmv_f64(A, (const double*)Zk(j), (double*)w, s, skip);
+            for (int i = 0; i <= j; ++i) {
+                const double* hprev = i > 0 ? H + (size_t)(i - 1) * m + j : nullptr;
+                const T* vprev = i > 0 ? Vk(i - 1) : nullptr;
+                hipLaunchKernelGGL(k_mgs<T>, dim3(g), dim3(kBlock), 0, s, n, w, vprev, hprev, (const T*)Vk(i),
+                                   H + (size_t)i * m + j, skip, part, cnt);
+                HSPS_CHECK_LAUNCH();
+            }
+            hipLaunchKernelGGL(k_mgs_last<T>, dim3(g), dim3(kBlock), 0, s, n, j, w, (const T*)Vk(j), H,
+                               csb.as<double>(), snb.as<double>(), gb.as<double>(), d, part, cnt);
+            HSPS_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_gm_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, w, (const GmState*)d);
+            HSPS_CHECK_LAUNCH();
+            if ((j + 1) % steps_per_poll == 0 && j + 1 < cm) {
+                GmState* slot = hst + (k & 1);
+                HSPS_HIP(hipMemcpyAsync(slot, d, sizeof(GmState), hipMemcpyDeviceToHost, s));
+                HSPS_HIP(hipEventRecord(ev[k & 1].e, s));
+                bool done = false;
+                if (pending) {
+                    HSPS_HIP(hipEventSynchronize(ev[(k - 1) & 1].e));
+                    done = hst[(k - 1) & 1].cycle_done != 0;
+                }
+                pending = true;
+                ++k;
+                if (done) break;
+            }
+        }
+        hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, (const double*)H, (const double*)gb.as<double>(),
+                           yb.as<double>(), (const GmState*)d);
+        hipLaunchKernelGGL(k_gm_xupdate<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, (const T*)Zp, ld,
+                           (const double*)yb.as<double>(), (const GmState*)d);
+        HSPS_CHECK_LAUNCH();
+        HSPS_HIP(hipMemcpyAsync(hst + 2, d, sizeof(GmState), hipMemcpyDeviceToHost, s));
+        HSPS_HIP(hipStreamSynchronize(s));
+        const GmState cur = hst[2];
+        const int its = cur.jdone;
+        if (first) { st.initial_residual = cur.beta; first = false; }
+        total += its;
+        ++cycles;
+        rel = (its == 0 ? cur.beta : cur.res) / bnorm;
+        if (MIXED) {
+            // a cycle that starts below tol is the refinement loop's exit
+            if (its == 0) { conv = rel < p.tol; break; }
+        } else if (rel < p.tol || its == 0) {  // fgmres_solver.cpp:221-225
+            conv = rel < p.tol;
+            break;
+        }
+    }
+    st.iterations = total;
+    st.restarts = cycles - 1;
+    st.final_residual = rel;
+    st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, cnt, s);
+    if (MIXED) conv = conv && st.true_residual < p.tol;
+    const size_t vb = MIXED ? 4 : 8;
+    finish_stats(&st, (long long)total * (2LL * A.nnz + 4LL * n),
+                 (long long)total * (spmv_bytes(n, A.nnz) + 4LL * n * (long long)vb));
+    return finish(conv);
+}
+
+}  // namespace hsps_dev
+
+using namespace hsps_dev;
+
+static void check_solve_args(hsps_csr A, hsps_precond M, const double* b, double* x) {
+    if (!A || !b || !x) throw Error(HSPS_ERR_INVALID, "solve: null argument");
+    if (A->rows != A->cols) throw Error(HSPS_ERR_SIZE, "solve: matrix must be square");
+    if (M && M->n != A->rows) throw Error(HSPS_ERR_SIZE, "solve: preconditioner size != matrix size");
+}
+
+extern "C" {
+
+int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const hsps_params* pp, hsps_stats* st) {
+    return guarded([&] {
+        check_solve_args(A, M, b, x);
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        A->ctx->set_current();
+        return p.precision == HSPS_MIXED ? run_cg<float>(*A, M, b, x, p, st) : run_cg<double>(*A, M, b, x, p, st);
+    });
+}
+
+int hsps_fgmres_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const hsps_params* pp,
+                      hsps_stats* st) {
+    return guarded([&] {
+        check_solve_args(A, M, b, x);
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        A->ctx->set_current();
+        return p.precision == HSPS_MIXED ? run_fgmres<float>(*A, M, b, x, p, st)
+                                         : run_fgmres<double>(*A, M, b, x, p, st);
+    });
+}
+
+}  // extern "C"

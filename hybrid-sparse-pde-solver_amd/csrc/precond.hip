@@ -1,0 +1,505 @@
+// =============================================================================
+// precond.hip — device preconditioners behind PreconditionerBase
+// (preconditioner_base.hpp:16-48):
+//   Jacobi   z = omega D^-1 r                      (jacobi.cpp:13-51)
+//   ILU      L\U from ILU(0)/ILUT (host, exact), level-scheduled device
+//            forward/backward sweeps               (ilu.cpp:31-271)
+//   SA-AMG   host aggregation (bit-exact), smoothed P, R = P^T, Galerkin;
+//            device V(pre,post) cycle with fused smoother/residual SpMV
+//            epilogues and a dense coarsest operator (amg.cpp:30-343)
+// =============================================================================
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <type_traits>
+
+#include "device_common.hpp"
+#include "host_setup.hpp"
+#include "precond.hpp"
+
+namespace hsps_dev {
+
+static constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+// ---- element-wise kernels ---------------------------------------------------------
+template <class T>
+__global__ void k_diag_scale(int n, const T* __restrict__ d, const T* __restrict__ r, T* __restrict__ z,
+                             const int* skip) {
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = d[i] * r[i];
+}
+
+// x = omega * dinv * rhs : the first smoothing sweep from a zero guess
+// (A*0 = 0 in amg.cpp:247-252).
+template <class T>
+__global__ void k_first_sweep(int n, T omega, const T* __restrict__ dinv, const T* __restrict__ rhs,
+                              T* __restrict__ x, const int* skip) {
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] = omega * dinv[i] * (rhs[i] - T(0));
+}
+
+template <class T>
+__global__ void k_copy(int n, const T* __restrict__ a, T* __restrict__ b, const int* skip) {
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+// Dense z = G rhs (row-major n x n), one wave64 per row.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_dense_gemv(int n, const T* __restrict__ G, const T* __restrict__ rhs, T* __restrict__ z, const int* skip) {
+    if (skip && *skip) return;
+    const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    const T* g = G + (size_t)row * n;
+    for (int c = lane; c < n; c += 64) s += (double)g[c] * (double)rhs[c];
+    s = wave_sum(s);
+    if (lane == 0) z[row] = (T)s;
+}
+
+// Level-scheduled triangular sweeps in one workgroup: rows of a level are
+// independent; a workgroup barrier separates levels.  Forward: unit lower
+// (ilu.cpp:225-239, columns sorted so the scan stops at j >= i).  Backward:
+// divide by U(i,i) (ilu.cpp:244-262).
+constexpr int kTriThreads = 1024;
+template <class V, class T>
+__global__ void __launch_bounds__(kTriThreads)
+k_tri_lower(int n_levels, const int* __restrict__ lptr, const int* __restrict__ order, const int* __restrict__ rp,
+            const int* __restrict__ ci, const V* __restrict__ lu, const T* __restrict__ r, T* y, const int* skip) {
+    if (skip && *skip) return;
+    for (int lev = 0; lev < n_levels; ++lev) {
+        const int a = lptr[lev], b = lptr[lev + 1];
+        for (int idx = a + (int)threadIdx.x; idx < b; idx += kTriThreads) {
+            const int i = order[idx];
+            T s = r[i];
+            for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                const int j = ci[k];
+                if (j >= i) break;
+                s -= (T)lu[k] * y[j];
+            }
+            y[i] = s;
+        }
+        __syncthreads();
+    }
+}
+template <class V, class T>
+__global__ void __launch_bounds__(kTriThreads)
+k_tri_upper(int n_levels, const int* __restrict__ lptr, const int* __restrict__ order, const int* __restrict__ rp,
+            const int* __restrict__ ci, const V* __restrict__ lu, const T* __restrict__ inv_d, const T* __restrict__ y,
+            T* z, const int* skip) {
+    if (skip && *skip) return;
+    for (int lev = 0; lev < n_levels; ++lev) {
+        const int a = lptr[lev], b = lptr[lev + 1];
+        for (int idx = a + (int)threadIdx.x; idx < b; idx += kTriThreads) {
+            const int i = order[idx];
+            T s = y[i];
+            for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                const int j = ci[k];
+                if (j > i) s -= (T)lu[k] * z[j];
+            }
+            z[i] = s * inv_d[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- helpers ---------------------------------------------------------------------
+static DevBuf upload(Ctx* c, const void* h, size_t bytes) {
+    DevBuf b(c, bytes);
+    if (bytes) HSPS_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+    return b;
+}
+static DevBuf upload_f32(Ctx* c, const std::vector<double>& v) {
+    std::vector<float> f(v.begin(), v.end());
+    return upload(c, f.data(), f.size() * sizeof(float));
+}
+static hsps_host::HostCsr to_host(const Csr& A) {
+    hsps_host::HostCsr h;
+    h.rows = A.rows;
+    h.cols = A.cols;
+    h.rp = A.h_rp;
+    h.ci = A.h_ci;
+    h.v = A.h_v;
+    return h;
+}
+static std::unique_ptr<Csr> to_dev(Ctx* c, hsps_host::HostCsr&& h) {
+    return make_csr(c, h.rows, h.cols, std::move(h.rp), std::move(h.ci), std::move(h.v), true);
+}
+
+static TriSchedule make_schedule(Ctx* c, const std::vector<int>& lev, int n_levels) {
+    TriSchedule s;
+    const int n = (int)lev.size();
+    s.n_levels = n_levels;
+    s.h_level_ptr.assign(n_levels + 1, 0);
+    for (int i = 0; i < n; ++i) s.h_level_ptr[lev[i] + 1]++;
+    for (int l = 0; l < n_levels; ++l) {
+        s.max_level_width = std::max(s.max_level_width, s.h_level_ptr[l + 1]);
+        s.h_level_ptr[l + 1] += s.h_level_ptr[l];
+    }
+    std::vector<int> order(n), pos(s.h_level_ptr.begin(), s.h_level_ptr.end() - 1);
+    for (int i = 0; i < n; ++i) order[pos[lev[i]]++] = i;
+    s.order = upload(c, order.data(), order.size() * sizeof(int));
+    s.level_ptr = upload(c, s.h_level_ptr.data(), s.h_level_ptr.size() * sizeof(int));
+    return s;
+}
+
+// ---- setup -------------------------------------------------------------------------
+static void setup_jacobi(Precond& M, const Csr& A, double omega) {
+    std::vector<double> d(A.rows);
+    for (int i = 0; i < A.rows; ++i) {
+        double a = 0.0;
+        for (int k = A.h_rp[i]; k < A.h_rp[i + 1]; ++k)
+            if (A.h_ci[k] == i) { a = A.h_v[k]; break; }
+        if (std::abs(a) < kEps)
+            throw Error(HSPS_ERR_ZERO_DIAG, "JacobiPreconditioner: zero diagonal at row " + std::to_string(i));
+        d[i] = omega / a;
+    }
+    M.dinv64 = upload(M.ctx, d.data(), d.size() * sizeof(double));
+    M.dinv32 = upload_f32(M.ctx, d);
+}
+
+static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {
+    hsps_host::HostCsr F;
+    try {
+        F = (drop_tol > 0.0 || fill_p > 0) ? hsps_host::ilut(to_host(A), drop_tol, fill_p) : hsps_host::ilu0(to_host(A));
+    } catch (const std::runtime_error& e) {
+        throw Error(HSPS_ERR_PIVOT, e.what());
+    }
+    auto f = std::make_unique<IluFactor>();
+    // U(i,i) as the backward sweep reads it: 1.0 when row i has no diagonal
+    // entry; a near-zero value is an error (ilu.cpp:250-259), raised here at
+    // setup instead of at the first apply.
+    std::vector<double> invd(F.rows, 1.0);
+    for (int i = 0; i < F.rows; ++i) {
+        double d = 1.0;
+        for (int k = F.rp[i]; k < F.rp[i + 1]; ++k)
+            if (F.ci[k] == i) { d = F.v[k]; break; }
+        if (std::abs(d) < kEps)
+            throw Error(HSPS_ERR_PIVOT, "ILU backward sub: near-zero U diagonal at row " + std::to_string(i));
+        invd[i] = 1.0 / d;
+    }
+    int nl_lo = 0, nl_up = 0;
+    auto lo = hsps_host::tri_levels(F, true, &nl_lo);
+    auto up = hsps_host::tri_levels(F, false, &nl_up);
+    f->fwd = make_schedule(M.ctx, lo, nl_lo);
+    f->bwd = make_schedule(M.ctx, up, nl_up);
+    f->inv_diag64 = upload(M.ctx, invd.data(), invd.size() * sizeof(double));
+    f->inv_diag32 = upload_f32(M.ctx, invd);
+    f->tmp64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
+    f->tmp32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
+    f->lu = to_dev(M.ctx, std::move(F));
+    M.ilu = std::move(f);
+}
+
+static void alloc_level_work(Ctx* c, AmgLevelDev& L, int n) {
+    L.x64 = DevBuf(c, (size_t)n * 8);
+    L.x64b = DevBuf(c, (size_t)n * 8);
+    L.rhs64 = DevBuf(c, (size_t)n * 8);
+    L.res64 = DevBuf(c, (size_t)n * 8);
+    L.x32 = DevBuf(c, (size_t)n * 4);
+    L.x32b = DevBuf(c, (size_t)n * 4);
+    L.rhs32 = DevBuf(c, (size_t)n * 4);
+    L.res32 = DevBuf(c, (size_t)n * 4);
+}
+
+static void setup_amg(Precond& M, const Csr& A, const hsps_params& p) {
+    const int coarse_limit = 50;  // amg.hpp:99
+    M.amg_omega = p.amg_omega;
+    M.pre = p.amg_smooth_pre;
+    M.post = p.amg_smooth_post;
+    M.amg.clear();
+    hsps_host::HostCsr Af = to_host(A);
+    std::vector<hsps_host::HostCsr> hostA;
+    // level 0 (amg.cpp:189-196)
+    M.amg.emplace_back();
+    {
+        auto d = hsps_host::inv_diagonal(Af);
+        M.amg[0].dinv64 = upload(M.ctx, d.data(), d.size() * 8);
+        M.amg[0].dinv32 = upload_f32(M.ctx, d);
+    }
+    hostA.push_back(std::move(Af));
+    for (int lv = 0; lv < p.amg_levels - 1; ++lv) {  // amg.cpp:198-234
+        const hsps_host::HostCsr& F = hostA.back();
+        if (F.rows <= coarse_limit) break;
+        int nc = 0;
+        std::vector<int> agg = hsps_host::aggregate(F, p.amg_strong, &nc);
+        if (nc >= F.rows) break;
+        hsps_host::HostCsr P = hsps_host::smoothed_prolongation(F, agg, nc, M.amg_omega);
+        hsps_host::HostCsr R = hsps_host::transpose(P);
+        hsps_host::HostCsr Ac = hsps_host::spgemm(R, hsps_host::spgemm(F, P));
+        AmgLevelDev& cur = M.amg.back();
+        cur.agg = std::move(agg);
+        cur.P = to_dev(M.ctx, std::move(P));
+        cur.R = to_dev(M.ctx, std::move(R));
+        M.amg.emplace_back();
+        auto d = hsps_host::inv_diagonal(Ac);
+        M.amg.back().dinv64 = upload(M.ctx, d.data(), d.size() * 8);
+        M.amg.back().dinv32 = upload_f32(M.ctx, d);
+        hostA.push_back(std::move(Ac));
+    }
+    const int L = (int)M.amg.size();
+    for (int lv = 1; lv < L; ++lv) {
+        hsps_host::HostCsr h = hostA[lv];
+        M.amg[lv].A = to_dev(M.ctx, std::move(h));
+    }
+    for (int lv = 0; lv < L; ++lv) alloc_level_work(M.ctx, M.amg[lv], hostA[lv].rows);
+    // coarsest-level dense operator (amg.cpp:260-291)
+    auto G = hsps_host::coarse_operator(hostA.back());
+    M.coarse_n = hostA.back().rows;
+    M.coarse_inv64 = upload(M.ctx, G.data(), G.size() * 8);
+    M.coarse_inv32 = upload_f32(M.ctx, G);
+}
+
+// ---- apply -------------------------------------------------------------------------
+template <class T> struct Lv;
+template <> struct Lv<double> {
+    static double* x(AmgLevelDev& L) { return L.x64.as<double>(); }
+    static double* xb(AmgLevelDev& L) { return L.x64b.as<double>(); }
+    static double* rhs(AmgLevelDev& L) { return L.rhs64.as<double>(); }
+    static double* res(AmgLevelDev& L) { return L.res64.as<double>(); }
+    static const double* dinv(const AmgLevelDev& L) { return L.dinv64.as<double>(); }
+    static const double* G(const Precond& M) { return M.coarse_inv64.as<double>(); }
+};
+template <> struct Lv<float> {
+    static float* x(AmgLevelDev& L) { return L.x32.as<float>(); }
+    static float* xb(AmgLevelDev& L) { return L.x32b.as<float>(); }
+    static float* rhs(AmgLevelDev& L) { return L.rhs32.as<float>(); }
+    static float* res(AmgLevelDev& L) { return L.res32.as<float>(); }
+    static const float* dinv(const AmgLevelDev& L) { return L.dinv32.as<float>(); }
+    static const float* G(const Precond& M) { return M.coarse_inv32.as<float>(); }
+};
+
+static void smooth(const Csr& A, const double* d, double w, const double* rhs, const double* xi, double* xo,
+                   hipStream_t s, const int* skip) {
+    smooth_f64(A, d, w, rhs, xi, xo, s, skip);
+}
+static void smooth(const Csr& A, const float* d, double w, const float* rhs, const float* xi, float* xo,
+                   hipStream_t s, const int* skip) {
+    smooth_f32(A, d, (float)w, rhs, xi, xo, s, skip);
+}
+static void resid(const Csr& A, const double* b, const double* x, double* r, hipStream_t s, const int* skip) {
+    residual_f64(A, b, x, r, s, nullptr, nullptr, nullptr, skip);
+}
+static void resid(const Csr& A, const float* b, const float* x, float* r, hipStream_t s, const int* skip) {
+    residual_f32(A, b, x, r, s, skip);
+}
+static void fill_zero(int n, double* x, hipStream_t s, const int* skip) { fill_zero_f64(n, x, s, skip); }
+static void fill_zero(int n, float* x, hipStream_t s, const int* skip) { fill_zero_f32(n, x, s, skip); }
+static void mv(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip) { spmv_f64(A, x, y, s, skip); }
+static void mv(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip) { spmv_f32(A, x, y, s, skip); }
+static void mv_add(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip) {
+    spmv_add_f64(A, x, y, s, skip);
+}
+static void mv_add(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip) {
+    spmv_add_f32(A, x, y, s, skip);
+}
+
+// V-cycle (amg.cpp:296-335) on level lv: rhs -> x (x starts at zero).  `out`
+// is where the level's result must land (level 0: the caller's z).
+template <class T>
+static void vcycle(const Precond& Mc, const Csr& A0, int lv, const T* rhs, T* out, hipStream_t s, const int* skip) {
+    Precond& M = const_cast<Precond&>(Mc);
+    const int L = (int)M.amg.size();
+    AmgLevelDev& Lv_ = M.amg[lv];
+    const Csr& A = lv == 0 ? A0 : *Lv_.A;
+    const int n = A.rows;
+    if (lv == L - 1) {
+        hipLaunchKernelGGL(k_dense_gemv<T>, dim3((n + 3) / 4), dim3(kBlock), 0, s, n, Lv<T>::G(M), rhs, out, skip);
+        HSPS_CHECK_LAUNCH();
+        return;
+    }
+    const T* dinv = Lv<T>::dinv(Lv_);
+    // out-of-place sweeps after the first: (pre-1) + post; land the last in out
+    const int sweeps = (M.pre > 0 ? M.pre - 1 : 0) + M.post;
+    T* cur = (sweeps % 2 == 0) ? out : Lv<T>::xb(Lv_);
+    T* oth = (cur == out) ? Lv<T>::xb(Lv_) : out;
+    if (M.pre > 0) {
+        hipLaunchKernelGGL(k_first_sweep<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (T)M.amg_omega, dinv, rhs, cur,
+                           skip);
+        HSPS_CHECK_LAUNCH();
+        for (int k = 1; k < M.pre; ++k) {
+            smooth(A, dinv, M.amg_omega, rhs, cur, oth, s, skip);
+            std::swap(cur, oth);
+        }
+    } else {
+        fill_zero(n, cur, s, skip);
+    }
+    T* res = Lv<T>::res(Lv_);
+    resid(A, rhs, cur, res, s, skip);
+    AmgLevelDev& Cn = M.amg[lv + 1];
+    T* rc = Lv<T>::rhs(Cn);
+    T* ec = Lv<T>::x(Cn);
+    mv(*Lv_.R, res, rc, s, skip);
+    vcycle<T>(M, A0, lv + 1, rc, ec, s, skip);
+    mv_add(*Lv_.P, ec, cur, s, skip);
+    for (int k = 0; k < M.post; ++k) {
+        smooth(A, dinv, M.amg_omega, rhs, cur, oth, s, skip);
+        std::swap(cur, oth);
+    }
+}
+
+template <class T>
+static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int* skip) {
+    const int n = M.n;
+    switch (M.kind) {
+        case HSPS_PRECOND_NONE:
+            hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, z, skip);
+            HSPS_CHECK_LAUNCH();
+            break;
+        case HSPS_PRECOND_JACOBI: {
+            const T* d = std::is_same<T, double>::value ? (const T*)M.dinv64.p : (const T*)M.dinv32.p;
+            hipLaunchKernelGGL(k_diag_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, z, skip);
+            HSPS_CHECK_LAUNCH();
+            break;
+        }
+        case HSPS_PRECOND_ILU: {
+            const IluFactor& f = *M.ilu;
+            const bool dbl = std::is_same<T, double>::value;
+            T* y = dbl ? (T*)f.tmp64.p : (T*)f.tmp32.p;
+            const T* invd = dbl ? (const T*)f.inv_diag64.p : (const T*)f.inv_diag32.p;
+            if (dbl) {
+                hipLaunchKernelGGL((k_tri_lower<double, T>), dim3(1), dim3(kTriThreads), 0, s, f.fwd.n_levels,
+                                   f.fwd.level_ptr.as<int>(), f.fwd.order.as<int>(), f.lu->rp.as<int>(),
+                                   f.lu->ci.as<int>(), f.lu->v64.as<double>(), r, y, skip);
+                HSPS_CHECK_LAUNCH();
+                hipLaunchKernelGGL((k_tri_upper<double, T>), dim3(1), dim3(kTriThreads), 0, s, f.bwd.n_levels,
+                                   f.bwd.level_ptr.as<int>(), f.bwd.order.as<int>(), f.lu->rp.as<int>(),
+                                   f.lu->ci.as<int>(), f.lu->v64.as<double>(), invd, (const T*)y, z, skip);
+                HSPS_CHECK_LAUNCH();
+            } else {
+                hipLaunchKernelGGL((k_tri_lower<float, T>), dim3(1), dim3(kTriThreads), 0, s, f.fwd.n_levels,
+                                   f.fwd.level_ptr.as<int>(), f.fwd.order.as<int>(), f.lu->rp.as<int>(),
+                                   f.lu->ci.as<int>(), f.lu->v32.as<float>(), r, y, skip);
+                HSPS_CHECK_LAUNCH();
+                hipLaunchKernelGGL((k_tri_upper<float, T>), dim3(1), dim3(kTriThreads), 0, s, f.bwd.n_levels,
+                                   f.bwd.level_ptr.as<int>(), f.bwd.order.as<int>(), f.lu->rp.as<int>(),
+                                   f.lu->ci.as<int>(), f.lu->v32.as<float>(), invd, (const T*)y, z, skip);
+                HSPS_CHECK_LAUNCH();
+            }
+            break;
+        }
+        case HSPS_PRECOND_AMG:
+            vcycle<T>(M, *M.A0, 0, r, z, s, skip);
+            break;
+        default:
+            throw Error(HSPS_ERR_INVALID, "unknown preconditioner kind");
+    }
+}
+
+void precond_apply_f64(const Precond& M, const double* r, double* z, hipStream_t s, const int* skip) {
+    apply_t<double>(M, r, z, s, skip);
+}
+void precond_apply_f32(const Precond& M, const float* r, float* z, hipStream_t s, const int* skip) {
+    apply_t<float>(M, r, z, s, skip);
+}
+int precond_kind(const Precond& M) { return M.kind; }
+const double* precond_jacobi_dinv64(const Precond& M) { return M.dinv64.as<double>(); }
+const float* precond_jacobi_dinv32(const Precond& M) { return M.dinv32.as<float>(); }
+
+}  // namespace hsps_dev
+
+using namespace hsps_dev;
+
+extern "C" {
+
+int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precond* out, double* setup_seconds) {
+    return guarded([&] {
+        if (!A || !out) throw Error(HSPS_ERR_INVALID, "hsps_precond_create: null argument");
+        if (A->rows != A->cols) throw Error(HSPS_ERR_SIZE, "preconditioner needs a square matrix");
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        const auto t0 = std::chrono::steady_clock::now();
+        A->ctx->set_current();
+        auto* M = new hsps_precond_s;
+        M->kind = kind;
+        M->ctx = A->ctx;
+        M->n = A->rows;
+        M->A0 = A;
+        try {
+            switch (kind) {
+                case HSPS_PRECOND_NONE: break;
+                case HSPS_PRECOND_JACOBI: setup_jacobi(*M, *A, p.jacobi_omega); break;
+                case HSPS_PRECOND_ILU: setup_ilu(*M, *A, p.drop_tol, p.fill_per_row); break;
+                case HSPS_PRECOND_AMG: setup_amg(*M, *A, p); break;
+                default: throw Error(HSPS_ERR_INVALID, "SolverFactory: unknown PrecondType");
+            }
+            HSPS_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete M;
+            throw;
+        }
+        if (setup_seconds)
+            *setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = M;
+        return HSPS_OK;
+    });
+}
+
+int hsps_precond_destroy(hsps_precond M) {
+    return guarded([&] {
+        if (M) {
+            (void)hipSetDevice(M->ctx->device);
+            (void)hipStreamSynchronize(M->ctx->stream);
+            delete M;
+        }
+        return HSPS_OK;
+    });
+}
+
+int hsps_precond_kind(hsps_precond M) { return M ? M->kind : HSPS_ERR_INVALID; }
+
+int hsps_precond_apply_f64(hsps_precond M, const double* r, double* z) {
+    return guarded([&] {
+        if (!M || !r || !z) throw Error(HSPS_ERR_INVALID, "hsps_precond_apply_f64: null argument");
+        M->ctx->set_current();
+        precond_apply_f64(*M, r, z, M->ctx->stream, nullptr);
+        return HSPS_OK;
+    });
+}
+
+int hsps_amg_levels(hsps_precond M) {
+    if (!M || M->kind != HSPS_PRECOND_AMG) return HSPS_ERR_INVALID;
+    return (int)M->amg.size();
+}
+
+static const Csr& amg_A(const Precond& M, int lv) { return lv == 0 ? *M.A0 : *M.amg[lv].A; }
+
+int hsps_amg_level_info(hsps_precond M, int lv, int* rows, int* nnz) {
+    return guarded([&] {
+        if (!M || M->kind != HSPS_PRECOND_AMG || lv < 0 || lv >= (int)M->amg.size())
+            throw Error(HSPS_ERR_INVALID, "hsps_amg_level_info: bad level");
+        const Csr& A = amg_A(*M, lv);
+        if (rows) *rows = A.rows;
+        if (nnz) *nnz = A.nnz;
+        return HSPS_OK;
+    });
+}
+
+int hsps_amg_aggregates(hsps_precond M, int lv, int* out) {
+    return guarded([&] {
+        if (!M || M->kind != HSPS_PRECOND_AMG || lv < 0 || lv + 1 >= (int)M->amg.size() || !out)
+            throw Error(HSPS_ERR_INVALID, "hsps_amg_aggregates: bad level");
+        const auto& a = M->amg[lv].agg;
+        std::copy(a.begin(), a.end(), out);
+        return HSPS_OK;
+    });
+}
+
+int hsps_amg_level_matrix(hsps_precond M, int lv, int which, int* rp, int* ci, double* v) {
+    return guarded([&] {
+        if (!M || M->kind != HSPS_PRECOND_AMG || lv < 0 || lv >= (int)M->amg.size())
+            throw Error(HSPS_ERR_INVALID, "hsps_amg_level_matrix: bad level");
+        if (which != 0 && lv + 1 >= (int)M->amg.size())
+            throw Error(HSPS_ERR_INVALID, "hsps_amg_level_matrix: coarsest level has no P/R");
+        const Csr& A = which == 0 ? amg_A(*M, lv) : which == 1 ? *M->amg[lv].P : *M->amg[lv].R;
+        std::copy(A.h_rp.begin(), A.h_rp.end(), rp);
+        std::copy(A.h_ci.begin(), A.h_ci.end(), ci);
+        std::copy(A.h_v.begin(), A.h_v.end(), v);
+        return HSPS_OK;
+    });
+}
+
+}  // extern "C"

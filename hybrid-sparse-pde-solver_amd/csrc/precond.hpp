@@ -1,0 +1,59 @@
+// =============================================================================
+// precond.hpp — device preconditioner objects (Jacobi, ILU(0)/ILUT, SA-AMG).
+// =============================================================================
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "hsps_internal.hpp"
+
+namespace hsps_dev {
+
+// Level-scheduled triangular factor: rows grouped into dependency levels
+// (host-computed), stored as a row permutation + level offsets.
+struct TriSchedule {
+    int n_levels = 0;
+    std::vector<int> h_level_ptr;  // n_levels+1 offsets into order
+    DevBuf order;                  // rows sorted by level
+    DevBuf level_ptr;
+    int max_level_width = 0;
+};
+
+struct IluFactor {
+    std::unique_ptr<Csr> lu;  // L\U in one CSR (unit L implicit)
+    DevBuf diag_pos;          // position of U(i,i) in lu (or -1)
+    DevBuf inv_diag64, inv_diag32;
+    TriSchedule fwd, bwd;
+    DevBuf tmp64, tmp32;      // y of L y = r
+};
+
+struct AmgLevelDev {
+    std::unique_ptr<Csr> A, P, R;
+    DevBuf dinv64, dinv32;
+    std::vector<int> agg;  // host aggregate map (levels < L-1)
+    // per-level work vectors (FP64 and FP32 views)
+    DevBuf x64, x64b, rhs64, res64;
+    DevBuf x32, x32b, rhs32, res32;
+};
+
+struct Precond {
+    int kind = HSPS_PRECOND_NONE;
+    Ctx* ctx = nullptr;
+    int n = 0;
+    const Csr* A0 = nullptr;  // the operator it was built for (owned by the caller)
+    // Jacobi
+    DevBuf dinv64, dinv32;
+    // ILU
+    std::unique_ptr<IluFactor> ilu;
+    // AMG
+    std::vector<AmgLevelDev> amg;
+    double amg_omega = 0.67;
+    int pre = 2, post = 2;
+    int coarse_n = 0;
+    DevBuf coarse_inv64, coarse_inv32;  // dense inverse of the coarsest operator (row-major)
+};
+
+}  // namespace hsps_dev
+
+struct hsps_precond_s : hsps_dev::Precond {};

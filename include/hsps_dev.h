@@ -1,0 +1,188 @@
+/*
+ * hsps_dev.h — C ABI of the MI355X (gfx950) solve path for HAMLSS.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No
+ * exception crosses this boundary; on a negative status, hsps_last_error()
+ * returns a thread-local message.  The C++17 host layer (hsps::SparseMatrix,
+ * SolverBase, PreconditionerBase, AdaptiveSelector) is written on top of it
+ * and rethrows the reference's exception types.
+ *
+ * Memory: vector arguments named d_* are DEVICE pointers (hsps_malloc or any
+ * HIP allocation on the context's device).  Matrix creation takes HOST CSR
+ * arrays and uploads them.  All work is enqueued on the context's stream;
+ * calls that return host scalars synchronise that stream.
+ *
+ * Each entry point cites the reference interface it replaces (paths relative
+ * to the reference repository).
+ */
+#ifndef HSPS_DEV_H
+#define HSPS_DEV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (errors are negative) -------------------------------- */
+enum {
+    HSPS_OK = 0,
+    HSPS_ERR_INVALID = -1,    /* bad argument / handle (std::invalid_argument) */
+    HSPS_ERR_SIZE = -2,       /* size mismatch (vector.cpp:41-42, sparse_matrix.cpp:69-72) */
+    HSPS_ERR_ZERO_DIAG = -3,  /* Jacobi zero diagonal (jacobi.cpp:28-30) */
+    HSPS_ERR_PIVOT = -4,      /* ILU near-zero pivot (ilu.cpp:65-67, 257-259) */
+    HSPS_ERR_HIP = -5,        /* HIP runtime failure */
+    HSPS_ERR_NOMEM = -6,      /* device allocation failed / over budget */
+    HSPS_ERR_UNSUPPORTED = -7 /* feature not available for this configuration */
+};
+
+/* precision of the Krylov inner loop */
+enum { HSPS_FP64 = 0, HSPS_MIXED = 1 };
+/* preconditioner kinds: PrecondType (types.hpp:40-47) */
+enum { HSPS_PRECOND_NONE = 0, HSPS_PRECOND_JACOBI = 1, HSPS_PRECOND_ILU = 2, HSPS_PRECOND_AMG = 3 };
+/* solver kinds: SolverType (types.hpp:32-38) */
+enum { HSPS_SOLVER_CG = 0, HSPS_SOLVER_FGMRES = 1 };
+
+typedef struct hsps_ctx_s* hsps_ctx;
+typedef struct hsps_csr_s* hsps_csr;
+typedef struct hsps_precond_s* hsps_precond;
+typedef struct hsps_batch_s* hsps_batch;
+
+/* SolverParams, types.hpp:119-143 (fields on the solve path), plus the
+ * device-only controls at the end. */
+typedef struct hsps_params {
+    double tol;            /* relative residual tolerance (1e-8) */
+    int max_iter;          /* max total Krylov iterations (2000) */
+    int restart_size;      /* FGMRES m (50) */
+    double drop_tol;       /* ILUT tau; 0 selects ILU(0) (1e-4) */
+    int fill_per_row;      /* ILUT extra fill p (0) */
+    int amg_levels;        /* max AMG levels (10) */
+    double amg_strong;     /* strength threshold theta (0.25) */
+    int amg_smooth_pre;    /* (2) */
+    int amg_smooth_post;   /* (2) */
+    double amg_omega;      /* Jacobi damping used by SA smoothing and smoother (0.67, amg.hpp:101) */
+    double jacobi_omega;   /* Jacobi preconditioner damping (1.0, jacobi.hpp:33) */
+    int precision;         /* HSPS_FP64 or HSPS_MIXED */
+    double reliable_delta; /* mixed CG: recompute the FP64 true residual when
+                              |r| drops below delta * max|r| since the last one (0.1) */
+    int check_every;       /* host polls the device convergence flag every k iterations (32) */
+} hsps_params;
+
+/* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64
+ * true residual the device path always reports. */
+typedef struct hsps_stats {
+    int iterations;
+    int converged;
+    int restarts;
+    int reliable_updates;  /* mixed CG: FP64 residual replacements */
+    double initial_residual;
+    double final_residual; /* the solver's own (recursive / Givens) estimate, relative */
+    double true_residual;  /* ||b - A x||_2 / ||b||_2 recomputed in FP64 at exit */
+    double solve_time_s;
+    double setup_time_s;
+    long long flop_count;  /* flop_model, energy_monitor.hpp:163-177 */
+    long long mem_bytes;
+    double energy_joules;  /* 2e-10 J/FLOP + 5e-9 J/B proxy (cg_solver.cpp:186-188) */
+} hsps_stats;
+
+/* ---- errors / defaults ------------------------------------------------- */
+const char* hsps_last_error(void);
+/* Fills the reference defaults of SolverParams (types.hpp:119-143). */
+void hsps_params_default(hsps_params* p);
+
+/* ---- context (replaces ParallelContext/backend selection,
+ *      parallel_context.cpp:31-50; one per device + stream) --------------- */
+int hsps_ctx_create(int device, hsps_ctx* out);
+int hsps_ctx_destroy(hsps_ctx ctx);
+/* Use an existing hipStream_t (NULL = the context's own stream). */
+int hsps_ctx_set_stream(hsps_ctx ctx, void* hip_stream);
+void* hsps_ctx_stream(hsps_ctx ctx);
+int hsps_ctx_sync(hsps_ctx ctx);
+/* Bytes of device memory currently held by handles of this context. */
+size_t hsps_ctx_bytes_in_use(hsps_ctx ctx);
+
+/* ---- device memory (replaces DeviceVector, cuda_backend.cpp:26-83) ------ */
+int hsps_malloc(hsps_ctx ctx, size_t bytes, void** d_ptr);
+int hsps_free(hsps_ctx ctx, void* d_ptr);
+int hsps_memcpy_h2d(hsps_ctx ctx, void* d_dst, const void* h_src, size_t bytes);
+int hsps_memcpy_d2h(hsps_ctx ctx, void* h_dst, const void* d_src, size_t bytes);
+int hsps_memcpy_d2d(hsps_ctx ctx, void* d_dst, const void* d_src, size_t bytes);
+int hsps_memset(hsps_ctx ctx, void* d_dst, int value, size_t bytes);
+
+/* ---- CSR matrix (SparseMatrix, sparse_matrix.hpp:34-170; int32 indices,
+ *      sorted columns).  Uploads FP64 values, an FP32 copy, the diagonal and
+ *      the SpMV row-block schedule. -------------------------------------- */
+int hsps_csr_create(hsps_ctx ctx, int rows, int cols, int nnz, const int* h_row_ptr,
+                    const int* h_col_idx, const double* h_values, hsps_csr* out);
+int hsps_csr_destroy(hsps_csr A);
+int hsps_csr_info(hsps_csr A, int* rows, int* cols, int* nnz);
+/* Replace the values (same pattern) — re-uploads FP64/FP32 values and diagonal. */
+int hsps_csr_update_values(hsps_csr A, const double* h_values);
+
+/* ---- SpMV (SparseMatrix::spmv, sparse_matrix.cpp:68-87;
+ *      cusparseSpMV, cuda_backend.cpp:379-411) -------------------------- */
+int hsps_spmv_f64(hsps_csr A, const double* d_x, double* d_y);
+int hsps_spmv_f32(hsps_csr A, const float* d_x, float* d_y);
+/* r = b - A x in FP64 (cg_solver.cpp:52-54, solver_base.cpp:8-15). */
+int hsps_residual_f64(hsps_csr A, const double* d_b, const double* d_x, double* d_r);
+
+/* ---- BLAS-1 (Vector, vector.cpp:39-119; cublasD*, cuda_backend.cpp:297-376).
+ *      Reductions accumulate in FP64 and return to the host. -------------- */
+int hsps_dot_f64(hsps_ctx ctx, int n, const double* d_x, const double* d_y, double* h_out);
+int hsps_norm2_f64(hsps_ctx ctx, int n, const double* d_x, double* h_out);
+int hsps_axpy_f64(hsps_ctx ctx, int n, double alpha, const double* d_x, double* d_y);   /* y += a x */
+int hsps_axpby_f64(hsps_ctx ctx, int n, double a, const double* d_x, double b, double* d_y); /* y = a x + b y */
+int hsps_scale_f64(hsps_ctx ctx, int n, double alpha, double* d_x);
+
+/* ---- preconditioners (PreconditionerBase, preconditioner_base.hpp:16-48;
+ *      JacobiPreconditioner jacobi.cpp:13-51; ILUPreconditioner ilu.cpp:31-271;
+ *      AMGPreconditioner amg.cpp:183-343).  setup_seconds may be NULL. ---- */
+int hsps_precond_create(hsps_csr A, int kind, const hsps_params* p, hsps_precond* out,
+                        double* setup_seconds);
+int hsps_precond_destroy(hsps_precond M);
+int hsps_precond_kind(hsps_precond M);
+/* z = M^-1 r in FP64 (z is overwritten; AMG starts from z = 0, amg.cpp:340-343). */
+int hsps_precond_apply_f64(hsps_precond M, const double* d_r, double* d_z);
+/* AMG introspection: number of levels; rows/nnz of level lv; the host
+ * aggregate map of level lv (length rows(lv); lv < levels-1). */
+int hsps_amg_levels(hsps_precond M);
+int hsps_amg_level_info(hsps_precond M, int lv, int* rows, int* nnz);
+int hsps_amg_aggregates(hsps_precond M, int lv, int* h_out);
+/* Copy level lv's operator (which=0 A, 1 P, 2 R) to host CSR arrays sized by
+ * hsps_amg_level_info / the P,R shapes (rows(lv) x rows(lv+1)). */
+int hsps_amg_level_matrix(hsps_precond M, int lv, int which, int* h_row_ptr, int* h_col_idx,
+                          double* h_values);
+
+/* ---- Krylov solves (SolverBase::solve, solver_base.hpp:32-35;
+ *      CGSolver::solve cg_solver.cpp:37-191; FGMRESSolver::solve
+ *      fgmres_solver.cpp:165-241).  d_x is the initial guess on entry and the
+ *      solution on exit.  M may be NULL (unpreconditioned).  Returns 1 when
+ *      converged, 0 when not, <0 on error. ------------------------------ */
+int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x, const hsps_params* p,
+                  hsps_stats* stats);
+int hsps_fgmres_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x,
+                      const hsps_params* p, hsps_stats* stats);
+
+/* ---- matrix features for the adaptive ladder
+ *      (AdaptiveSelector::extract_features, adaptive_selector.cpp:38-220).
+ *      out[0..7] = n, nnz, diag_dominance, symmetry_residual, frobenius,
+ *      estimated_cond, lambda_max, is_spd.  The 30-step Lanczos runs on the
+ *      device; the O(nnz) scans run on the host copy. ------------------- */
+int hsps_features(hsps_csr A, double* h_out);
+
+/* ---- batched solves (no reference counterpart: many independent systems in
+ *      one launch).  Systems are concatenated: system s owns rows
+ *      [row_off[s], row_off[s+1]) and its columns index inside that range
+ *      (block-diagonal).  CG + Jacobi, FP64 or mixed with FP64 refinement. -- */
+int hsps_batch_create(hsps_ctx ctx, int n_systems, const int* h_row_off, const int* h_row_ptr,
+                      const int* h_col_idx, const double* h_values, hsps_batch* out);
+int hsps_batch_destroy(hsps_batch B);
+/* d_b, d_x: concatenated FP64 vectors (length row_off[n_systems]).
+ * h_iters/h_converged/h_true_res: per-system results (may be NULL). */
+int hsps_batch_solve(hsps_batch B, const double* d_b, double* d_x, const hsps_params* p,
+                     int* h_iters, int* h_converged, double* h_true_res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSPS_DEV_H */

@@ -1,0 +1,223 @@
+"""GPU parity: the gfx950 path (through the C ABI) against the FP64 CPU oracle
+and the reference's fixtures.
+
+Tolerances
+  SpMV FP64: |y - y_ref|_i <= 1e-13 * (|A||x|)_i   (summation order / FMA only)
+  SpMV FP32: |y - y_ref|_i <= 2e-6 * (|A||x|)_i    (FP32 values and vectors)
+  solves:    true FP64 rel-res <= tol; solution within 1e-6 relative of the
+             oracle's; iteration counts within 5% of the reference's.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CV, SV = 0.7648421872844885, 0.644217687237691  # cos 0.7, sin 0.7
+
+
+def dev_csr(H, ctx, A):
+    rp, ci, v = A.arrays()
+    r, c, _ = A.shape_nnz()
+    return H.Csr(ctx, r, c, rp, ci, v)
+
+
+def random_csr(rng, n, m, row_lens):
+    rows, cols, vals = [], [], []
+    for i, L in enumerate(row_lens):
+        cs = rng.choice(m, size=min(L, m), replace=False)
+        rows += [i] * len(cs)
+        cols += list(cs)
+        vals += list(rng.uniform(-1, 1, len(cs)))
+    return O.Csr.from_coo(n, m, rows, cols, vals)
+
+
+def absAx(A, x):
+    rp, ci, v = A.arrays()
+    Aabs = O.Csr.from_csr(A.shape_nnz()[0], A.shape_nnz()[1], rp, ci, np.abs(v))
+    return Aabs.spmv(np.abs(x))
+
+
+@pytest.mark.parametrize("case", ["poisson64", "convdiff48", "mixed_rows", "long_rows", "tiny", "empty_rows"])
+def test_spmv_matches_oracle(H, ctx, case):
+    rng = np.random.default_rng(11)
+    if case == "poisson64":
+        A = O.Csr.poisson(64)
+    elif case == "convdiff48":
+        A = O.Csr.convdiff(48, 0.01, CV, SV)
+    elif case == "mixed_rows":  # row lengths 0..300: LDS groups of 1..64 lanes per row
+        A = random_csr(rng, 3000, 5000, rng.integers(0, 300, 3000))
+    elif case == "long_rows":  # rows longer than one LDS stage (2048)
+        A = random_csr(rng, 40, 20000, [5000, 3, 2049, 0, 4096] + [7] * 35)
+    elif case == "tiny":
+        A = O.Csr.from_coo(1, 1, [0], [0], [2.0])
+    else:
+        A = O.Csr.from_coo(6, 6, [0, 2, 5], [1, 2, 0], [1.0, -2.0, 3.0])
+    n, m, _ = A.shape_nnz()
+    x = rng.uniform(-1, 1, m)
+    Ad = dev_csr(H, ctx, A)
+    xd, yd = ctx.array(x), ctx.array(n=n)
+    Ad.spmv(xd, yd)
+    y = yd.download()
+    yref = A.spmv(x)
+    bound = 1e-13 * absAx(A, x) + 1e-300
+    assert np.all(np.abs(y - yref) <= bound), np.max(np.abs(y - yref) / bound)
+    # FP32 path
+    x32, y32 = ctx.array(x.astype(np.float32), dtype=np.float32), ctx.array(n=n, dtype=np.float32)
+    Ad.spmv(x32, y32, precision=H.HSPS_MIXED)
+    y2 = y32.download().astype(np.float64)
+    assert np.all(np.abs(y2 - yref) <= 2e-6 * absAx(A, x) + 1e-30)
+
+
+def test_spmv_poisson_closed_form(H, ctx):
+    """A*1 for unscaled Poisson: 4 - (#neighbours) per row (0 interior)."""
+    n = 256
+    A = O.Csr.poisson(n)
+    Ad = dev_csr(H, ctx, A)
+    xd, yd = ctx.array(np.ones(n * n)), ctx.array(n=n * n)
+    Ad.spmv(xd, yd)
+    y = yd.download().reshape(n, n)
+    expect = np.zeros((n, n))
+    expect[0, :] += 1
+    expect[-1, :] += 1
+    expect[:, 0] += 1
+    expect[:, -1] += 1
+    np.testing.assert_array_equal(y, expect)
+
+
+def test_blas1_exact_small_integers(H, ctx):
+    x = np.arange(1000, dtype=np.float64) % 7 - 3
+    y = np.arange(1000, dtype=np.float64) % 5 - 2
+    xd, yd = ctx.array(x), ctx.array(y)
+    import ctypes as C
+    out = C.c_double()
+    H.check(H.lib().hsps_dot_f64(ctx.h, 1000, xd.ptr, yd.ptr, C.byref(out)))
+    assert out.value == float(np.dot(x, y))
+    H.check(H.lib().hsps_axpy_f64(ctx.h, 1000, 2.0, xd.ptr, yd.ptr))
+    np.testing.assert_array_equal(yd.download(), y + 2 * x)
+    H.check(H.lib().hsps_axpby_f64(ctx.h, 1000, 1.0, xd.ptr, -1.0, yd.ptr))
+    np.testing.assert_array_equal(yd.download(), x - (y + 2 * x))
+    H.check(H.lib().hsps_norm2_f64(ctx.h, 1000, xd.ptr, C.byref(out)))
+    assert abs(out.value - np.linalg.norm(x)) <= 1e-14 * np.linalg.norm(x)
+
+
+def _solve_dev(H, ctx, A, b, solver, pre, **kw):
+    Ad = dev_csr(H, ctx, A)
+    p = H.default_params(**kw)
+    M = H.Precond(Ad, pre, p) if pre != "none" else None
+    bd, xd = ctx.array(b), ctx.array(n=len(b))
+    ok, st = H.solve(Ad, bd, xd, solver, M, p)
+    return ok, st, xd.download()
+
+
+def _rel(A, b, x):
+    return np.linalg.norm(b - A.spmv(x)) / np.linalg.norm(b)
+
+
+def test_readme_table_on_gpu(H, ctx, golden):
+    """README.md:195-199 iteration counts, FP64 device path."""
+    A = O.Csr.poisson(64)
+    b = np.ones(4096)
+    ref = golden["readme_b1_64"]
+    for solver, pre, key in [("cg", "jacobi", "cg_jacobi"), ("cg", "ilu", "cg_ilu0"),
+                             ("fgmres", "jacobi", "fgmres50_jacobi"), ("fgmres", "ilu", "fgmres50_ilu0"),
+                             ("fgmres", "amg", "fgmres50_amg")]:
+        ok, st, x = _solve_dev(H, ctx, A, b, solver, pre, tol=1e-8, max_iter=2000, restart_size=50, drop_tol=0.0)
+        assert ok, key
+        it = ref[key]["iters"]
+        assert abs(st.iterations - it) <= max(1, 0.05 * it), (key, st.iterations, it)
+        assert st.true_residual < 1e-8
+        assert abs(np.linalg.norm(x) - ref[key]["x_norm"]) <= 1e-6 * ref[key]["x_norm"], key
+
+
+@pytest.mark.parametrize("idx", range(6))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_seeded_solves(H, ctx, golden, idx, precision):
+    c = golden["seeded"][idx]
+    A = O.Csr.poisson(c["n"]) if c["kind"] == "poisson" else O.Csr.convdiff(c["n"], c["eps"], c["bx"], c["by"])
+    xs = O.uniform(c["seed"], A.rows)
+    b = A.spmv(xs)
+    for key, solver, pre in [("cg_jacobi", "cg", "jacobi"), ("fgmres30_jacobi", "fgmres", "jacobi"),
+                             ("fgmres30_ilu0", "fgmres", "ilu"), ("fgmres30_amg", "fgmres", "amg")]:
+        if key not in c:
+            continue
+        ref = c[key]
+        ok, st, x = _solve_dev(H, ctx, A, b, solver, pre, tol=1e-8, max_iter=20000, restart_size=30, drop_tol=0.0,
+                               precision=precision)
+        assert ok, (key, precision)
+        assert _rel(A, b, x) < 1e-8
+        xo, sto = O.solve(A, b, solver, pre, tol=1e-8, max_iter=20000, restart_size=30, drop_tol=0.0)
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), (key, precision)
+        it = ref["iters"]
+        slack = 0.05 if precision == "fp64" else 0.10
+        assert abs(st.iterations - it) <= max(1, slack * it), (key, precision, st.iterations, it)
+
+
+def test_amg_hierarchy_bit_exact(H, ctx, golden):
+    for c in golden["amg"]:
+        name = c["case"]
+        if not name.startswith("poisson") or name == "poisson180":
+            continue
+        n = int(name[7:].split("_")[0])
+        A = O.Csr.poisson(n)
+        Ad = dev_csr(H, ctx, A)
+        M = H.Precond(Ad, "amg", H.default_params(amg_strong=c["theta"]))
+        assert M.amg_levels() == len(c["levels"]), name
+        oa = O.Amg(A, amg_strong=c["theta"])
+        for lv, L in enumerate(c["levels"]):
+            assert M.amg_level_info(lv) == (L["rows"], L["nnz"]), (name, lv)
+            if lv + 1 < len(c["levels"]):
+                np.testing.assert_array_equal(M.amg_aggregates(lv), oa.aggregates(lv))
+
+
+def test_amg_vcycle_matches_oracle(H, ctx):
+    A = O.Csr.poisson(48)
+    Ad = dev_csr(H, ctx, A)
+    M = H.Precond(Ad, "amg")
+    r = O.uniform(5, A.rows)
+    rd, zd = ctx.array(r), ctx.array(n=A.rows)
+    M.apply(rd, zd)
+    z = zd.download()
+    zo = O.Amg(A).apply(r)
+    assert np.linalg.norm(z - zo) <= 1e-10 * np.linalg.norm(zo)
+
+
+def test_ilu_apply_matches_oracle(H, ctx):
+    for A, tau in [(O.Csr.convdiff(24, 0.05, 1.0, 0.5), 0.0), (O.Csr.convdiff(16, 0.1, 5.0, 0.0), 1e-4)]:
+        Ad = dev_csr(H, ctx, A)
+        M = H.Precond(Ad, "ilu", H.default_params(drop_tol=tau))
+        r = O.uniform(3, A.rows)
+        rd, zd = ctx.array(r), ctx.array(n=A.rows)
+        M.apply(rd, zd)
+        zo = O.precond_apply(A, r, "ilu", drop_tol=tau)
+        assert np.linalg.norm(zd.download() - zo) <= 1e-12 * np.linalg.norm(zo)
+
+
+def test_reference_unit_known_answers(H, ctx):
+    """tests/test_solvers.cpp:42-76, 151-167 of the reference."""
+    A = O.Csr.from_coo(1, 1, [0], [0], [2.0])
+    ok, st, x = _solve_dev(H, ctx, A, np.array([4.0]), "cg", "jacobi", tol=1e-9, max_iter=3000)
+    assert ok and abs(x[0] - 2.0) < 1e-10
+    D = O.Csr.from_coo(4, 4, [0, 1, 2, 3], [0, 1, 2, 3], [2.0, 3.0, 5.0, 7.0])
+    ok, st, x = _solve_dev(H, ctx, D, np.array([4.0, 9.0, 15.0, 21.0]), "cg", "jacobi", tol=1e-9)
+    np.testing.assert_allclose(x, [2, 3, 3, 3], atol=1e-10)
+    D3 = O.Csr.from_coo(3, 3, [0, 1, 2], [0, 1, 2], [3.0, 6.0, 9.0])
+    ok, st, x = _solve_dev(H, ctx, D3, np.array([3.0, 6.0, 9.0]), "fgmres", "jacobi", tol=1e-9, restart_size=40)
+    assert ok
+    np.testing.assert_allclose(x, [1, 1, 1], atol=1e-10)
+    # zero RHS: converged, 0 iterations (test_solvers.cpp:117-129)
+    P = O.Csr.poisson(8)
+    ok, st, x = _solve_dev(H, ctx, P, np.zeros(64), "cg", "jacobi")
+    assert ok and st.iterations == 0
+
+
+def test_errors_as_status_codes(H, ctx):
+    A = O.Csr.from_coo(3, 3, [0, 1, 2], [0, 1, 2], [1.0, 0.0, 2.0])
+    Ad = dev_csr(H, ctx, A)
+    with pytest.raises(H.HspsError) as e:
+        H.Precond(Ad, "jacobi")
+    assert e.value.code == -3 and "zero diagonal at row 1" in str(e.value)
+    with pytest.raises(H.HspsError) as e:
+        H.Precond(Ad, "ilu", H.default_params(drop_tol=0.0))
+    assert e.value.code == -4
