@@ -271,6 +271,7 @@ void hsps_params_default(hsps_params* p) {
     p->precision = HSPS_FP64;
     p->reliable_delta = 0.1;
     p->check_every = 32;
+    p->batch_tile_rows = 0;
 }
 
 int hsps_ctx_create(int device, hsps_ctx* out) {

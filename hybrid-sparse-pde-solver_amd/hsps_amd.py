@@ -24,7 +24,7 @@ class Params(C.Structure):
                 ("drop_tol", C.c_double), ("fill_per_row", C.c_int), ("amg_levels", C.c_int),
                 ("amg_strong", C.c_double), ("amg_smooth_pre", C.c_int), ("amg_smooth_post", C.c_int),
                 ("amg_omega", C.c_double), ("jacobi_omega", C.c_double), ("precision", C.c_int),
-                ("reliable_delta", C.c_double), ("check_every", C.c_int)]
+                ("reliable_delta", C.c_double), ("check_every", C.c_int), ("batch_tile_rows", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -290,6 +290,68 @@ class Precond:
                                           v.ctypes.data_as(_dp)))
         z = rp[-1]
         return rp, ci[:z], v[:z]
+
+
+def concat_systems(systems):
+    """Concatenate per-system CSRs (local column indices) into the batched
+    block-diagonal layout: row_off, global row_ptr, global col_idx, values."""
+    sizes = [len(rp) - 1 for rp, _, _ in systems]
+    row_off = np.zeros(len(systems) + 1, np.int64)
+    np.cumsum(sizes, out=row_off[1:])
+    nnz = [len(v) for _, _, v in systems]
+    nz_off = np.zeros(len(systems) + 1, np.int64)
+    np.cumsum(nnz, out=nz_off[1:])
+    rp = np.empty(row_off[-1] + 1, np.int64)
+    ci = np.empty(nz_off[-1], np.int32)
+    v = np.empty(nz_off[-1], np.float64)
+    for s, (srp, sci, sv) in enumerate(systems):
+        rp[row_off[s]:row_off[s + 1]] = np.asarray(srp[:-1], np.int64) + nz_off[s]
+        ci[nz_off[s]:nz_off[s + 1]] = np.asarray(sci, np.int64) + row_off[s]
+        v[nz_off[s]:nz_off[s + 1]] = sv
+    rp[-1] = nz_off[-1]
+    return row_off.astype(np.int32), rp.astype(np.int32), ci, v
+
+
+class Batch:
+    """Many independent systems solved in one launch (hsps_batch_*)."""
+
+    def __init__(self, ctx, row_off, rp, ci, v):
+        self.ctx = ctx
+        row_off = np.ascontiguousarray(row_off, np.int32)
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        h = _vp()
+        check(lib().hsps_batch_create(ctx.h, len(row_off) - 1, row_off.ctypes.data_as(_ip), rp.ctypes.data_as(_ip),
+                                      ci.ctypes.data_as(_ip), v.ctypes.data_as(_dp), C.byref(h)))
+        self.h = h
+        self.n_systems = len(row_off) - 1
+        self.n = int(row_off[-1])
+        self.row_off = row_off
+
+    @classmethod
+    def from_systems(cls, ctx, systems):
+        return cls(ctx, *concat_systems(systems))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hsps_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, x, params=None):
+        p = params or default_params()
+        it = np.zeros(self.n_systems, np.int32)
+        cv = np.zeros(self.n_systems, np.int32)
+        tr = np.zeros(self.n_systems, np.float64)
+        rc = check(lib().hsps_batch_solve(self.h, b.ptr, x.ptr, C.byref(p), it.ctypes.data_as(_ip),
+                                          cv.ctypes.data_as(_ip), tr.ctypes.data_as(_dp)))
+        return bool(rc), it, cv, tr
 
 
 def solve(A, b, x, solver="cg", M=None, params=None):

@@ -66,6 +66,7 @@ typedef struct hsps_params {
     double reliable_delta; /* mixed CG: recompute the FP64 true residual when
                               |r| drops below delta * max|r| since the last one (0.1) */
     int check_every;       /* host polls the device convergence flag every k iterations (32) */
+    int batch_tile_rows;   /* batched solves: rows per workgroup tile; 0 = autotune on first solve (0) */
 } hsps_params;
 
 /* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64

@@ -1,0 +1,176 @@
+// Preconditioners and Krylov solvers through the C++ host layer; cases follow
+// the reference's tests/test_preconditioners.cpp:26-159 and
+// tests/test_solvers.cpp:42-264, plus mixed-precision variants.
+#include "hsps_test.hpp"
+#include "preconditioners/amg.hpp"
+#include "preconditioners/ilu.hpp"
+#include "preconditioners/jacobi.hpp"
+#include "solvers/cg_solver.hpp"
+#include "solvers/fgmres_solver.hpp"
+#include "solvers/solver_factory.hpp"
+
+using namespace hsps;
+
+static SolverParams params() {
+    SolverParams p;
+    p.tol = 1e-9;
+    p.max_iter = 3000;
+    p.restart_size = 40;
+    return p;
+}
+static double rel_res(const SparseMatrix& A, const Vector& b, const Vector& x) {
+    Vector r = A * x;
+    double s = 0, bn = 0;
+    for (Index i = 0; i < b.size(); ++i) { s += (b[i] - r[i]) * (b[i] - r[i]); bn += b[i] * b[i]; }
+    return std::sqrt(s / bn);
+}
+
+TEST(jacobi_diagonal) {
+    auto D = SparseMatrix::from_coo(3, 3, {0, 1, 2}, {0, 1, 2}, {2.0, 4.0, 8.0});
+    JacobiPreconditioner j;
+    j.setup(D);
+    Vector r = {4.0, 8.0, 16.0}, z(3);
+    j.apply(r, z);
+    for (int i = 0; i < 3; ++i) expect_near(z[i], 2.0, 1e-14, "D^-1 r");
+}
+TEST(jacobi_poisson_quarter) {
+    auto A = SparseMatrix::poisson_2d(8);
+    JacobiPreconditioner j;
+    j.setup(A);
+    Vector r(64, 1.0), z(64);
+    j.apply(r, z);
+    for (int i = 0; i < 64; ++i) expect_near(z[i], 0.25, 1e-14, "1/4");
+}
+TEST(jacobi_zero_diagonal_throws) {
+    auto A = SparseMatrix::from_coo(2, 2, {0, 1}, {1, 0}, {1.0, 1.0});
+    JacobiPreconditioner j;
+    expect_throw<std::runtime_error>([&] { j.setup(A); }, "zero diagonal");
+}
+TEST(ilu_diagonal_exact) {
+    auto D = SparseMatrix::from_coo(4, 4, {0, 1, 2, 3}, {0, 1, 2, 3}, {2.0, 3.0, 5.0, 7.0});
+    ILUPreconditioner ilu;
+    ilu.setup(D);
+    expect_true(ilu.is_ready(), "ready");
+    Vector r = {6.0, 9.0, 15.0, 21.0}, z(4);
+    ilu.apply(r, z);
+    for (int i = 0; i < 4; ++i) expect_near(z[i], 3.0, 1e-12, "exact");
+}
+TEST(ilu_reduces_residual) {
+    auto A = SparseMatrix::poisson_2d(8);
+    ILUPreconditioner ilu;
+    ilu.setup(A);
+    Vector r(64, 1.0), z(64);
+    ilu.apply(r, z);
+    Vector Az = A * z;
+    double res = 0;
+    for (int i = 0; i < 64; ++i) res = std::max(res, std::abs(Az[i] - r[i]));
+    expect_less(res, r.norm_inf(), "ILU reduces residual");
+}
+TEST(amg_hierarchy_and_vcycle) {
+    auto A = SparseMatrix::poisson_2d(16);
+    AMGPreconditioner amg;
+    amg.set_max_levels(5);
+    amg.setup(A);
+    expect_true(amg.is_ready() && amg.levels() >= 2, "levels");
+    expect_true(amg.level_size(0).first == 256 && amg.level_size(1).first == 128, "256 -> 128 (pairs)");
+    Vector rhs(256, 1.0), z;
+    amg.apply(rhs, z);
+    expect_true(z.size() == 256, "size");
+    expect_less((rhs - A * z).norm2(), rhs.norm2(), "V-cycle reduces residual");
+}
+TEST(cg_1x1) {
+    auto A = SparseMatrix::from_coo(1, 1, {0}, {0}, {2.0});
+    Vector b = {4.0}, x(1, 0.0);
+    auto j = std::make_shared<JacobiPreconditioner>(); j->setup(A);
+    CGSolver cg; cg.set_params(params()); cg.set_preconditioner(j);
+    SolverStats s;
+    expect_true(cg.solve(A, b, x, s), "converged");
+    expect_near(x[0], 2.0, 1e-10, "x = 2");
+}
+TEST(cg_poisson_jacobi_and_ilu) {
+    auto A = SparseMatrix::poisson_2d(16);
+    Vector b(256, 1.0);
+    auto j = std::make_shared<JacobiPreconditioner>(); j->setup(A);
+    auto i = std::make_shared<ILUPreconditioner>(); i->setup(A);
+    CGSolver c1, c2;
+    c1.set_params(params()); c1.set_preconditioner(j);
+    c2.set_params(params()); c2.set_preconditioner(i);
+    Vector x1(256, 0.0), x2(256, 0.0);
+    SolverStats s1, s2;
+    c1.solve(A, b, x1, s1);
+    c2.solve(A, b, x2, s2);
+    expect_true(s1.converged && s2.converged, "both converged");
+    expect_less(rel_res(A, b, x1), 1e-8, "rel res");
+    expect_less(s2.iterations, s1.iterations, "ILU fewer iterations than Jacobi");
+}
+TEST(cg_zero_rhs_and_stats) {
+    auto A = SparseMatrix::poisson_2d(8);
+    auto j = std::make_shared<JacobiPreconditioner>(); j->setup(A);
+    CGSolver cg; cg.set_params(params()); cg.set_preconditioner(j);
+    Vector b(64, 0.0), x(64, 0.0);
+    SolverStats s;
+    expect_true(cg.solve(A, b, x, s) && s.iterations == 0, "zero RHS: 0 iterations");
+    Vector b1(64, 1.0), x1(64, 0.0);
+    SolverStats s1;
+    cg.solve(A, b1, x1, s1);
+    expect_true(s1.iterations > 0 && s1.flop_count > 0 && s1.mem_bytes > 0 && s1.energy_joules > 0, "stats");
+    expect_true(s1.solver_used == SolverType::CG, "solver type");
+}
+TEST(fgmres_variants) {
+    auto A = SparseMatrix::poisson_2d(16);
+    Vector b(256, 1.0);
+    for (auto pt : {PrecondType::JACOBI, PrecondType::ILU, PrecondType::AMG}) {
+        SolverParams p = params();
+        p.drop_tol = 0.0;
+        auto pair = SolverFactory::make(SolverType::FGMRES, pt, A, p);
+        Vector x(256, 0.0);
+        SolverStats s;
+        expect_true(pair.first->solve(A, b, x, s), to_string(pt));
+        expect_less(rel_res(A, b, x), 1e-8, to_string(pt));
+    }
+}
+TEST(fgmres_convdiff_and_restarts) {
+    auto A = SparseMatrix::convection_diffusion_2d(12, 0.1, 5.0, 0.0);
+    Vector b(144, 1.0), x(144, 0.0);
+    auto ilu = std::make_shared<ILUPreconditioner>(); ilu->setup(A);
+    FGMRESSolver f; f.set_params(params()); f.set_preconditioner(ilu);
+    SolverStats s;
+    expect_true(f.solve(A, b, x, s), "conv-diff converged");
+    expect_less(rel_res(A, b, x), 1e-8, "rel res");
+    auto P = SparseMatrix::poisson_2d(16);
+    SolverParams p = params();
+    p.restart_size = 5;
+    auto ilu2 = std::make_shared<ILUPreconditioner>(); ilu2->setup(P);
+    FGMRESSolver g; g.set_params(p); g.set_preconditioner(ilu2);
+    Vector b2(256, 1.0), x2(256, 0.0);
+    SolverStats s2;
+    g.solve(P, b2, x2, s2);
+    expect_true(s2.restarts >= 1, "restarts tracked");
+}
+TEST(mixed_precision_matches_fp64) {
+    auto A = SparseMatrix::poisson_2d(48);
+    Vector b(48 * 48, 1.0);
+    for (auto solver : {SolverType::CG, SolverType::FGMRES}) {
+        SolverParams p = params();
+        p.tol = 1e-8;
+        p.restart_size = 30;
+        auto f64 = SolverFactory::make(solver, PrecondType::JACOBI, A, p);
+        p.precision = Precision::MIXED;
+        auto mix = SolverFactory::make(solver, PrecondType::JACOBI, A, p);
+        Vector x1(A.rows(), 0.0), x2(A.rows(), 0.0);
+        SolverStats s1, s2;
+        f64.first->solve(A, b, x1, s1);
+        mix.first->solve(A, b, x2, s2);
+        expect_true(s1.converged && s2.converged, "both converge");
+        expect_less(s2.true_residual, 1e-8, "mixed true residual");
+        expect_less((x1 - x2).norm2(), 1e-6 * x1.norm2(), "solutions agree");
+        expect_less(std::abs(s2.iterations - s1.iterations), 0.1 * s1.iterations + 2, "iteration counts close");
+    }
+}
+TEST(factory_parse) {
+    expect_true(SolverFactory::parse_solver("fgmres") == SolverType::FGMRES, "parse");
+    expect_true(SolverFactory::parse_precond("ILU0") == PrecondType::ILU, "parse");
+    expect_throw<std::invalid_argument>([] { SolverFactory::parse_solver("gmres2"); }, "unknown solver");
+    expect_throw<std::invalid_argument>([] { SolverFactory::parse_precond("ssor"); }, "unknown precond");
+}
+int main() { return run_all(); }

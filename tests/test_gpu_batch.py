@@ -1,0 +1,98 @@
+"""Batched PCG+Jacobi (hsps_batch_*) against the FP64 CPU oracle, per system.
+
+Bars: every system's true FP64 rel-res <= tol; its solution within 1e-6
+relative of the oracle's single-system PCG+Jacobi solution; iteration counts
+within 5% (FP64) / 10% (mixed, reliable-update refinement) of the oracle's;
+and a system's result is bit-identical alone and inside a mixed batch (same
+tile size), since tiles never straddle systems and each system's reductions
+run over its own tiles in a fixed order.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_systems(H, specs):
+    systems, oracles = [], []
+    for kind, n, eps in specs:
+        if kind == "poisson":
+            arr, A = H.poisson_2d(n), O.Csr.poisson(n)
+        elif kind == "aniso":
+            arr, A = H.anisotropic_2d(n, eps), O.Csr.aniso(n, eps)
+        else:
+            arr, A = H.convection_diffusion_2d(n, 1.0, 0.0, 0.0), O.Csr.convdiff(n, 1.0, 0.0, 0.0)
+        systems.append(arr)
+        oracles.append(A)
+    return systems, oracles
+
+
+SPECS = [("poisson", 32, 0), ("aniso", 40, 0.01), ("poisson", 1, 0), ("aniso", 17, 1.0), ("poisson", 64, 0),
+         ("aniso", 64, 0.001), ("poisson", 5, 0), ("sym", 24, 0), ("poisson", 100, 0), ("aniso", 33, 0.2)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_matches_oracle(H, ctx, precision):
+    systems, oracles = make_systems(H, SPECS)
+    B = H.Batch.from_systems(ctx, systems)
+    rng_b = []
+    for s, A in enumerate(oracles):
+        xs = O.uniform(100 + s, A.rows)
+        rng_b.append(A.spmv(xs))
+    b = np.concatenate(rng_b)
+    bd, xd = ctx.array(b), ctx.array(n=len(b))
+    p = H.default_params(tol=1e-8, max_iter=5000, precision=precision)
+    ok, iters, conv, tr = B.solve(bd, xd, p)
+    x = xd.download()
+    assert ok and conv.all(), (iters, conv, tr)
+    off = B.row_off
+    for s, A in enumerate(oracles):
+        xs_dev = x[off[s]:off[s + 1]]
+        bs = rng_b[s]
+        rel = np.linalg.norm(bs - A.spmv(xs_dev)) / np.linalg.norm(bs)
+        assert rel < 1e-8 and abs(rel - tr[s]) <= 1e-3 * rel + 1e-16, (s, rel, tr[s])
+        xo, st = O.solve(A, bs, "cg", "jacobi", tol=1e-8, max_iter=5000)
+        assert np.linalg.norm(xs_dev - xo) <= 1e-6 * np.linalg.norm(xo), s
+        slack = 0.05 if precision == "fp64" else 0.10
+        assert abs(iters[s] - st.iterations) <= max(1, slack * st.iterations), (s, iters[s], st.iterations)
+
+
+def test_batch_alone_equals_in_mix(H, ctx):
+    systems, oracles = make_systems(H, SPECS)
+    b_all = [A.spmv(O.uniform(7 + s, A.rows)) for s, A in enumerate(oracles)]
+    p = H.default_params(tol=1e-8, max_iter=5000, precision="mixed", batch_tile_rows=128)
+    B = H.Batch.from_systems(ctx, systems)
+    bd, xd = ctx.array(np.concatenate(b_all)), ctx.array(n=B.n)
+    ok, it_mix, _, _ = B.solve(bd, xd, p)
+    x_mix = xd.download()
+    assert ok
+    for s in [0, 4, 5, 8]:
+        Bs = H.Batch.from_systems(ctx, [systems[s]])
+        bs, xs = ctx.array(b_all[s]), ctx.array(n=len(b_all[s]))
+        ok1, it1, _, _ = Bs.solve(bs, xs, p)
+        assert ok1 and it1[0] == it_mix[s]
+        np.testing.assert_array_equal(xs.download(), x_mix[B.row_off[s]:B.row_off[s + 1]])
+
+
+def test_batch_zero_rhs_and_warm_start(H, ctx):
+    systems, oracles = make_systems(H, [("poisson", 16, 0), ("poisson", 20, 0)])
+    B = H.Batch.from_systems(ctx, systems)
+    b = np.concatenate([np.zeros(256), oracles[1].spmv(O.uniform(3, 400))])
+    bd, xd = ctx.array(b), ctx.array(n=len(b))
+    ok, it, cv, tr = B.solve(bd, xd, H.default_params(tol=1e-8))
+    assert ok and it[0] == 0 and cv.all()
+    # warm start from the converged x: 0 iterations
+    ok, it2, cv2, _ = B.solve(bd, xd, H.default_params(tol=1e-8))
+    assert ok and it2[1] == 0
+
+
+def test_batch_rejects_bad_input(H, ctx):
+    rp, ci, v = H.poisson_2d(4)
+    ci_bad = ci.copy()
+    ci_bad[0] = 20  # outside the 16-row block of system 0
+    with pytest.raises(H.HspsError) as e:
+        H.Batch(ctx, np.array([0, 16, 32], np.int32), np.concatenate([rp[:-1], rp + rp[-1]]).astype(np.int32),
+                np.concatenate([ci_bad, ci + 16]), np.concatenate([v, v]))
+    assert e.value.code == -1

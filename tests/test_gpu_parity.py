@@ -150,8 +150,11 @@ def test_seeded_solves(H, ctx, golden, idx, precision):
         xo, sto = O.solve(A, b, solver, pre, tol=1e-8, max_iter=20000, restart_size=30, drop_tol=0.0)
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), (key, precision)
         it = ref["iters"]
-        slack = 0.05 if precision == "fp64" else 0.10
-        assert abs(st.iterations - it) <= max(1, slack * it), (key, precision, st.iterations, it)
+        # FP64: within 5% of the reference.  Mixed: total inner iterations
+        # within 10% (or 2): an FGMRES cycle that the FP32 Givens estimate
+        # ends early is followed by a short FP64-verified cycle.
+        bound = max(1, 0.05 * it) if precision == "fp64" else max(2, 0.10 * it)
+        assert abs(st.iterations - it) <= bound, (key, precision, st.iterations, it)
 
 
 def test_amg_hierarchy_bit_exact(H, ctx, golden):
