@@ -1,26 +1,29 @@
 // =============================================================================
 // batch.hip — batched PCG + Jacobi for many independent systems (no reference
-// counterpart; the reference solves one system per call).
+// counterpart; the reference solves one system per call, cg_solver.cpp:37-191
+// gives the per-system algorithm followed here).
 //
 // Layout: the systems' CSRs are concatenated (block-diagonal, global column
-// indices).  Each system is cut into tiles of <= tile_rows rows (and <= one
-// LDS stage of nonzeros); one workgroup owns one tile.  Systems are sorted by
-// size and grouped into waves whose working set fits the 256 MB Infinity
-// Cache, so a wave's iterations run out of MALL instead of HBM.
-//
-// Per iteration three launches over the tiles of the active wave, each with
-// per-system reductions finished by the system's last-arriving tile:
-//   K1  q = A p and p.q  -> alpha          (or, in RESID state, the FP64 true
-//                                           residual r = b - A x and r.z)
-//   K2  x += alpha p, r -= alpha q, r.r, r.z -> rel, rho, beta, decisions
-//   K3  p = D^-1 r + beta p                  (or, in FLUSH state, x64 += x_lo)
+// indices).  Systems are sorted by nonzeros (largest first) and split in two
+// classes:
+//  * small systems (rows <= kSmallRows): one persistent 1024-thread
+//    workgroup per system runs the whole PCG loop with workgroup barriers
+//    only — no launches per iteration, vectors stay in L2;
+//  * large systems: cut into tiles of <= tile_rows rows (one workgroup per
+//    tile), grouped into waves whose working set fits the 256 MB Infinity
+//    Cache.  Per iteration five launches over the wave's tiles:
+//      K1  q = A p, p.q partials  (FLUSH state: FP64 true residual r = b - A x)
+//      F1  per-system finish: alpha (or breakdown / reliable-update decisions)
+//      K2  x += alpha p, r -= alpha q, r.r / r.z partials
+//      F2  per-system finish: rel, rho, beta, convergence
+//      K3  p = D^-1 r + beta p   (FLUSH state: x64 += x_lo)
+//    The chunk of iterations is captured once per wave in a hipGraph and
+//    replayed; the host polls the wave's states one chunk behind.
+// The two classes run concurrently on two streams.
 // Mixed precision: FP32 matrix values and Krylov vectors, FP64 reductions and
-// scalars, x accumulated in FP64 through an FP32 low-order part that is
-// flushed at every reliable update.  A system whose |r| drops below
-// delta*max|r| (or below tol) goes FLUSH -> RESID -> PUPDATE: its FP64 true
-// residual replaces r (and confirms convergence) without extra launches.
-// The wave's iteration chunk is captured once in a hipGraph and replayed; the
-// host polls the wave's states one chunk behind.
+// scalars, x accumulated in FP64 through an FP32 low-order part flushed at
+// every reliable update (|r| < delta * max|r| since the last one, or |r| <
+// tol to confirm convergence against the FP64 true residual).
 // =============================================================================
 #include <algorithm>
 #include <chrono>
@@ -34,8 +37,10 @@
 namespace hsps_dev {
 
 static constexpr double kEps = std::numeric_limits<double>::epsilon();
+constexpr int kSmallRows = 20000;   // persistent one-workgroup path up to this size
+constexpr int kSmallThreads = 1024;
 
-enum : int { BS_NORMAL = 0, BS_FLUSH = 1, BS_RESID = 2, BS_PUPDATE = 3 };
+enum : int { BS_NORMAL = 0, BS_FLUSH = 1, BS_PUPDATE = 2 };
 enum : int { BR_NONE = 0, BR_RELIABLE = 1, BR_CHECK = 2, BR_RESTART = 3 };
 
 struct BState {
@@ -55,65 +60,27 @@ struct BArgs {
     const int* tile_sys;
     const int* sys_t0;
     const int* sys_t1;
+    const int* sys_row0;  // internal order
+    const int* sys_row1;
     BState* st;
-    unsigned* cnt;
     double* partials;
     int T;  // total tiles
     double tol, delta;
     int max_iter;
 };
 
-// ---- per-system reduction: last-arriving tile of the system sums partials ------------
-template <int NV>
-__device__ __forceinline__ bool sys_sum(double (&v)[NV], const BArgs& B, int tile, int s, double (&tot)[NV]) {
-    __shared__ double red[NV * (kBlock / 64)];
-    __shared__ int is_last;
-    block_sum<NV>(v, red);
-    const int t0 = B.sys_t0[s], t1 = B.sys_t1[s];
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) B.partials[(size_t)k * B.T + tile] = v[k];
-        const unsigned t = __hip_atomic_fetch_add(&B.cnt[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == (unsigned)(t1 - t0 - 1));
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&B.partials[(size_t)k * B.T + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    block_sum<NV>(acc, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-        B.cnt[s] = 0u;
-        return true;
-    }
-    return false;
-}
+template <class T> struct Prec;
+template <> struct Prec<double> {
+    static __device__ const double* val(const BArgs& B) { return B.v64; }
+    static __device__ const double* dinv(const BArgs& B) { return B.dinv64; }
+};
+template <> struct Prec<float> {
+    static __device__ const float* val(const BArgs& B) { return B.v32; }
+    static __device__ const float* dinv(const BArgs& B) { return B.dinv32; }
+};
 
-__device__ __forceinline__ bool sys_ticket(const BArgs& B, int s) {
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        const unsigned t = __hip_atomic_fetch_add(&B.cnt[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == (unsigned)(B.sys_t1[s] - B.sys_t0[s] - 1));
-    }
-    __syncthreads();
-    if (is_last && threadIdx.x == 0) {
-        B.cnt[s] = 0u;
-        return true;
-    }
-    return false;
-}
-
-// Row sums of a tile (rows r0..r1) of A*x, LDS-staged; calls f(row, sum) once
-// per row from one lane.  Same staging rule as the single-system SpMV.
+// Row sums of a tile (rows r0..r1) of A*x, LDS-staged; f(row, sum) once per
+// row from one lane.  Same staging rule as the single-system SpMV.
 template <class Acc, class V, class X, class F>
 __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict__ rp, const int* __restrict__ ci,
                                           const V* __restrict__ val, const X* __restrict__ x, Acc* prod, F&& f) {
@@ -143,22 +110,41 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict_
     }
 }
 
-template <class T> struct Prec;
-template <> struct Prec<double> {
-    static __device__ const double* val(const BArgs& B) { return B.v64; }
-    static __device__ const double* dinv(const BArgs& B) { return B.dinv64; }
-};
-template <> struct Prec<float> {
-    static __device__ const float* val(const BArgs& B) { return B.v32; }
-    static __device__ const float* dinv(const BArgs& B) { return B.dinv32; }
-};
+// Per-system finish: one workgroup per system of [k0, k0+gridDim.x); sums the
+// system's tile partials in tile order and runs op(S, tot) in thread 0.
+template <int NV, class Op>
+__global__ void __launch_bounds__(kBlock) k_b_finish(BArgs B, int k0, Op op) {
+    const int s = k0 + blockIdx.x;
+    BState* S = &B.st[s];
+    if (!op.active(*S)) return;
+    const int t0 = B.sys_t0[s], t1 = B.sys_t1[s];
+    __shared__ double red[NV * (kBlock / 64)];
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += B.partials[(size_t)k * B.T + t];
+    }
+    block_sum<NV>(acc, red);
+    if (threadIdx.x == 0) op(*S, acc, B);
+}
 
-// ---- init: r64 = b - A x, |b|, |r| per system ----------------------------------------
+template <int NV>
+__device__ __forceinline__ void tile_partial(double (&v)[NV], const BArgs& B, int tile) {
+    __shared__ double red[NV * (kBlock / 64)];
+    block_sum<NV>(v, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) B.partials[(size_t)k * B.T + tile] = v[k];
+    }
+}
+
+// ---- init: r64 = b - A x ; |r|, |b| --------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 k_b_init1(BArgs B, int t0, const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r64) {
     __shared__ double prod[kLdsNnz];
     const int tile = t0 + blockIdx.x;
-    const int s = B.tile_sys[tile];
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
     double red[2] = {0.0, 0.0};
     tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, x, prod, [&](int i, double sum) {
@@ -167,27 +153,32 @@ k_b_init1(BArgs B, int t0, const double* __restrict__ b, const double* __restric
         red[0] += ri * ri;
     });
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) red[1] += b[i] * b[i];
-    double tot[2];
-    if (!sys_sum<2>(red, B, tile, s, tot)) return;
-    BState S{};
-    S.bnorm = sqrt(tot[1]);
-    const double r0n = sqrt(tot[0]);
-    S.rmax = r0n;
-    if (S.bnorm < kEps) {  // cg_solver.cpp:60-66
-        S.done = S.conv = S.halt = 1;
-        S.rel = r0n;
-    } else if (r0n / S.bnorm < B.tol) {
-        S.done = S.conv = S.halt = 1;
-        S.rel = r0n / S.bnorm;
-    } else {
-        S.rel = r0n / S.bnorm;
-        S.halt = B.max_iter <= 0 ? 1 : 0;
-    }
-    S.true_rel = S.rel;
-    B.st[s] = S;
+    tile_partial<2>(red, B, tile);
 }
 
-// r = (T) r64, x_lo = 0, p = D^-1 r, rho = r.z
+struct OpInit1 {
+    __device__ bool active(const BState&) const { return true; }
+    __device__ void operator()(BState& S0, const double* tot, const BArgs& B) const {
+        BState S{};
+        S.bnorm = sqrt(tot[1]);
+        const double r0n = sqrt(tot[0]);
+        S.rmax = r0n;
+        if (S.bnorm < kEps) {  // cg_solver.cpp:60-66
+            S.done = S.conv = S.halt = 1;
+            S.rel = r0n;
+        } else if (r0n / S.bnorm < B.tol) {
+            S.done = S.conv = S.halt = 1;
+            S.rel = r0n / S.bnorm;
+        } else {
+            S.rel = r0n / S.bnorm;
+            S.halt = B.max_iter <= 0 ? 1 : 0;
+        }
+        S.true_rel = S.rel;
+        S0 = S;
+    }
+};
+
+// r = (T) r64, x_lo = 0, p = D^-1 r, r.z partials
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
 k_b_init2(BArgs B, int t0, const double* __restrict__ r64, T* __restrict__ r, T* __restrict__ p,
@@ -206,34 +197,36 @@ k_b_init2(BArgs B, int t0, const double* __restrict__ r64, T* __restrict__ r, T*
         if (MIXED) xlo[i] = 0.0f;
         red[0] += (double)ri * (double)zi;
     }
-    double tot[1];
-    if (!sys_sum<1>(red, B, tile, s, tot)) return;
-    B.st[s].rho = tot[0];
-    B.st[s].rho_old = tot[0];
+    tile_partial<1>(red, B, tile);
 }
 
-// ---- K1 ------------------------------------------------------------------------------------
+struct OpInit2 {
+    __device__ bool active(const BState& S) const { return !S.halt; }
+    __device__ void operator()(BState& S, const double* tot, const BArgs&) const {
+        S.rho = tot[0];
+        S.rho_old = tot[0];
+    }
+};
+
+// ---- K1 / F1 -------------------------------------------------------------------------------
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
 k_b_k1(BArgs B, int t0, const T* __restrict__ p, T* __restrict__ q, T* __restrict__ r, const double* __restrict__ b,
        const double* __restrict__ x64) {
-    using Acc = T;
     __shared__ double prod_d[kLdsNnz];
     const int tile = t0 + blockIdx.x;
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
     const int flag = S0->flag;
-    if (flag != BS_NORMAL && flag != BS_RESID) return;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
     double red[2] = {0.0, 0.0};
-    if (flag == BS_NORMAL) {
-        tile_spmv<Acc>(r0, r1, B.rp, B.ci, Prec<T>::val(B), p, reinterpret_cast<Acc*>(prod_d), [&](int i, Acc sum) {
-            const T qi = (T)sum;
-            q[i] = qi;
-            red[0] += (double)p[i] * (double)qi;
+    if (flag != BS_FLUSH) {
+        tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), p, reinterpret_cast<T*>(prod_d), [&](int i, T sum) {
+            q[i] = sum;
+            red[0] += (double)p[i] * (double)sum;
         });
-    } else {
+    } else {  // x was flushed by the previous K3: FP64 true residual replaces r
         const T* dinv = Prec<T>::dinv(B);
         tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, x64, prod_d, [&](int i, double sum) {
             const double rh = b[i] - sum;
@@ -243,46 +236,52 @@ k_b_k1(BArgs B, int t0, const T* __restrict__ p, T* __restrict__ q, T* __restric
             red[1] += (double)ri * (double)(T)(dinv[i] * ri);
         });
     }
-    double tot[2];
-    if (!sys_sum<2>(red, B, tile, s, tot)) return;
-    BState& S = B.st[s];
-    if (flag == BS_NORMAL) {
-        const double pq = tot[0];
-        if (fabs(pq) < kEps * 1e6 * fabs(S.rho)) {  // cg_solver.cpp:109-120: restart
-            S.iter += 1;
-            S.restarts += 1;
-            S.flag = BS_FLUSH;
-            S.reason = BR_RESTART;
-            if (S.iter >= B.max_iter) S.halt = 1;
-        } else {
-            S.alpha = S.rho / pq;
-        }
-        return;
-    }
-    // RESID: the FP64 true residual replaced r
-    const double rn = sqrt(tot[0]);
-    S.true_rel = rn / S.bnorm;
-    S.rmax = rn;
-    if (S.reason != BR_RESTART) S.reliable += 1;
-    if (S.reason == BR_CHECK && S.true_rel < B.tol) {
-        S.rel = S.true_rel;
-        S.done = S.conv = S.halt = 1;
-        S.flag = BS_NORMAL;
-        return;
-    }
-    if (S.reason == BR_RESTART) {
-        S.rho = tot[1];
-        S.rho_old = tot[1];
-        S.beta = 0.0;
-    } else {
-        S.rho_old = S.rho;
-        S.rho = tot[1];
-        S.beta = S.rho / S.rho_old;
-    }
-    S.flag = BS_PUPDATE;
+    tile_partial<2>(red, B, tile);
 }
 
-// ---- K2 ------------------------------------------------------------------------------------
+struct OpF1 {
+    __device__ bool active(const BState& S) const { return !S.halt; }
+    __device__ void operator()(BState& S, const double* tot, const BArgs& B) const {
+        if (S.flag != BS_FLUSH) {
+            const double pq = tot[0];
+            if (fabs(pq) < kEps * 1e6 * fabs(S.rho)) {  // cg_solver.cpp:109-120: restart
+                S.iter += 1;
+                S.restarts += 1;
+                S.flag = BS_FLUSH;
+                S.reason = BR_RESTART;
+                if (S.iter >= B.max_iter) S.halt = 1;
+            } else {
+                S.alpha = S.rho / pq;
+                S.flag = BS_NORMAL;
+            }
+            return;
+        }
+        // FLUSH: the FP64 true residual replaced r
+        const double rn = sqrt(tot[0]);
+        S.true_rel = rn / S.bnorm;
+        S.rmax = rn;
+        if (S.reason != BR_RESTART) S.reliable += 1;
+        if (S.reason == BR_CHECK && S.true_rel < B.tol) {
+            S.rel = S.true_rel;
+            S.done = S.conv = S.halt = 1;
+            S.flag = BS_NORMAL;
+            return;
+        }
+        if (S.reason == BR_RESTART) {
+            S.rho = tot[1];
+            S.rho_old = tot[1];
+            S.beta = 0.0;
+        } else {
+            S.rho_old = S.rho;
+            S.rho = tot[1];
+            S.beta = S.rho / S.rho_old;
+        }
+        S.flag = BS_PUPDATE;
+        S.reason = BR_NONE;
+    }
+};
+
+// ---- K2 / F2 -------------------------------------------------------------------------------
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
 k_b_k2(BArgs B, int t0, const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ r,
@@ -304,42 +303,46 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p, const T* __restrict__ q, T* __r
         red[0] += rd * rd;
         red[1] += rd * (double)(T)(dinv[i] * ri);
     }
-    double tot[2];
-    if (!sys_sum<2>(red, B, tile, s, tot)) return;
-    BState& S = B.st[s];
-    const double rn = sqrt(tot[0]);
-    S.rel = rn / S.bnorm;
-    S.iter += 1;
-    if (S.rel < B.tol) {  // cg_solver.cpp:143
-        if (MIXED) {
-            S.flag = BS_FLUSH;
-            S.reason = BR_CHECK;
-        } else {
-            S.done = S.conv = S.halt = 1;
+    tile_partial<2>(red, B, tile);
+}
+
+template <bool MIXED>
+struct OpF2 {
+    __device__ bool active(const BState& S) const { return !S.halt && S.flag == BS_NORMAL; }
+    __device__ void operator()(BState& S, const double* tot, const BArgs& B) const {
+        const double rn = sqrt(tot[0]);
+        S.rel = rn / S.bnorm;
+        S.iter += 1;
+        if (S.rel < B.tol) {  // cg_solver.cpp:143
+            if (MIXED) {
+                S.flag = BS_FLUSH;
+                S.reason = BR_CHECK;
+            } else {
+                S.done = S.conv = S.halt = 1;
+            }
+            return;
         }
-        return;
-    }
-    if (MIXED) {
-        S.rmax = fmax(S.rmax, rn);
-        if (rn < B.delta * S.rmax) {
+        if (MIXED) {
+            S.rmax = fmax(S.rmax, rn);
+            if (rn < B.delta * S.rmax) {
+                S.flag = BS_FLUSH;
+                S.reason = BR_RELIABLE;
+                return;
+            }
+        }
+        S.rho_old = S.rho;
+        S.rho = tot[1];
+        if (fabs(S.rho_old) < kEps * 1e8 * fabs(S.rho)) {  // cg_solver.cpp:157-167: restart
+            S.restarts += 1;
             S.flag = BS_FLUSH;
-            S.reason = BR_RELIABLE;
+            S.reason = BR_RESTART;
             if (S.iter >= B.max_iter) S.halt = 1;
             return;
         }
-    }
-    S.rho_old = S.rho;
-    S.rho = tot[1];
-    if (fabs(S.rho_old) < kEps * 1e8 * fabs(S.rho)) {  // cg_solver.cpp:157-167: restart
-        S.restarts += 1;
-        S.flag = BS_FLUSH;
-        S.reason = BR_RESTART;
+        S.beta = S.rho / S.rho_old;
         if (S.iter >= B.max_iter) S.halt = 1;
-        return;
     }
-    S.beta = S.rho / S.rho_old;
-    if (S.iter >= B.max_iter) S.halt = 1;
-}
+};
 
 // ---- K3 ------------------------------------------------------------------------------------
 template <class T, bool MIXED>
@@ -352,25 +355,23 @@ k_b_k3(BArgs B, int t0, T* __restrict__ p, const T* __restrict__ r, double* __re
     if (S0->halt) return;
     const int flag = S0->flag;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
-    if (flag == BS_NORMAL || flag == BS_PUPDATE) {
-        const double beta = S0->beta;
-        const T* dinv = Prec<T>::dinv(B);
-        for (int i = r0 + threadIdx.x; i < r1; i += kBlock)
-            p[i] = (T)((double)(T)(dinv[i] * r[i]) + beta * (double)p[i]);
-        if (flag == BS_PUPDATE && sys_ticket(B, s)) B.st[s].flag = BS_NORMAL;
-    } else if (flag == BS_FLUSH) {
+    if (flag == BS_FLUSH) {
         if (MIXED)
             for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
                 x64[i] += (double)xlo[i];
                 xlo[i] = 0.0f;
             }
-        if (sys_ticket(B, s)) B.st[s].flag = BS_RESID;
+        return;
     }
+    const double beta = S0->beta;
+    const T* dinv = Prec<T>::dinv(B);
+    for (int i = r0 + threadIdx.x; i < r1; i += kBlock)
+        p[i] = (T)((double)(T)(dinv[i] * r[i]) + beta * (double)p[i]);
 }
 
-// ---- final: x64 += x_lo (all rows) and the FP64 true residual per system -------------------
-__global__ void k_b_flush_all(int n, double* __restrict__ x64, float* __restrict__ xlo) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+// ---- final: x64 += x_lo and the FP64 true residual per (large) system ----------------------
+__global__ void k_b_flush_range(int a, int e, double* __restrict__ x64, float* __restrict__ xlo) {
+    for (int i = a + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
         x64[i] += (double)xlo[i];
         xlo[i] = 0.0f;
     }
@@ -380,17 +381,191 @@ __global__ void __launch_bounds__(kBlock)
 k_b_final(BArgs B, int t0, const double* __restrict__ b, const double* __restrict__ x) {
     __shared__ double prod[kLdsNnz];
     const int tile = t0 + blockIdx.x;
-    const int s = B.tile_sys[tile];
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
     double red[1] = {0.0};
     tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, x, prod, [&](int i, double sum) {
         const double ri = b[i] - sum;
         red[0] += ri * ri;
     });
-    double tot[1];
-    if (!sys_sum<1>(red, B, tile, s, tot)) return;
-    BState& S = B.st[s];
-    S.true_rel = S.bnorm > kEps ? sqrt(tot[0]) / S.bnorm : sqrt(tot[0]);
+    tile_partial<1>(red, B, tile);
+}
+
+struct OpFinal {
+    __device__ bool active(const BState&) const { return true; }
+    __device__ void operator()(BState& S, const double* tot, const BArgs&) const {
+        S.true_rel = S.bnorm > kEps ? sqrt(tot[0]) / S.bnorm : sqrt(tot[0]);
+    }
+};
+
+// ---- small systems: one persistent workgroup per system ---------------------------------------
+template <int NV>
+__device__ __forceinline__ void wg_sum(double (&v)[NV], double* red) {
+    constexpr int W = kSmallThreads / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();  // red[] reuse across calls
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k * W + wid] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s += red[k * W + w];
+        v[k] = s;  // every thread gets the total
+    }
+}
+
+// The whole PCG of cg_solver.cpp:37-191 for one system, with the same
+// reliable-update refinement as the tiled path in mixed mode.
+template <class T, bool MIXED>
+__global__ void __launch_bounds__(kSmallThreads)
+k_b_small(BArgs B, int k0, const double* __restrict__ b, double* __restrict__ x64, T* __restrict__ r,
+          T* __restrict__ p, T* __restrict__ q, float* __restrict__ xlo) {
+    __shared__ double red[4 * (kSmallThreads / 64)];
+    const int s = k0 + blockIdx.x;
+    const int a = B.sys_row0[s], e = B.sys_row1[s];
+    const int tid = threadIdx.x;
+    const int* __restrict__ rp = B.rp;
+    const int* __restrict__ ci = B.ci;
+    const T* __restrict__ val = Prec<T>::val(B);
+    const T* __restrict__ dinv = Prec<T>::dinv(B);
+    // true residual into r (rounded), returns (|r64|^2, r.z)
+    auto true_residual = [&](double (&o)[2]) {
+        o[0] = o[1] = 0.0;
+        for (int i = a + tid; i < e; i += kSmallThreads) {
+            double sum = 0.0;
+            for (int k = rp[i]; k < rp[i + 1]; ++k) sum += B.v64[k] * x64[ci[k]];
+            const double rh = b[i] - sum;
+            const T ri = (T)rh;
+            r[i] = ri;
+            o[0] += rh * rh;
+            o[1] += (double)ri * (double)(T)(dinv[i] * ri);
+        }
+        wg_sum<2>(o, red);
+    };
+    BState S{};
+    double bb[1] = {0.0};
+    for (int i = a + tid; i < e; i += kSmallThreads) bb[0] += b[i] * b[i];
+    wg_sum<1>(bb, red);
+    S.bnorm = sqrt(bb[0]);
+    double o[2];
+    true_residual(o);
+    const double r0n = sqrt(o[0]);
+    S.rmax = r0n;
+    S.rel = S.bnorm < kEps ? r0n : r0n / S.bnorm;
+    S.true_rel = S.rel;
+    if (S.bnorm < kEps || S.rel < B.tol) {
+        S.done = S.conv = S.halt = 1;
+    } else {
+        // p = z = D^-1 r ; rho = r.z
+        for (int i = a + tid; i < e; i += kSmallThreads) {
+            p[i] = (T)(dinv[i] * r[i]);
+            if (MIXED) xlo[i] = 0.0f;
+        }
+        S.rho = S.rho_old = o[1];
+        __syncthreads();
+        while (S.iter < B.max_iter) {
+            // q = A p ; p.q
+            double v1[1] = {0.0};
+            for (int i = a + tid; i < e; i += kSmallThreads) {
+                T sum = 0;
+                for (int k = rp[i]; k < rp[i + 1]; ++k) sum += val[k] * p[ci[k]];
+                q[i] = sum;
+                v1[0] += (double)p[i] * (double)sum;
+            }
+            wg_sum<1>(v1, red);
+            int reason = BR_NONE;
+            const double pq = v1[0];
+            if (fabs(pq) < kEps * 1e6 * fabs(S.rho)) {
+                S.iter += 1;
+                S.restarts += 1;
+                reason = BR_RESTART;
+            } else {
+                const double alpha = S.rho / pq;
+                double v2[2] = {0.0, 0.0};
+                for (int i = a + tid; i < e; i += kSmallThreads) {
+                    if (MIXED) xlo[i] = (float)((double)xlo[i] + alpha * (double)p[i]);
+                    else x64[i] = x64[i] + alpha * (double)p[i];
+                    const T ri = (T)((double)r[i] - alpha * (double)q[i]);
+                    r[i] = ri;
+                    v2[0] += (double)ri * (double)ri;
+                    v2[1] += (double)ri * (double)(T)(dinv[i] * ri);
+                }
+                wg_sum<2>(v2, red);
+                const double rn = sqrt(v2[0]);
+                S.rel = rn / S.bnorm;
+                S.iter += 1;
+                if (MIXED) S.rmax = fmax(S.rmax, rn);
+                if (S.rel < B.tol) {
+                    if (!MIXED) { S.done = S.conv = 1; break; }
+                    reason = BR_CHECK;
+                } else if (MIXED && rn < B.delta * S.rmax) {
+                    reason = BR_RELIABLE;
+                } else {
+                    S.rho_old = S.rho;
+                    S.rho = v2[1];
+                    if (fabs(S.rho_old) < kEps * 1e8 * fabs(S.rho)) {
+                        S.restarts += 1;
+                        reason = BR_RESTART;
+                    } else {
+                        S.beta = S.rho / S.rho_old;
+                    }
+                }
+            }
+            if (reason != BR_NONE) {
+                if (MIXED) {
+                    for (int i = a + tid; i < e; i += kSmallThreads) {
+                        x64[i] += (double)xlo[i];
+                        xlo[i] = 0.0f;
+                    }
+                }
+                __syncthreads();
+                true_residual(o);
+                const double rn = sqrt(o[0]);
+                S.true_rel = rn / S.bnorm;
+                S.rmax = rn;
+                if (reason != BR_RESTART) S.reliable += 1;
+                if (reason == BR_CHECK && S.true_rel < B.tol) {
+                    S.rel = S.true_rel;
+                    S.done = S.conv = 1;
+                    break;
+                }
+                if (reason == BR_RESTART) {
+                    S.rho = S.rho_old = o[1];
+                    S.beta = 0.0;
+                } else {
+                    S.rho_old = S.rho;
+                    S.rho = o[1];
+                    S.beta = S.rho / S.rho_old;
+                }
+            }
+            if (S.iter >= B.max_iter) break;
+            for (int i = a + tid; i < e; i += kSmallThreads)
+                p[i] = (T)((double)(T)(dinv[i] * r[i]) + S.beta * (double)p[i]);
+            __syncthreads();
+        }
+        S.halt = 1;
+        if (MIXED)
+            for (int i = a + tid; i < e; i += kSmallThreads) {
+                x64[i] += (double)xlo[i];
+                xlo[i] = 0.0f;
+            }
+        __syncthreads();
+        double t[1] = {0.0};
+        for (int i = a + tid; i < e; i += kSmallThreads) {
+            double sum = 0.0;
+            for (int k = rp[i]; k < rp[i + 1]; ++k) sum += B.v64[k] * x64[ci[k]];
+            const double rh = b[i] - sum;
+            t[0] += rh * rh;
+        }
+        wg_sum<1>(t, red);
+        S.true_rel = sqrt(t[0]) / S.bnorm;
+    }
+    if (tid == 0) B.st[s] = S;
 }
 
 // =============================================================================
@@ -398,26 +573,33 @@ k_b_final(BArgs B, int t0, const double* __restrict__ b, const double* __restric
 // =============================================================================
 struct Batch {
     Ctx* ctx = nullptr;
+    hipStream_t aux = nullptr;  // small-system stream
+    hipEvent_t join = nullptr, fork = nullptr;
     int S = 0;  // systems
     int N = 0;  // rows
     long long nnz = 0;
+    int n_large = 0;  // internal systems [0, n_large) are tiled; the rest are small
     std::vector<int> h_row_off;  // user order, S+1
     std::vector<int> h_rp;       // global row_ptr, N+1
-    std::vector<int> order;      // internal position -> user system id (sorted by nnz, descending)
-    DevBuf rp, ci, v64, v32, dinv64, dinv32;
-    // schedule (depends on tile_rows)
+    std::vector<int> order;      // internal position -> user system id (by nnz, descending)
+    DevBuf rp, ci, v64, v32, dinv64, dinv32, sys_row0, sys_row1;
+    // tiled schedule (depends on tile_rows)
     int tile_rows = 0;
     int T = 0;
     DevBuf tile_r0, tile_r1, tile_sys, sys_t0, sys_t1;
     std::vector<int> h_sys_t0, h_sys_t1;
-    std::vector<std::pair<int, int>> waves;  // internal system ranges
+    std::vector<std::pair<int, int>> waves;  // internal system ranges within [0, n_large)
     // per-solve work (reused)
-    DevBuf st, cnt, partials, r64, r, p, q, xlo;
+    DevBuf st, partials, r64, r, p, q, xlo;
     int work_prec = -1;
-    // cached graphs per (wave, precision)
     struct G { hipGraphExec_t exec = nullptr; int wave = -1, prec = -1, chunk = 0; };
     std::vector<G> graphs;
-    ~Batch() { drop_graphs(); }
+    ~Batch() {
+        drop_graphs();
+        if (join) (void)hipEventDestroy(join);
+        if (fork) (void)hipEventDestroy(fork);
+        if (aux) (void)hipStreamDestroy(aux);
+    }
     void drop_graphs() {
         for (auto& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -429,7 +611,7 @@ static void build_tiles(Batch& B, int tile_rows) {
     std::vector<int> r0, r1, sys;
     B.h_sys_t0.assign(B.S, 0);
     B.h_sys_t1.assign(B.S, 0);
-    for (int k = 0; k < B.S; ++k) {
+    for (int k = 0; k < B.n_large; ++k) {
         const int s = B.order[k];
         const int a = B.h_row_off[s], e = B.h_row_off[s + 1];
         B.h_sys_t0[k] = (int)r0.size();
@@ -446,11 +628,6 @@ static void build_tiles(Batch& B, int tile_rows) {
             r1.push_back(j);
             sys.push_back(k);
             i = j;
-        }
-        if (a == e) {  // empty system: one empty tile keeps the reduction well-formed
-            r0.push_back(a);
-            r1.push_back(a);
-            sys.push_back(k);
         }
         B.h_sys_t1[k] = (int)r0.size();
     }
@@ -469,13 +646,13 @@ static void build_tiles(Batch& B, int tile_rows) {
     B.drop_graphs();
 }
 
-// Waves: consecutive systems (largest first) whose FP32 working set
+// Waves over the large systems: consecutive systems whose FP32 working set
 // (8 B/nnz + ~36 B/row of vectors) stays under the MALL budget.
 static void build_waves(Batch& B, size_t budget) {
     B.waves.clear();
     int start = 0;
     size_t acc = 0;
-    for (int k = 0; k < B.S; ++k) {
+    for (int k = 0; k < B.n_large; ++k) {
         const int s = B.order[k];
         const size_t rows = B.h_row_off[s + 1] - B.h_row_off[s];
         const size_t nz = B.h_rp[B.h_row_off[s + 1]] - B.h_rp[B.h_row_off[s]];
@@ -487,7 +664,7 @@ static void build_waves(Batch& B, size_t budget) {
         }
         acc += ws;
     }
-    if (start < B.S) B.waves.push_back({start, B.S});
+    if (start < B.n_large) B.waves.push_back({start, B.n_large});
 }
 
 static BArgs args_of(const Batch& B, const hsps_params& p) {
@@ -503,8 +680,9 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     a.tile_sys = B.tile_sys.as<int>();
     a.sys_t0 = B.sys_t0.as<int>();
     a.sys_t1 = B.sys_t1.as<int>();
+    a.sys_row0 = B.sys_row0.as<int>();
+    a.sys_row1 = B.sys_row1.as<int>();
     a.st = B.st.as<BState>();
-    a.cnt = B.cnt.as<unsigned>();
     a.partials = B.partials.as<double>();
     a.T = B.T;
     a.tol = p.tol;
@@ -514,15 +692,18 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
 }
 
 template <class T, bool MIXED>
-static void enqueue_iters(const Batch& B, const BArgs& a, int t0, int nt, int iters, const double* b, double* x,
-                          hipStream_t s) {
+static void enqueue_iters(const Batch& B, const BArgs& a, int k0, int k1, int t0, int nt, int iters, const double* b,
+                          double* x, hipStream_t s) {
     T* r = B.r.as<T>();
     T* p = B.p.as<T>();
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
+    const int ns = k1 - k0;
     for (int it = 0; it < iters; ++it) {
         hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, (const T*)p, q, r, b, (const double*)x);
+        hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF1{});
         hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, (const T*)p, (const T*)q, r, x, xl);
+        hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF2<MIXED>{});
         hipLaunchKernelGGL((k_b_k3<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, p, (const T*)r, x, xl);
     }
     HSPS_CHECK_LAUNCH();
@@ -532,8 +713,6 @@ static void ensure_work(Batch& B, int prec) {
     const size_t n = (size_t)std::max(B.N, 1);
     if (!B.st.p) {
         B.st = DevBuf(B.ctx, sizeof(BState) * (size_t)std::max(B.S, 1));
-        B.cnt = DevBuf(B.ctx, sizeof(unsigned) * (size_t)std::max(B.S, 1));
-        HSPS_HIP(hipMemset(B.cnt.p, 0, B.cnt.bytes));
         B.r64 = DevBuf(B.ctx, n * sizeof(double));
     }
     if (B.work_prec != prec) {
@@ -547,26 +726,14 @@ static void ensure_work(Batch& B, int prec) {
     }
 }
 
-static void launch_init(Batch& B, const BArgs& a, const double* b, double* x, int prec, hipStream_t s) {
-    if (B.T == 0) return;
-    hipLaunchKernelGGL(k_b_init1, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x, B.r64.as<double>());
-    if (prec == HSPS_MIXED)
-        hipLaunchKernelGGL((k_b_init2<float, true>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
-                           (const double*)B.r64.as<double>(), B.r.as<float>(), B.p.as<float>(), B.xlo.as<float>());
-    else
-        hipLaunchKernelGGL((k_b_init2<double, false>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
-                           (const double*)B.r64.as<double>(), B.r.as<double>(), B.p.as<double>(), (float*)nullptr);
-    HSPS_CHECK_LAUNCH();
-}
-
 static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const double* b, double* x, int prec,
                          hipStream_t s, bool use_graph) {
     const int k0 = B.waves[w].first, k1 = B.waves[w].second;
     const int t0 = B.h_sys_t0[k0], nt = B.h_sys_t1[k1 - 1] - t0;
     if (nt <= 0) return;
     auto direct = [&]() {
-        if (prec == HSPS_MIXED) enqueue_iters<float, true>(B, a, t0, nt, chunk, b, x, s);
-        else enqueue_iters<double, false>(B, a, t0, nt, chunk, b, x, s);
+        if (prec == HSPS_MIXED) enqueue_iters<float, true>(B, a, k0, k1, t0, nt, chunk, b, x, s);
+        else enqueue_iters<double, false>(B, a, k0, k1, t0, nt, chunk, b, x, s);
     };
     if (!use_graph) { direct(); return; }
     Batch::G* g = nullptr;
@@ -575,14 +742,21 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const doubl
     if (!g) {
         hipGraph_t graph = nullptr;
         HSPS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        direct();
+        try {
+            direct();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
         HSPS_HIP(hipStreamEndCapture(s, &graph));
         Batch::G ng;
         ng.wave = w;
         ng.prec = prec;
         ng.chunk = chunk;
-        HSPS_HIP(hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0));
+        const hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
+        HSPS_HIP(e);
         B.graphs.push_back(ng);
         g = &B.graphs.back();
     }
@@ -596,21 +770,47 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
     const int prec = p.precision == HSPS_MIXED ? HSPS_MIXED : HSPS_FP64;
     ensure_work(B, prec);
     const BArgs a = args_of(B, p);
-    launch_init(B, a, b, x, prec, s);
+    // small systems: one persistent workgroup each, on the aux stream
+    const int n_small = B.S - B.n_large;
+    HSPS_HIP(hipEventRecord(B.fork, s));
+    HSPS_HIP(hipStreamWaitEvent(B.aux, B.fork, 0));
+    if (n_small > 0) {
+        if (prec == HSPS_MIXED)
+            hipLaunchKernelGGL((k_b_small<float, true>), dim3(n_small), dim3(kSmallThreads), 0, B.aux, a, B.n_large, b, x,
+                               B.r.as<float>(), B.p.as<float>(), B.q.as<float>(), B.xlo.as<float>());
+        else
+            hipLaunchKernelGGL((k_b_small<double, false>), dim3(n_small), dim3(kSmallThreads), 0, B.aux, a, B.n_large, b,
+                               x, B.r.as<double>(), B.p.as<double>(), B.q.as<double>(), (float*)nullptr);
+        HSPS_CHECK_LAUNCH();
+    }
+    HSPS_HIP(hipEventRecord(B.join, B.aux));
+    // large systems: tiled waves on the main stream
+    if (B.T > 0) {
+        hipLaunchKernelGGL(k_b_init1, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x, B.r64.as<double>());
+        hipLaunchKernelGGL((k_b_finish<2, OpInit1>), dim3(B.n_large), dim3(kBlock), 0, s, a, 0, OpInit1{});
+        if (prec == HSPS_MIXED)
+            hipLaunchKernelGGL((k_b_init2<float, true>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
+                               (const double*)B.r64.as<double>(), B.r.as<float>(), B.p.as<float>(), B.xlo.as<float>());
+        else
+            hipLaunchKernelGGL((k_b_init2<double, false>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
+                               (const double*)B.r64.as<double>(), B.r.as<double>(), B.p.as<double>(), (float*)nullptr);
+        hipLaunchKernelGGL((k_b_finish<1, OpInit2>), dim3(B.n_large), dim3(kBlock), 0, s, a, 0, OpInit2{});
+        HSPS_CHECK_LAUNCH();
+    }
     const int chunk = std::max(1, std::min(p.check_every > 0 ? p.check_every : 32, 64));
     std::vector<BState> host(B.S);
     BState* pin = nullptr;
     HSPS_HIP(hipHostMalloc((void**)&pin, sizeof(BState) * (size_t)std::max(B.S, 1) * 2, hipHostMallocDefault));
-    hipEvent_t ev[2];
-    HSPS_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HSPS_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    hipEvent_t ev[2] = {nullptr, nullptr};
     try {
+        HSPS_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HSPS_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
         for (int w = 0; w < (int)B.waves.size(); ++w) {
             const int k0 = B.waves[w].first, k1 = B.waves[w].second;
             const int ns = k1 - k0;
             int k = 0, chunks = 0;
             bool pending = false;
-            const int cap = (p.max_iter + chunk - 1) / chunk + 2;
+            const int cap = (p.max_iter + chunk - 1) / chunk + 4;
             for (;;) {
                 launch_chunk(B, a, w, chunk, b, x, prec, s, use_graph);
                 ++chunks;
@@ -629,18 +829,30 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
                 if (all || chunks >= cap || (max_chunks > 0 && chunks >= max_chunks)) break;
             }
         }
-        if (prec == HSPS_MIXED)
-            hipLaunchKernelGGL(k_b_flush_all, dim3(grid_for(B.N)), dim3(kBlock), 0, s, B.N, x, B.xlo.as<float>());
-        if (B.T) hipLaunchKernelGGL(k_b_final, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x);
-        HSPS_CHECK_LAUNCH();
+        if (B.T > 0) {
+            if (prec == HSPS_MIXED) {
+                // flush every large system's rows (systems are not contiguous in row order)
+                for (int k = 0; k < B.n_large; ++k) {
+                    const int sid = B.order[k];
+                    const int a0 = B.h_row_off[sid], a1 = B.h_row_off[sid + 1];
+                    hipLaunchKernelGGL(k_b_flush_range, dim3(grid_for(a1 - a0)), dim3(kBlock), 0, s, a0, a1, x,
+                                       B.xlo.as<float>());
+                }
+            }
+            hipLaunchKernelGGL(k_b_final, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x);
+            hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(B.n_large), dim3(kBlock), 0, s, a, 0, OpFinal{});
+            HSPS_CHECK_LAUNCH();
+        }
+        HSPS_HIP(hipStreamWaitEvent(s, B.join, 0));
         HSPS_HIP(hipMemcpyAsync(pin, B.st.p, sizeof(BState) * B.S, hipMemcpyDeviceToHost, s));
         HSPS_HIP(hipStreamSynchronize(s));
         std::memcpy(host.data(), pin, sizeof(BState) * B.S);
     } catch (...) {
         (void)hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(B.aux);
         (void)hipHostFree(pin);
-        (void)hipEventDestroy(ev[0]);
-        (void)hipEventDestroy(ev[1]);
+        if (ev[0]) (void)hipEventDestroy(ev[0]);
+        if (ev[1]) (void)hipEventDestroy(ev[1]);
         throw;
     }
     (void)hipHostFree(pin);
@@ -649,14 +861,19 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
     return host;
 }
 
-// Project autotuner: times a few chunks of the first wave for each tile size
-// on scratch copies of x, keeps the fastest.
+// Project autotuner: times a few chunks of the tiled waves for each tile size
+// on a scratch copy of x and keeps the fastest.
 static void autotune(Batch& B, const double* b, const double* x0, const hsps_params& p) {
+    if (B.n_large == 0) {
+        build_tiles(B, 256);
+        return;
+    }
     const int cands[] = {64, 128, 256};
     hipStream_t s = B.ctx->stream;
     DevBuf xs(B.ctx, (size_t)std::max(B.N, 1) * sizeof(double));
     hsps_params q = p;
     q.check_every = 16;
+    q.max_iter = 48;  // bounds the small-system kernel during tuning
     int best = 256;
     double best_t = 1e30;
     for (int tr : cands) {
@@ -715,6 +932,9 @@ int hsps_batch_create(hsps_ctx ctx, int S, const int* row_off, const int* rp, co
             B->h_row_off.assign(row_off, row_off + S + 1);
             B->h_rp.assign(rp, rp + N + 1);
             ctx->set_current();
+            HSPS_HIP(hipStreamCreateWithFlags(&B->aux, hipStreamNonBlocking));
+            HSPS_HIP(hipEventCreateWithFlags(&B->join, hipEventDisableTiming));
+            HSPS_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
             B->rp = DevBuf(ctx, sizeof(int) * (size_t)(N + 1));
             B->ci = DevBuf(ctx, sizeof(int) * (size_t)std::max<long long>(nnz, 1));
             B->v64 = DevBuf(ctx, sizeof(double) * (size_t)std::max<long long>(nnz, 1));
@@ -738,6 +958,20 @@ int hsps_batch_create(hsps_ctx ctx, int S, const int* row_off, const int* rp, co
             std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) {
                 return rp[row_off[a + 1]] - rp[row_off[a]] > rp[row_off[b + 1]] - rp[row_off[b]];
             });
+            B->n_large = 0;
+            for (int k = 0; k < S; ++k)
+                if (row_off[B->order[k] + 1] - row_off[B->order[k]] > kSmallRows) B->n_large = k + 1;
+            std::vector<int> r0(S), r1(S);
+            for (int k = 0; k < S; ++k) {
+                r0[k] = row_off[B->order[k]];
+                r1[k] = row_off[B->order[k] + 1];
+            }
+            B->sys_row0 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
+            B->sys_row1 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
+            if (S) {
+                HSPS_HIP(hipMemcpy(B->sys_row0.p, r0.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+                HSPS_HIP(hipMemcpy(B->sys_row1.p, r1.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+            }
             build_tiles(*B, 256);
             B->tile_rows = 0;  // not tuned yet
             build_waves(*B, (size_t)192 << 20);
@@ -745,6 +979,7 @@ int hsps_batch_create(hsps_ctx ctx, int S, const int* row_off, const int* rp, co
             delete B;
             throw;
         }
+        ctx_retain(ctx);
         *out = B;
         return HSPS_OK;
     });
@@ -753,9 +988,12 @@ int hsps_batch_create(hsps_ctx ctx, int S, const int* row_off, const int* rp, co
 int hsps_batch_destroy(hsps_batch B) {
     return guarded([&] {
         if (B) {
-            (void)hipSetDevice(B->ctx->device);
-            (void)hipStreamSynchronize(B->ctx->stream);
+            Ctx* c = B->ctx;
+            (void)hipSetDevice(c->device);
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipStreamSynchronize(B->aux);
             delete B;
+            ctx_release(c);
         }
         return HSPS_OK;
     });
@@ -779,11 +1017,10 @@ int hsps_batch_solve(hsps_batch B, const double* b, double* x, const hsps_params
         for (int k = 0; k < B->S; ++k) {
             const int s = B->order[k];
             const BState& S = st[k];
-            const int c = S.conv;
             if (iters) iters[s] = S.iter;
-            if (conv) conv[s] = c;
+            if (conv) conv[s] = S.conv;
             if (true_res) true_res[s] = S.true_rel;
-            n_conv += c;
+            n_conv += S.conv;
         }
         return n_conv == B->S ? 1 : 0;
     });

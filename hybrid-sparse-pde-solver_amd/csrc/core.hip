@@ -18,6 +18,18 @@ namespace hsps_dev {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+void ctx_retain(Ctx* c) { c->refs.fetch_add(1); }
+void ctx_release(Ctx* c) {
+    if (c->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->red_partials.reset();
+    c->red_out.reset();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete static_cast<hsps_ctx_s*>(c);
+}
+
 // ---- DevBuf ------------------------------------------------------------------
 DevBuf::DevBuf(Ctx* c, size_t b) : ctx(c), bytes(b) {
     if (b == 0) return;
@@ -156,11 +168,16 @@ void spmv_add_f32(const Csr& A, const float* x, float* y, hipStream_t s, const i
     launch_spmv(A, A.v32.as<float>(), x, e, s);
 }
 void residual_f64(const Csr& A, const double* b, const double* x, double* r, hipStream_t s, double* d_nrm2,
-                  double* partials, unsigned* counter, const int* skip) {
+                  double* partials, const int* skip) {
     if (d_nrm2) {
         EpiResidual<double, true> e;
-        e.flag = skip; e.b = b; e.r = r; e.out = d_nrm2; e.partials = partials; e.counter = counter;
+        e.flag = skip; e.b = b; e.r = r; e.partials = partials;
         launch_spmv(A, A.v64.as<double>(), x, e, s);
+        OpStore<1> op;
+        op.out = d_nrm2;
+        op.flag = skip;
+        launch_finish<1>(partials, A.n_blocks, op, s);
+        HSPS_CHECK_LAUNCH();
     } else {
         EpiResidual<double, false> e;
         e.flag = skip; e.b = b; e.r = r;
@@ -189,16 +206,17 @@ void smooth_f32(const Csr& A, const float* dinv, float omega, const float* rhs, 
 int reduction_blocks(int n) { return std::min(grid_for(n, 8), kMaxRedBlocks); }
 
 __global__ void __launch_bounds__(kBlock)
-k_dot(int n, const double* __restrict__ x, const double* __restrict__ y, double* out, double* partials,
-      unsigned* counter) {
+k_dot(int n, const double* __restrict__ x, const double* __restrict__ y, double* partials) {
     double v[1] = {0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[0] += x[i] * y[i];
-    double tot[1];
-    if (grid_sum<1>(v, partials, counter, tot)) *out = tot[0];
+    block_partial<1>(v, partials);
 }
-void dot_f64_dev(int n, const double* x, const double* y, double* d_out, double* partials, unsigned* counter,
-                 hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(reduction_blocks(n)), dim3(kBlock), 0, s, n, x, y, d_out, partials, counter);
+void dot_f64_dev(int n, const double* x, const double* y, double* d_out, double* partials, hipStream_t s) {
+    const int g = reduction_blocks(n);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, s, n, x, y, partials);
+    OpStore<1> op;
+    op.out = d_out;
+    launch_finish<1>(partials, g, op, s);
     HSPS_CHECK_LAUNCH();
 }
 
@@ -289,9 +307,7 @@ int hsps_ctx_create(int device, hsps_ctx* out) {
             HSPS_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
             c->stream = c->own_stream;
             c->red_partials = DevBuf(c, sizeof(double) * kMaxRedBlocks * 4);
-            c->red_counter = DevBuf(c, 256);
             c->red_out = DevBuf(c, 256);
-            HSPS_HIP(hipMemset(c->red_counter.p, 0, 256));
             HSPS_HIP(hipHostMalloc((void**)&c->pinned, 64 * sizeof(double), hipHostMallocDefault));
         } catch (...) {
             delete c;
@@ -304,15 +320,7 @@ int hsps_ctx_create(int device, hsps_ctx* out) {
 
 int hsps_ctx_destroy(hsps_ctx ctx) {
     return guarded([&] {
-        if (!ctx) return HSPS_OK;
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
-        ctx->red_partials.reset();
-        ctx->red_counter.reset();
-        ctx->red_out.reset();
-        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
-        delete ctx;
+        if (ctx) ctx_release(ctx);
         return HSPS_OK;
     });
 }
@@ -332,7 +340,7 @@ int hsps_ctx_sync(hsps_ctx ctx) {
         return HSPS_OK;
     });
 }
-size_t hsps_ctx_bytes_in_use(hsps_ctx ctx) { return ctx ? ctx->bytes_in_use : 0; }
+size_t hsps_ctx_bytes_in_use(hsps_ctx ctx) { return ctx ? ctx->bytes_in_use.load() : size_t(0); }
 
 int hsps_malloc(hsps_ctx ctx, size_t bytes, void** d_ptr) {
     return guarded([&] {
@@ -345,6 +353,7 @@ int hsps_malloc(hsps_ctx ctx, size_t bytes, void** d_ptr) {
             (void)hipGetLastError();
             throw Error(HSPS_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
         }
+        ctx_retain(ctx);
         return HSPS_OK;
     });
 }
@@ -355,6 +364,7 @@ int hsps_free(hsps_ctx ctx, void* d_ptr) {
             ctx->set_current();
             HSPS_HIP(hipStreamSynchronize(ctx->stream));
             HSPS_HIP(hipFree(d_ptr));
+            ctx_release(ctx);
         }
         return HSPS_OK;
     });
@@ -408,6 +418,7 @@ int hsps_csr_create(hsps_ctx ctx, int rows, int cols, int nnz, const int* rp, co
                           std::vector<double>(v, v + nnz));
         auto* h = new hsps_csr_s;
         static_cast<Csr&>(*h) = std::move(*A);
+        ctx_retain(ctx);
         *out = h;
         return HSPS_OK;
     });
@@ -416,9 +427,11 @@ int hsps_csr_create(hsps_ctx ctx, int rows, int cols, int nnz, const int* rp, co
 int hsps_csr_destroy(hsps_csr A) {
     return guarded([&] {
         if (A) {
-            (void)hipSetDevice(A->ctx->device);
-            (void)hipStreamSynchronize(A->ctx->stream);
+            Ctx* c = A->ctx;
+            (void)hipSetDevice(c->device);
+            (void)hipStreamSynchronize(c->stream);
             delete A;
+            ctx_release(c);
         }
         return HSPS_OK;
     });
@@ -473,8 +486,7 @@ int hsps_residual_f64(hsps_csr A, const double* b, const double* x, double* r) {
 
 static double ctx_dot(hsps_ctx ctx, int n, const double* x, const double* y) {
     ctx->set_current();
-    dot_f64_dev(n, x, y, ctx->red_out.as<double>(), ctx->red_partials.as<double>(), ctx->red_counter.as<unsigned>(),
-                ctx->stream);
+    dot_f64_dev(n, x, y, ctx->red_out.as<double>(), ctx->red_partials.as<double>(), ctx->stream);
     HSPS_HIP(hipMemcpyAsync(ctx->pinned, ctx->red_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HSPS_HIP(hipStreamSynchronize(ctx->stream));
     return ctx->pinned[0];

@@ -1,6 +1,6 @@
 // =============================================================================
 // device_common.hpp — gfx950 device building blocks: wave64 / workgroup /
-// grid reductions with a deterministic last-workgroup finish, and the CSR
+// two-phase grid reductions (partials + finish kernel), and the CSR
 // row-block SpMV template with pluggable epilogues.
 // =============================================================================
 #pragma once
@@ -47,44 +47,67 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
     }
 }
 
-// Grid-wide sum of NV doubles.  Every workgroup writes its partial; the last
-// one to arrive (agent-scope acq_rel ticket) sums all partials in a fixed
-// order — deterministic for a given grid — and returns true in its thread 0
-// with the totals in tot[].  The ticket counter is reset for the next use.
-// partials: NV * gridDim.x doubles; counter: one zeroed unsigned.
+// Two-phase grid reductions.  Phase 1 (in the producing kernel): every
+// workgroup writes its NV partial sums to partials[k * gridDim.x + block].
+// Phase 2 (a one-workgroup finish kernel launched next on the same stream):
+// sums the partials in a fixed order — deterministic for a given grid — and
+// runs the caller's scalar logic.  No cross-workgroup signalling inside a
+// kernel: on gfx950 an agent-scope release/acquire per workgroup costs an L2
+// writeback + invalidate (buffer_wbl2 / buffer_inv), which destroys the L2
+// reuse of the SpMV gathers; the kernel boundary gives visibility for free.
 template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter,
-                                         double (&tot)[NV]) {
+__device__ __forceinline__ void block_partial(double (&v)[NV], double* partials) {
     __shared__ double red[NV * (kBlock / 64)];
-    __shared__ int is_last;
     block_sum<NV>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
-        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+}
+
+// Phase 2 helper: sum nparts partials (NV planes) with the whole workgroup;
+// totals valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void finish_sum(const double* partials, int nparts, double (&tot)[NV]) {
+    __shared__ double red[NV * (kBlock / 64)];
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&partials[k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < NV; ++k) acc[k] += partials[(size_t)k * nparts + b];
     }
-    __syncthreads();  // red[] reuse
     block_sum<NV>(acc, red);
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-        *counter = 0u;
-        return true;
-    }
-    return false;
+    for (int k = 0; k < NV; ++k) tot[k] = acc[k];
 }
+
+// One-workgroup finish kernel: op.skip() gates it (uniform), op(tot) runs in
+// thread 0.
+template <int NV, class Op>
+__global__ void __launch_bounds__(kBlock) k_finish(const double* partials, int nparts, Op op) {
+    if (op.skip()) return;
+    double tot[NV];
+    finish_sum<NV>(partials, nparts, tot);
+    if (threadIdx.x == 0) op(tot);
+}
+
+template <int NV, class Op>
+inline void launch_finish(const double* partials, int nparts, const Op& op, hipStream_t s) {
+    hipLaunchKernelGGL((k_finish<NV, Op>), dim3(1), dim3(kBlock), 0, s, partials, nparts, op);
+}
+
+// Finish op that stores the NV totals (optionally behind a skip flag).
+template <int NV>
+struct OpStore {
+    double* out;
+    const int* flag = nullptr;
+    __device__ bool skip() const { return flag && *flag; }
+    __device__ void operator()(const double* tot) const {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] = tot[k];
+    }
+};
 
 template <class T> struct AccOf;
 template <> struct AccOf<double> { using type = double; };
@@ -96,8 +119,8 @@ template <> struct AccOf<float> { using type = float; };
 //    LDS with tpr lanes per row (tpr = pow2 <= 256/rows, <= 64).
 //  * long-row path (one row with nnz > kLdsNnz): the whole workgroup strides
 //    the row and reduces.
-// Epi supplies: skip(), row(row, sum, red[]), NV, finish(tot[]) and the
-// reduction buffers partials/counter when NV > 0.
+// Epi supplies: skip(), row(row, sum, red[]), NV, and the partials buffer
+// (NV * n_blocks doubles) when NV > 0; a finish kernel consumes the partials.
 template <typename V, typename T, class Epi>
 __global__ void __launch_bounds__(kBlock)
 k_spmv(const int* __restrict__ blocks, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -133,10 +156,7 @@ k_spmv(const int* __restrict__ blocks, const int* __restrict__ rp, const int* __
             if (l == 0) epi.row(row, s, red);
         }
     }
-    if constexpr (Epi::NV > 0) {
-        double tot[Epi::NV];
-        if (grid_sum<Epi::NV>(red, epi.partials, epi.counter, tot)) epi.finish(tot);
-    }
+    if constexpr (Epi::NV > 0) block_partial<Epi::NV>(red, epi.partials);
 }
 
 // ---- basic epilogues ------------------------------------------------------------
@@ -150,9 +170,7 @@ struct EpiStore : SkipFlag {
     static constexpr int NV = 0;
     T* y;
     double* partials = nullptr;
-    unsigned* counter = nullptr;
     template <class A> __device__ void row(int i, A s, double*) const { y[i] = (T)s; }
-    __device__ void finish(const double*) const {}
 };
 
 template <typename T>
@@ -160,27 +178,20 @@ struct EpiAdd : SkipFlag {
     static constexpr int NV = 0;
     T* y;
     double* partials = nullptr;
-    unsigned* counter = nullptr;
     template <class A> __device__ void row(int i, A s, double*) const { y[i] = (T)((A)y[i] + s); }
-    __device__ void finish(const double*) const {}
 };
 
-// r = b - A x; optional ||r||^2 -> *out
+// r = b - A x; with NORM, ||r||^2 partials for a finish kernel
 template <typename T, bool NORM>
 struct EpiResidual : SkipFlag {
     static constexpr int NV = NORM ? 1 : 0;
     const T* b;
     T* r;
-    double* out = nullptr;
     double* partials = nullptr;
-    unsigned* counter = nullptr;
     template <class A> __device__ void row(int i, A s, double* red) const {
         const T v = (T)((A)b[i] - s);
         r[i] = v;
         if constexpr (NORM) red[0] += (double)v * (double)v;
-    }
-    __device__ void finish(const double* tot) const {
-        if constexpr (NORM) *out = tot[0];
     }
 };
 
@@ -194,11 +205,9 @@ struct EpiSmooth : SkipFlag {
     T* xo;
     T omega;
     double* partials = nullptr;
-    unsigned* counter = nullptr;
     template <class A> __device__ void row(int i, A s, double*) const {
         xo[i] = (T)((A)xi[i] + (A)omega * (A)dinv[i] * ((A)rhs[i] - s));
     }
-    __device__ void finish(const double*) const {}
 };
 
 template <typename V, typename T, class Epi>

@@ -21,7 +21,7 @@ __global__ void k_fill(int n, double v, double* __restrict__ x) {
 // Aq -= alpha q + beta q_prev ; then ||Aq||^2 (adaptive_selector.cpp:80-83)
 __global__ void __launch_bounds__(kBlock)
 k_lanczos_orth(int n, double* __restrict__ aq, const double* __restrict__ q, const double* __restrict__ qp,
-               const double* alpha, double beta, double* out, double* partials, unsigned* counter) {
+               const double* alpha, double beta, double* partials) {
     const double a = *alpha;
     double red[1] = {0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -29,8 +29,7 @@ k_lanczos_orth(int n, double* __restrict__ aq, const double* __restrict__ q, con
         aq[i] = v;
         red[0] += v * v;
     }
-    double tot[1];
-    if (grid_sum<1>(red, partials, counter, tot)) *out = tot[0];
+    block_partial<1>(red, partials);
 }
 
 // q_prev = q ; q = aq / beta
@@ -69,10 +68,9 @@ static void features(const Csr& A, double* out) {
     Ctx* ctx = A.ctx;
     hipStream_t s = ctx->stream;
     const int K = std::min(n, 30);
-    DevBuf q(ctx, n * 8), qp(ctx, n * 8), aq(ctx, n * 8), scal(ctx, 64), counter(ctx, 64);
+    DevBuf q(ctx, n * 8), qp(ctx, n * 8), aq(ctx, n * 8), scal(ctx, 64);
     const int grid = std::max(reduction_blocks(n), 1);
     DevBuf partials(ctx, sizeof(double) * std::max(grid, 1) * 2);
-    HSPS_HIP(hipMemsetAsync(counter.p, 0, 64, s));
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, n, 1.0 / std::sqrt((double)n), q.as<double>());
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, n, 0.0, qp.as<double>());
     std::vector<double> al, be;
@@ -80,11 +78,12 @@ static void features(const Csr& A, double* out) {
     double h[2];
     for (int j = 0; j < K; ++j) {
         spmv_f64(A, q.as<double>(), aq.as<double>(), s);
-        dot_f64_dev(n, q.as<double>(), aq.as<double>(), scal.as<double>(), partials.as<double>(),
-                    counter.as<unsigned>(), s);
+        dot_f64_dev(n, q.as<double>(), aq.as<double>(), scal.as<double>(), partials.as<double>(), s);
         hipLaunchKernelGGL(k_lanczos_orth, dim3(grid), dim3(kBlock), 0, s, n, aq.as<double>(), q.as<double>(),
-                           qp.as<double>(), (const double*)scal.as<double>(), bj, scal.as<double>() + 1,
-                           partials.as<double>(), counter.as<unsigned>());
+                           qp.as<double>(), (const double*)scal.as<double>(), bj, partials.as<double>());
+        OpStore<1> op;
+        op.out = scal.as<double>() + 1;
+        launch_finish<1>(partials.as<double>(), grid, op, s);
         HSPS_CHECK_LAUNCH();
         HSPS_HIP(hipMemcpyAsync(h, scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         HSPS_HIP(hipStreamSynchronize(s));

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -67,16 +68,22 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// A context lives until hsps_ctx_destroy has been called AND every handle or
+// hsps_malloc allocation made through it has been released, so handles may
+// be destroyed in any order (e.g. by a garbage collector at exit).
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
-    size_t bytes_in_use = 0;
+    std::atomic<size_t> bytes_in_use{0};
+    std::atomic<int> refs{1};
     // scratch for standalone BLAS-1 reductions (stream ordered)
-    DevBuf red_partials, red_counter, red_out;
+    DevBuf red_partials, red_out;
     double* pinned = nullptr;  // host pinned readback area (64 doubles)
     void set_current() const { HSPS_HIP(hipSetDevice(device)); }
 };
+void ctx_retain(Ctx* c);
+void ctx_release(Ctx* c);
 
 // SpMV schedule constants (gfx950: 64-wide waves, 160 KB LDS per CU).
 constexpr int kBlock = 256;       // threads per workgroup (4 waves)
@@ -108,10 +115,10 @@ std::unique_ptr<Csr> make_csr(Ctx* ctx, int rows, int cols, std::vector<int> rp,
 // SpMV variants (spmv.hip)
 void spmv_f64(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip = nullptr);
 void spmv_f32(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip = nullptr);
-// r = b - A x (FP64), and optionally ||r||^2 into *d_out (device scalar).
+// r = b - A x (FP64), and optionally ||r||^2 into *d_nrm2 (device scalar;
+// needs a partials buffer of >= A.n_blocks doubles).
 void residual_f64(const Csr& A, const double* b, const double* x, double* r, hipStream_t s,
-                  double* d_nrm2 = nullptr, double* partials = nullptr, unsigned* counter = nullptr,
-                  const int* skip = nullptr);
+                  double* d_nrm2 = nullptr, double* partials = nullptr, const int* skip = nullptr);
 // y += A x (FP64 or FP32), y = A x as the 'add' flag says
 void spmv_add_f64(const Csr& A, const double* x, double* y, hipStream_t s, const int* skip);
 void spmv_add_f32(const Csr& A, const float* x, float* y, hipStream_t s, const int* skip);
@@ -123,8 +130,8 @@ void smooth_f32(const Csr& A, const float* dinv, float omega, const float* rhs, 
 void residual_f32(const Csr& A, const float* b, const float* x, float* r, hipStream_t s, const int* skip);
 
 // BLAS-1 (blas.hip)
-void dot_f64_dev(int n, const double* x, const double* y, double* d_out, double* partials, unsigned* counter,
-                 hipStream_t s);
+// x.y into *d_out (device scalar); partials >= reduction_blocks(n) doubles.
+void dot_f64_dev(int n, const double* x, const double* y, double* d_out, double* partials, hipStream_t s);
 void axpy_f64(int n, double a, const double* x, double* y, hipStream_t s);
 void axpby_f64(int n, double a, const double* x, double b, double* y, hipStream_t s);
 void scale_f64(int n, double a, double* x, hipStream_t s);

@@ -1,26 +1,28 @@
 // =============================================================================
 // krylov.hip — device-resident PCG and FGMRES(m).
 //
-// PCG (cg_solver.cpp:37-191): per iteration three kernels —
-//   K1  q = A p fused with the p.q reduction; its last workgroup forms alpha
-//       (and flags the p.q breakdown, :109-120);
-//   K2  x += alpha p, r -= alpha q, z = D^-1 r fused with the r.r and r.z
-//       reductions; its last workgroup forms rel = |r|/|b|, rho, beta and the
-//       convergence / rho-breakdown / iteration-limit decisions;
-//   K3  p = z + beta p.
+// PCG (cg_solver.cpp:37-191), per iteration:
+//   K1  q = A p fused with the p.q partial sums      F1  alpha (or p.q breakdown, :109-120)
+//   K2  x += alpha p, r -= alpha q, z = D^-1 r with the r.r / r.z partials
+//                                                    F2  rel = |r|/|b|, rho, beta and the
+//                                                        convergence / rho-breakdown /
+//                                                        iteration-limit decisions
+//   K3  p = z + beta p
+// F1/F2 are one-workgroup finish kernels that sum the partials in a fixed
+// order (deterministic) and update the device scalar state.
 // FGMRES(m) (fgmres_solver.cpp:36-241): right-preconditioned, modified
 // Gram-Schmidt with each axpy fused into the next dot, Givens rotations and
-// the least-squares residual kept on the device (FP64).
+// the least-squares residual kept on the device in FP64.
 //
 // Precision: HSPS_FP64 runs everything in FP64.  HSPS_MIXED runs the SpMV,
-// preconditioner and Krylov vectors in FP32 with FP64 reductions and
-// FP64 H/Givens; the FP64 true residual b - A x is recomputed
+// preconditioner and Krylov vectors in FP32 with FP64 reductions and FP64
+// H/Givens; the FP64 true residual b - A x is recomputed
 //   * CG: by reliable updates — whenever |r| falls below delta * max|r|
 //     since the last one, and to confirm convergence;
 //   * FGMRES: at every restart (each cycle is one refinement step), and a
 //     solve only ends when a cycle starts below tol.
-// The host polls a device flag every check_every iterations, one chunk ahead,
-// so the GPU never waits on the host within a chunk.
+// The host polls the device state every check_every iterations, one chunk
+// ahead, so the GPU never waits on the host within a chunk.
 // =============================================================================
 #include <algorithm>
 #include <chrono>
@@ -49,25 +51,26 @@ struct GmState {
     int jdone, cycle_done, m, pad;
 };
 
-// ---- small device helpers ---------------------------------------------------------
-template <class T> __device__ __forceinline__ T to_t(double v) { return (T)v; }
-
 // ---- CG kernels -----------------------------------------------------------------------
 template <class T>
 struct EpiCgPq {
     static constexpr int NV = 1;
     const T* p;
     T* q;
-    CgState* st;
+    const CgState* st;
     double* partials;
-    unsigned* counter;
     __device__ bool skip() const { return st->halt != 0; }
     template <class A> __device__ void row(int i, A s, double* red) const {
         const T qi = (T)s;
         q[i] = qi;
         red[0] += (double)p[i] * (double)qi;
     }
-    __device__ void finish(const double* tot) const {
+};
+
+struct OpAlpha {  // F1
+    CgState* st;
+    __device__ bool skip() const { return st->halt != 0; }
+    __device__ void operator()(const double* tot) const {
         const double pq = tot[0];
         if (fabs(pq) < kEps * 1e6 * fabs(st->rho)) {  // cg_solver.cpp:109
             st->flag = FLAG_BREAK_PQ;
@@ -82,10 +85,10 @@ struct EpiCgPq {
 // PREC: 0 none (z = r), 1 Jacobi (z = dinv r, inline), 2 external (z computed
 // after this kernel).  XT: type of the x being updated (FP64 x, or the FP32
 // low-order accumulator in mixed mode).
-template <class T, class XT, int PREC, bool MIXED>
+template <class T, class XT, int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_cg_update(int n, XT* __restrict__ x, const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ r,
-            const T* __restrict__ dinv, CgState* st, double* partials, unsigned* counter) {
+            const T* __restrict__ dinv, const CgState* st, double* partials) {
     if (st->halt) return;
     const double alpha = st->alpha;
     double red[2] = {0.0, 0.0};
@@ -98,63 +101,75 @@ k_cg_update(int n, XT* __restrict__ x, const T* __restrict__ p, const T* __restr
         if constexpr (PREC == 1) red[1] += rd * (double)(T)(dinv[i] * ri);
         if constexpr (PREC == 0) red[1] += rd * rd;
     }
-    double tot[2];
-    if (!grid_sum<2>(red, partials, counter, tot)) return;
-    const double rn = sqrt(tot[0]);
-    const double rel = rn / st->bnorm;
-    st->rel = rel;
-    st->iter += 1;
-    if (rel < st->tol) {  // cg_solver.cpp:143
-        if (MIXED) st->flag = FLAG_CHECK; else st->done = 1;
-        st->halt = 1;
-        return;
-    }
-    if (MIXED) {
-        st->rmax = fmax(st->rmax, rn);
-        if (rn < st->delta * st->rmax) {
-            st->flag = FLAG_RELIABLE;
+    block_partial<2>(red, partials);
+}
+
+template <int PREC, bool MIXED>
+struct OpUpdate {  // F2
+    CgState* st;
+    __device__ bool skip() const { return st->halt != 0; }
+    __device__ void operator()(const double* tot) const {
+        const double rn = sqrt(tot[0]);
+        const double rel = rn / st->bnorm;
+        st->rel = rel;
+        st->iter += 1;
+        if (rel < st->tol) {  // cg_solver.cpp:143
+            if (MIXED) st->flag = FLAG_CHECK; else st->done = 1;
             st->halt = 1;
             return;
         }
+        if (MIXED) {
+            st->rmax = fmax(st->rmax, rn);
+            if (rn < st->delta * st->rmax) {
+                st->flag = FLAG_RELIABLE;
+                st->halt = 1;
+                return;
+            }
+        }
+        if (PREC == 2) return;  // rho from the r.z pass after the external apply
+        st->rho_old = st->rho;
+        st->rho = tot[1];
+        if (fabs(st->rho_old) < kEps * 1e8 * fabs(st->rho)) {  // cg_solver.cpp:157
+            st->flag = FLAG_BREAK_RHO;
+            st->halt = 1;
+            return;
+        }
+        st->beta = st->rho / st->rho_old;
+        if (st->iter >= st->max_iter) st->halt = 1;
     }
-    if (PREC == 2) return;  // rho from k_cg_rz after the external apply
-    st->rho_old = st->rho;
-    st->rho = tot[1];
-    if (fabs(st->rho_old) < kEps * 1e8 * fabs(st->rho)) {  // cg_solver.cpp:157
-        st->flag = FLAG_BREAK_RHO;
-        st->halt = 1;
-        return;
-    }
-    st->beta = st->rho / st->rho_old;
-    if (st->iter >= st->max_iter) st->halt = 1;
-}
+};
 
-// rho = r.z for an external preconditioner (z already applied).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-k_cg_rz(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st, double* partials, unsigned* counter) {
-    if (st->halt) return;
+k_dot_t(int n, const T* __restrict__ a, const T* __restrict__ b, const int* skip, double* partials) {
+    if (skip && *skip) return;
     double red[1] = {0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) red[0] += (double)r[i] * (double)z[i];
-    double tot[1];
-    if (!grid_sum<1>(red, partials, counter, tot)) return;
-    st->rho_old = st->rho;
-    st->rho = tot[0];
-    if (fabs(st->rho_old) < kEps * 1e8 * fabs(st->rho)) {
-        st->flag = FLAG_BREAK_RHO;
-        st->halt = 1;
-        return;
-    }
-    st->beta = st->rho / st->rho_old;
-    if (st->iter >= st->max_iter) st->halt = 1;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) red[0] += (double)a[i] * (double)b[i];
+    block_partial<1>(red, partials);
 }
 
-// p = z + beta p (cg_solver.cpp:171); with init=true: p = z and rho = r.z
-// (cg_solver.cpp:77-85) from a freshly formed r.
+struct OpRz {  // rho = r.z for an external preconditioner
+    CgState* st;
+    __device__ bool skip() const { return st->halt != 0; }
+    __device__ void operator()(const double* tot) const {
+        st->rho_old = st->rho;
+        st->rho = tot[0];
+        if (fabs(st->rho_old) < kEps * 1e8 * fabs(st->rho)) {
+            st->flag = FLAG_BREAK_RHO;
+            st->halt = 1;
+            return;
+        }
+        st->beta = st->rho / st->rho_old;
+        if (st->iter >= st->max_iter) st->halt = 1;
+    }
+};
+
+// p = z + beta p (cg_solver.cpp:171); with INIT: p = z and the r.z partials
+// for rho (cg_solver.cpp:77-85) from a freshly formed r.
 template <class T, int PREC, bool INIT>
 __global__ void __launch_bounds__(kBlock)
 k_cg_p(int n, T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ dinv,
-       CgState* st, double* partials, unsigned* counter) {
+       const CgState* st, double* partials) {
     if (!INIT && st->halt) return;
     const double beta = INIT ? 0.0 : st->beta;
     double red[1] = {0.0};
@@ -170,14 +185,17 @@ k_cg_p(int n, T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ 
             p[i] = (T)((double)zi + beta * (double)p[i]);
         }
     }
-    if constexpr (INIT) {
-        double tot[1];
-        if (grid_sum<1>(red, partials, counter, tot)) {
-            st->rho = tot[0];
-            st->rho_old = tot[0];
-        }
-    }
+    if constexpr (INIT) block_partial<1>(red, partials);
 }
+
+struct OpInitRho {
+    CgState* st;
+    __device__ bool skip() const { return false; }
+    __device__ void operator()(const double* tot) const {
+        st->rho = tot[0];
+        st->rho_old = tot[0];
+    }
+};
 
 // Mixed: x64 += x_lo; x_lo = 0.
 __global__ void k_flush(int n, double* __restrict__ x, float* __restrict__ xlo) {
@@ -187,12 +205,11 @@ __global__ void k_flush(int n, double* __restrict__ x, float* __restrict__ xlo) 
     }
 }
 
-// Mixed: replace r by the FP64 true residual rounded to FP32, and rebuild
-// rho / beta from it.  The p direction is kept (reliable update).
+// Mixed: replace r by the FP64 true residual rounded to FP32 (r.z partials).
 template <int PREC>
 __global__ void __launch_bounds__(kBlock)
 k_cg_reliable(int n, const double* __restrict__ r64, float* __restrict__ r, const float* __restrict__ dinv,
-              CgState* st, double* partials, unsigned* counter) {
+              double* partials) {
     double red[1] = {0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const float ri = (float)r64[i];
@@ -200,28 +217,37 @@ k_cg_reliable(int n, const double* __restrict__ r64, float* __restrict__ r, cons
         if constexpr (PREC == 1) red[0] += (double)ri * (double)(dinv[i] * ri);
         if constexpr (PREC == 0) red[0] += (double)ri * (double)ri;
     }
-    double tot[1];
-    if (!grid_sum<1>(red, partials, counter, tot)) return;
-    const double rn = sqrt(st->rr_true);
-    const double rel = rn / st->bnorm;
-    st->reliable += 1;
-    st->rmax = rn;
-    if (st->flag == FLAG_CHECK && rel < st->tol) {
-        st->rel = rel;
-        st->done = 1;
-        st->halt = 1;
-        st->flag = FLAG_NONE;
-        return;
-    }
-    st->flag = FLAG_NONE;
-    st->halt = 0;
-    if (PREC != 2) {
-        st->rho_old = st->rho;
-        st->rho = tot[0];
-        st->beta = st->rho / st->rho_old;
-        if (st->iter >= st->max_iter) st->halt = 1;
-    }
+    block_partial<1>(red, partials);
 }
+
+// Rebuild rho / beta from the replaced r; the p direction is kept
+// (reliable update).  A CHECK whose true residual is below tol ends the solve.
+template <int PREC>
+struct OpReliable {
+    CgState* st;
+    __device__ bool skip() const { return false; }
+    __device__ void operator()(const double* tot) const {
+        const double rn = sqrt(st->rr_true);
+        const double rel = rn / st->bnorm;
+        st->reliable += 1;
+        st->rmax = rn;
+        if (st->flag == FLAG_CHECK && rel < st->tol) {
+            st->rel = rel;
+            st->done = 1;
+            st->halt = 1;
+            st->flag = FLAG_NONE;
+            return;
+        }
+        st->flag = FLAG_NONE;
+        st->halt = 0;
+        if (PREC != 2) {
+            st->rho_old = st->rho;
+            st->rho = tot[0];
+            st->beta = st->rho / st->rho_old;
+            if (st->iter >= st->max_iter) st->halt = 1;
+        }
+    }
+};
 
 // ---- FGMRES kernels ---------------------------------------------------------------------
 // V0 = r / beta, or stop when beta < tol_abs (fgmres_solver.cpp:82-96).
@@ -243,11 +269,11 @@ __global__ void k_gm_start(int n, const double* __restrict__ r, T* __restrict__ 
 }
 
 // One modified Gram-Schmidt step, with the previous step's axpy fused in:
-//   w -= h_prev * v_prev  (when h_prev given);  H[i][j] = w . v_i
+//   w -= h_prev * v_prev  (when h_prev given);  partials of w . v_i
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_mgs(int n, T* __restrict__ w, const T* __restrict__ vprev, const double* hprev, const T* __restrict__ vi,
-      double* hout, const int* skip, double* partials, unsigned* counter) {
+      const int* skip, double* partials) {
     if (*skip) return;
     const double h = hprev ? *hprev : 0.0;
     double red[1] = {0.0};
@@ -259,8 +285,30 @@ k_mgs(int n, T* __restrict__ w, const T* __restrict__ vprev, const double* hprev
         }
         red[0] += (double)wi * (double)vi[i];
     }
-    double tot[1];
-    if (grid_sum<1>(red, partials, counter, tot)) *hout = tot[0];
+    block_partial<1>(red, partials);
+}
+
+struct OpStoreH {  // H[i][j] = w . v_i
+    double* h;
+    const int* flag;
+    __device__ bool skip() const { return *flag != 0; }
+    __device__ void operator()(const double* tot) const { *h = tot[0]; }
+};
+
+// w -= H[j][j] v_j ; partials of |w|^2
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_mgs_last(int n, const double* hjj, T* __restrict__ w, const T* __restrict__ vj, const GmState* st,
+           double* partials) {
+    if (st->cycle_done) return;
+    const double h = *hjj;
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const T wi = (T)((double)w[i] - h * (double)vj[i]);
+        w[i] = wi;
+        red[0] += (double)wi * (double)wi;
+    }
+    block_partial<1>(red, partials);
 }
 
 __device__ void gm_givens(double f, double g, double& cs, double& sn, double& r) {  // fgmres_solver.cpp:36-45
@@ -274,36 +322,32 @@ __device__ void gm_rot(double& dx, double& dy, double cs, double sn) {  // fgmre
     dx = t;
 }
 
-// Last MGS step + norm; then (one thread) the Givens update of column j and
-// the least-squares residual estimate (fgmres_solver.cpp:122-148).
-template <class T>
-__global__ void __launch_bounds__(kBlock)
-k_mgs_last(int n, int j, T* __restrict__ w, const T* __restrict__ vj, double* H, double* cs, double* sn, double* g,
-           GmState* st, double* partials, unsigned* counter) {
-    if (st->cycle_done) return;
-    const int m = st->m;
-    const double h = H[j * m + j];
-    double red[1] = {0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const T wi = (T)((double)w[i] - h * (double)vj[i]);
-        w[i] = wi;
-        red[0] += (double)wi * (double)wi;
+// H[j+1][j] = |w|; Givens update of column j and the least-squares residual
+// estimate (fgmres_solver.cpp:126-148).
+struct OpGivens {
+    int j;
+    double* H;
+    double* cs;
+    double* sn;
+    double* g;
+    GmState* st;
+    __device__ bool skip() const { return st->cycle_done != 0; }
+    __device__ void operator()(const double* tot) const {
+        const int m = st->m;
+        const double hn = sqrt(tot[0]);
+        H[(j + 1) * m + j] = hn;
+        st->inv_h = hn > kEps ? 1.0 / hn : 0.0;  // lucky breakdown leaves V_{j+1} = 0
+        for (int i = 0; i < j; ++i) gm_rot(H[i * m + j], H[(i + 1) * m + j], cs[i], sn[i]);
+        double rv;
+        gm_givens(H[j * m + j], H[(j + 1) * m + j], cs[j], sn[j], rv);
+        H[j * m + j] = rv;
+        H[(j + 1) * m + j] = 0.0;
+        gm_rot(g[j], g[j + 1], cs[j], sn[j]);
+        st->res = fabs(g[j + 1]);
+        st->jdone = j + 1;
+        if (st->res < st->tol_abs) st->cycle_done = 1;
     }
-    double tot[1];
-    if (!grid_sum<1>(red, partials, counter, tot)) return;
-    const double hn = sqrt(tot[0]);
-    H[(j + 1) * m + j] = hn;
-    st->inv_h = hn > kEps ? 1.0 / hn : 0.0;  // lucky breakdown leaves V_{j+1} = 0
-    for (int i = 0; i < j; ++i) gm_rot(H[i * m + j], H[(i + 1) * m + j], cs[i], sn[i]);
-    double rv;
-    gm_givens(H[j * m + j], H[(j + 1) * m + j], cs[j], sn[j], rv);
-    H[j * m + j] = rv;
-    H[(j + 1) * m + j] = 0.0;
-    gm_rot(g[j], g[j + 1], cs[j], sn[j]);
-    st->res = fabs(g[j + 1]);
-    st->jdone = j + 1;
-    if (st->res < st->tol_abs) st->cycle_done = 1;
-}
+};
 
 template <class T>
 __global__ void k_gm_scale(int n, T* __restrict__ v, const GmState* st) {
@@ -363,8 +407,8 @@ static long long spmv_bytes(long long rows, long long nnz) { return (rows + 1 + 
 
 // True relative residual ||b - A x|| / ||b|| in FP64 (device), for stats.
 static double true_rel(const Csr& A, const double* b, const double* x, double* r64, double bnorm, double* d_out,
-                       double* partials, unsigned* counter, hipStream_t s) {
-    residual_f64(A, b, x, r64, s, d_out, partials, counter, nullptr);
+                       double* partials, hipStream_t s) {
+    residual_f64(A, b, x, r64, s, d_out, partials, nullptr);
     double h = 0.0;
     HSPS_HIP(hipMemcpyAsync(&h, d_out, sizeof(double), hipMemcpyDeviceToHost, s));
     HSPS_HIP(hipStreamSynchronize(s));
@@ -388,21 +432,18 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     DevBuf z(ctx, PREC == 2 ? nb * sizeof(T) : 0);
     DevBuf r64(ctx, nb * sizeof(double));
     DevBuf xlo(ctx, MIXED ? nb * sizeof(float) : 0);
-    const int red_grid = std::max(reduction_blocks(n), A.n_blocks);
-    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(red_grid, 1));
-    DevBuf counter(ctx, 64);
+    const int g = reduction_blocks(n);
+    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(std::max(g, A.n_blocks), 1));
     DevBuf dst(ctx, sizeof(CgState));
     DevBuf scal(ctx, 64);
     Pinned pin(sizeof(CgState) * 2);
     CgState* hst = static_cast<CgState*>(pin.p);
     CgState* d = dst.as<CgState>();
     double* part = partials.as<double>();
-    unsigned* cnt = counter.as<unsigned>();
-    HSPS_HIP(hipMemsetAsync(counter.p, 0, 64, s));
 
     // r = b - A x, ||r||^2, ||b||^2  (cg_solver.cpp:52-57)
-    residual_f64(A, b, x, r64.as<double>(), s, &d->rr0, part, cnt, nullptr);
-    dot_f64_dev(n, b, b, &d->bb, part, cnt, s);
+    residual_f64(A, b, x, r64.as<double>(), s, &d->rr0, part, nullptr);
+    dot_f64_dev(n, b, b, &d->bb, part, s);
     HSPS_HIP(hipMemcpyAsync(hst, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
     HSPS_HIP(hipStreamSynchronize(s));
     const double bnorm = std::sqrt(hst->bb), r0 = std::sqrt(hst->rr0);
@@ -441,62 +482,67 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     T* p_ = pv.as<T>();
     T* q_ = q.as<T>();
     T* z_ = z.as<T>();
-    const int g = reduction_blocks(n);
-    const Csr& Am = A;
 
     // r (working precision), z = M r, p = z, rho = r.z  (cg_solver.cpp:77-85)
-    auto start_direction = [&](bool init_rho) {
+    auto start_direction = [&]() {
         if (MIXED) copy_f64_to_f32(n, r64.as<double>(), (float*)rv, s);
         else HSPS_HIP(hipMemcpyAsync(rv, r64.p, nb * sizeof(double), hipMemcpyDeviceToDevice, s));
         if (PREC == 2) {
             if (MIXED) precond_apply_f32(*M, (const float*)rv, (float*)z_, s, nullptr);
             else precond_apply_f64(*M, (const double*)rv, (double*)z_, s, nullptr);
         }
-        (void)init_rho;
         switch (PREC) {
-            case 0: hipLaunchKernelGGL((k_cg_p<T, 0, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
-            case 1: hipLaunchKernelGGL((k_cg_p<T, 1, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
-            default: hipLaunchKernelGGL((k_cg_p<T, 2, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+            case 0: hipLaunchKernelGGL((k_cg_p<T, 0, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part); break;
+            case 1: hipLaunchKernelGGL((k_cg_p<T, 1, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part); break;
+            default: hipLaunchKernelGGL((k_cg_p<T, 2, true>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part); break;
         }
+        launch_finish<1>(part, g, OpInitRho{d}, s);
         HSPS_CHECK_LAUNCH();
     };
-    start_direction(true);
+    start_direction();
     if (MIXED) fill_zero_f32(n, xlo.as<float>(), s);
 
     auto spmv_pq = [&]() {
         EpiCgPq<T> e;
-        e.p = p_; e.q = q_; e.st = d; e.partials = part; e.counter = cnt;
-        if (MIXED) launch_spmv(Am, Am.v32.as<float>(), (const float*)p_, e, s);
-        else launch_spmv(Am, Am.v64.as<double>(), (const double*)p_, e, s);
+        e.p = p_; e.q = q_; e.st = d; e.partials = part;
+        if (MIXED) launch_spmv(A, A.v32.as<float>(), (const float*)p_, e, s);
+        else launch_spmv(A, A.v64.as<double>(), (const double*)p_, e, s);
+        launch_finish<1>(part, A.n_blocks, OpAlpha{d}, s);
     };
     auto update = [&]() {
         if (MIXED) {
             float* xl = xlo.as<float>();
             switch (PREC) {
-                case 0: hipLaunchKernelGGL((k_cg_update<T, float, 0, true>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part, cnt); break;
-                case 1: hipLaunchKernelGGL((k_cg_update<T, float, 1, true>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part, cnt); break;
-                default: hipLaunchKernelGGL((k_cg_update<T, float, 2, true>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part, cnt); break;
+                case 0: hipLaunchKernelGGL((k_cg_update<T, float, 0>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part); break;
+                case 1: hipLaunchKernelGGL((k_cg_update<T, float, 1>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part); break;
+                default: hipLaunchKernelGGL((k_cg_update<T, float, 2>), dim3(g), dim3(kBlock), 0, s, n, xl, p_, q_, rv, dinv, d, part); break;
             }
         } else {
             switch (PREC) {
-                case 0: hipLaunchKernelGGL((k_cg_update<T, double, 0, false>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part, cnt); break;
-                case 1: hipLaunchKernelGGL((k_cg_update<T, double, 1, false>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part, cnt); break;
-                default: hipLaunchKernelGGL((k_cg_update<T, double, 2, false>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part, cnt); break;
+                case 0: hipLaunchKernelGGL((k_cg_update<T, double, 0>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part); break;
+                case 1: hipLaunchKernelGGL((k_cg_update<T, double, 1>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part); break;
+                default: hipLaunchKernelGGL((k_cg_update<T, double, 2>), dim3(g), dim3(kBlock), 0, s, n, x, p_, q_, rv, dinv, d, part); break;
             }
+        }
+        switch (PREC) {
+            case 0: launch_finish<2>(part, g, OpUpdate<0, MIXED>{d}, s); break;
+            case 1: launch_finish<2>(part, g, OpUpdate<1, MIXED>{d}, s); break;
+            default: launch_finish<2>(part, g, OpUpdate<2, MIXED>{d}, s); break;
         }
         HSPS_CHECK_LAUNCH();
         if (PREC == 2) {
             if (MIXED) precond_apply_f32(*M, (const float*)rv, (float*)z_, s, &d->halt);
             else precond_apply_f64(*M, (const double*)rv, (double*)z_, s, &d->halt);
-            hipLaunchKernelGGL(k_cg_rz<T>, dim3(g), dim3(kBlock), 0, s, n, rv, z_, d, part, cnt);
+            hipLaunchKernelGGL(k_dot_t<T>, dim3(g), dim3(kBlock), 0, s, n, (const T*)rv, (const T*)z_, &d->halt, part);
+            launch_finish<1>(part, g, OpRz{d}, s);
             HSPS_CHECK_LAUNCH();
         }
     };
     auto pupdate = [&]() {
         switch (PREC) {
-            case 0: hipLaunchKernelGGL((k_cg_p<T, 0, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
-            case 1: hipLaunchKernelGGL((k_cg_p<T, 1, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
-            default: hipLaunchKernelGGL((k_cg_p<T, 2, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part, cnt); break;
+            case 0: hipLaunchKernelGGL((k_cg_p<T, 0, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part); break;
+            case 1: hipLaunchKernelGGL((k_cg_p<T, 1, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part); break;
+            default: hipLaunchKernelGGL((k_cg_p<T, 2, false>), dim3(g), dim3(kBlock), 0, s, n, p_, rv, z_, dinv, d, part); break;
         }
         HSPS_CHECK_LAUNCH();
     };
@@ -505,9 +551,7 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     Event ev[2];
     int k = 0;
     bool pending = false;
-    long long F = 0, B = 0;
     auto read_state = [&]() {
-        HSPS_HIP(hipStreamSynchronize(s));
         HSPS_HIP(hipMemcpyAsync(hst, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
         HSPS_HIP(hipStreamSynchronize(s));
     };
@@ -532,37 +576,40 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
         pending = false;
         k = 0;
         CgState cur = hst[0];
-        if (cur.done || cur.iter >= cur.max_iter) {
-            if (!(cur.flag == FLAG_CHECK || cur.flag == FLAG_RELIABLE) || cur.done) break;
-        }
+        const bool refine = MIXED && (cur.flag == FLAG_CHECK || cur.flag == FLAG_RELIABLE);
+        if (cur.done || (cur.iter >= cur.max_iter && !refine)) break;
         if (cur.flag == FLAG_BREAK_PQ || cur.flag == FLAG_BREAK_RHO) {
             // restart from the current x (cg_solver.cpp:109-120, 157-167)
             if (MIXED) hipLaunchKernelGGL(k_flush, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, xlo.as<float>());
-            residual_f64(A, b, x, r64.as<double>(), s, nullptr, nullptr, nullptr, nullptr);
-            start_direction(true);
+            residual_f64(A, b, x, r64.as<double>(), s, nullptr, nullptr, nullptr);
+            start_direction();
             cur.restarts += 1;
             cur.flag = FLAG_NONE;
             cur.halt = cur.iter >= cur.max_iter ? 1 : 0;
-            // write back flag/halt/restarts without disturbing rho (set by k_cg_p<INIT>)
-            HSPS_HIP(hipMemcpyAsync(&d->flag, &cur.flag, sizeof(int), hipMemcpyHostToDevice, s));
-            HSPS_HIP(hipMemcpyAsync(&d->halt, &cur.halt, sizeof(int), hipMemcpyHostToDevice, s));
+            int upd[3] = {cur.halt, 0, cur.flag};
+            HSPS_HIP(hipMemcpyAsync(&d->halt, &upd[0], sizeof(int), hipMemcpyHostToDevice, s));
+            HSPS_HIP(hipMemcpyAsync(&d->flag, &upd[2], sizeof(int), hipMemcpyHostToDevice, s));
             HSPS_HIP(hipMemcpyAsync(&d->restarts, &cur.restarts, sizeof(int), hipMemcpyHostToDevice, s));
             HSPS_HIP(hipStreamSynchronize(s));
             if (cur.halt) break;
             continue;
         }
-        if (MIXED && (cur.flag == FLAG_RELIABLE || cur.flag == FLAG_CHECK)) {
+        if (refine) {
             hipLaunchKernelGGL(k_flush, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, xlo.as<float>());
-            residual_f64(A, b, x, r64.as<double>(), s, &d->rr_true, part, cnt, nullptr);
+            residual_f64(A, b, x, r64.as<double>(), s, &d->rr_true, part, nullptr);
             switch (PREC) {
-                case 0: hipLaunchKernelGGL(k_cg_reliable<0>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, d, part, cnt); break;
-                case 1: hipLaunchKernelGGL(k_cg_reliable<1>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, d, part, cnt); break;
-                default: hipLaunchKernelGGL(k_cg_reliable<2>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, d, part, cnt); break;
+                case 0: hipLaunchKernelGGL(k_cg_reliable<0>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, part);
+                        launch_finish<1>(part, g, OpReliable<0>{d}, s); break;
+                case 1: hipLaunchKernelGGL(k_cg_reliable<1>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, part);
+                        launch_finish<1>(part, g, OpReliable<1>{d}, s); break;
+                default: hipLaunchKernelGGL(k_cg_reliable<2>, dim3(g), dim3(kBlock), 0, s, n, r64.as<double>(), (float*)rv, (const float*)dinv, part);
+                        launch_finish<1>(part, g, OpReliable<2>{d}, s); break;
             }
             HSPS_CHECK_LAUNCH();
             if (PREC == 2) {
                 precond_apply_f32(*M, (const float*)rv, (float*)z_, s, &d->halt);
-                hipLaunchKernelGGL(k_cg_rz<T>, dim3(g), dim3(kBlock), 0, s, n, rv, z_, d, part, cnt);
+                hipLaunchKernelGGL(k_dot_t<T>, dim3(g), dim3(kBlock), 0, s, n, (const T*)rv, (const T*)z_, &d->halt, part);
+                launch_finish<1>(part, g, OpRz{d}, s);
                 HSPS_CHECK_LAUNCH();
             }
             pupdate();
@@ -579,13 +626,11 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     st.restarts = fin.restarts;
     st.reliable_updates = fin.reliable;
     st.final_residual = fin.rel;
-    st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, cnt, s);
+    st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, s);
     const int conv = MIXED ? (st.true_residual < p.tol && fin.done) : (fin.rel < p.tol);
     const long long it = fin.iter;
     const size_t vb = MIXED ? 4 : 8;
-    F = it * (2LL * A.nnz + 10LL * n);
-    B = it * (spmv_bytes(n, A.nnz) + 7LL * n * (long long)vb);
-    finish_stats(&st, F, B);
+    finish_stats(&st, it * (2LL * A.nnz + 10LL * n), it * (spmv_bytes(n, A.nnz) + 7LL * n * (long long)vb));
     return finish(conv);
 }
 
@@ -605,9 +650,8 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     DevBuf V(ctx, ld * (m + 1) * sizeof(T));
     DevBuf Z(ctx, has_m ? ld * m * sizeof(T) : 0);
     DevBuf r64(ctx, ld * sizeof(double));
-    const int red_grid = std::max(reduction_blocks(n), A.n_blocks);
-    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(red_grid, 1));
-    DevBuf counter(ctx, 64);
+    const int g = reduction_blocks(n);
+    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(std::max(g, A.n_blocks), 1));
     DevBuf dst(ctx, sizeof(GmState));
     DevBuf Hb(ctx, sizeof(double) * (size_t)(m + 1) * m);
     DevBuf csb(ctx, sizeof(double) * m), snb(ctx, sizeof(double) * m), gb(ctx, sizeof(double) * (m + 1)),
@@ -617,9 +661,7 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     GmState* hst = static_cast<GmState*>(pin.p);
     GmState* d = dst.as<GmState>();
     double* part = partials.as<double>();
-    unsigned* cnt = counter.as<unsigned>();
     double* H = Hb.as<double>();
-    HSPS_HIP(hipMemsetAsync(counter.p, 0, 64, s));
     HSPS_HIP(hipMemsetAsync(Hb.p, 0, Hb.bytes, s));
     T* Vp = V.as<T>();
     T* Zp = has_m ? Z.as<T>() : Vp;
@@ -627,7 +669,7 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     auto Zk = [&](int k) { return Zp + (size_t)k * ld; };
 
     // ||b|| (fgmres_solver.cpp:182-188)
-    dot_f64_dev(n, b, b, scal.as<double>(), part, cnt, s);
+    dot_f64_dev(n, b, b, scal.as<double>(), part, s);
     double bb = 0.0;
     HSPS_HIP(hipMemcpyAsync(&bb, scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
     HSPS_HIP(hipStreamSynchronize(s));
@@ -649,7 +691,6 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     HSPS_HIP(hipMemcpyAsync(d, &init, sizeof init, hipMemcpyHostToDevice, s));
     HSPS_HIP(hipStreamSynchronize(s));
 
-    const int g = reduction_blocks(n);
     int total = 0, cycles = 0, conv = 0;
     double rel = 1.0;
     bool first = true;
@@ -657,7 +698,7 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     Event ev[2];
     while (total < p.max_iter) {
         const int cm = std::min(m, p.max_iter - total);
-        residual_f64(A, b, x, r64.as<double>(), s, &d->beta2, part, cnt, nullptr);
+        residual_f64(A, b, x, r64.as<double>(), s, &d->beta2, part, nullptr);
         hipLaunchKernelGGL(k_gm_start<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r64.as<double>(), Vk(0), d, gb.as<double>());
         HSPS_CHECK_LAUNCH();
         int k = 0;
@@ -674,13 +715,13 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
             for (int i = 0; i <= j; ++i) {
                 const double* hprev = i > 0 ? H + (size_t)(i - 1) * m + j : nullptr;
                 const T* vprev = i > 0 ? Vk(i - 1) : nullptr;
-                hipLaunchKernelGGL(k_mgs<T>, dim3(g), dim3(kBlock), 0, s, n, w, vprev, hprev, (const T*)Vk(i),
-                                   H + (size_t)i * m + j, skip, part, cnt);
+                hipLaunchKernelGGL(k_mgs<T>, dim3(g), dim3(kBlock), 0, s, n, w, vprev, hprev, (const T*)Vk(i), skip, part);
+                launch_finish<1>(part, g, OpStoreH{H + (size_t)i * m + j, skip}, s);
                 HSPS_CHECK_LAUNCH();
             }
-            hipLaunchKernelGGL(k_mgs_last<T>, dim3(g), dim3(kBlock), 0, s, n, j, w, (const T*)Vk(j), H,
-                               csb.as<double>(), snb.as<double>(), gb.as<double>(), d, part, cnt);
-            HSPS_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_mgs_last<T>, dim3(g), dim3(kBlock), 0, s, n, (const double*)(H + (size_t)j * m + j), w,
+                               (const T*)Vk(j), (const GmState*)d, part);
+            launch_finish<1>(part, g, OpGivens{j, H, csb.as<double>(), snb.as<double>(), gb.as<double>(), d}, s);
             hipLaunchKernelGGL(k_gm_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, w, (const GmState*)d);
             HSPS_CHECK_LAUNCH();
             if ((j + 1) % steps_per_poll == 0 && j + 1 < cm) {
@@ -721,7 +762,7 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     st.iterations = total;
     st.restarts = cycles - 1;
     st.final_residual = rel;
-    st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, cnt, s);
+    st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, s);
     if (MIXED) conv = conv && st.true_residual < p.tol;
     const size_t vb = MIXED ? 4 : 8;
     finish_stats(&st, (long long)total * (2LL * A.nnz + 4LL * n),

@@ -281,7 +281,7 @@ static void smooth(const Csr& A, const float* d, double w, const float* rhs, con
     smooth_f32(A, d, (float)w, rhs, xi, xo, s, skip);
 }
 static void resid(const Csr& A, const double* b, const double* x, double* r, hipStream_t s, const int* skip) {
-    residual_f64(A, b, x, r, s, nullptr, nullptr, nullptr, skip);
+    residual_f64(A, b, x, r, s, nullptr, nullptr, skip);
 }
 static void resid(const Csr& A, const float* b, const float* x, float* r, hipStream_t s, const int* skip) {
     residual_f32(A, b, x, r, s, skip);
@@ -433,6 +433,7 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
         }
         if (setup_seconds)
             *setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        ctx_retain(M->ctx);
         *out = M;
         return HSPS_OK;
     });
@@ -441,9 +442,11 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
 int hsps_precond_destroy(hsps_precond M) {
     return guarded([&] {
         if (M) {
-            (void)hipSetDevice(M->ctx->device);
-            (void)hipStreamSynchronize(M->ctx->stream);
+            Ctx* c = M->ctx;
+            (void)hipSetDevice(c->device);
+            (void)hipStreamSynchronize(c->stream);
             delete M;
+            ctx_release(c);
         }
         return HSPS_OK;
     });

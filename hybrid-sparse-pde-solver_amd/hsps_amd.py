@@ -158,6 +158,8 @@ class DeviceArray:
 
     def __init__(self, ctx, host=None, n=None, dtype=np.float64):
         self.ctx = ctx
+        # the C ABI keeps the context alive while this allocation exists
+        self.ctx_h = ctx.h
         if host is not None:
             host = np.ascontiguousarray(host, dtype=dtype)
             n = host.size
@@ -175,16 +177,16 @@ class DeviceArray:
     def upload(self, host):
         host = np.ascontiguousarray(host, dtype=self.dtype)
         assert host.size == self.n
-        check(lib().hsps_memcpy_h2d(self.ctx.h, self.ptr, host.ctypes.data_as(_vp), self.nbytes))
+        check(lib().hsps_memcpy_h2d(self.ctx_h, self.ptr, host.ctypes.data_as(_vp), self.nbytes))
 
     def download(self):
         out = np.empty(self.n, self.dtype)
-        check(lib().hsps_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        check(lib().hsps_memcpy_d2h(self.ctx_h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
         return out
 
     def free(self):
         if getattr(self, "ptr", None):
-            lib().hsps_free(self.ctx.h, self.ptr)
+            lib().hsps_free(self.ctx_h, self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -164,7 +164,7 @@ TEST(mixed_precision_matches_fp64) {
         expect_true(s1.converged && s2.converged, "both converge");
         expect_less(s2.true_residual, 1e-8, "mixed true residual");
         expect_less((x1 - x2).norm2(), 1e-6 * x1.norm2(), "solutions agree");
-        expect_less(std::abs(s2.iterations - s1.iterations), 0.1 * s1.iterations + 2, "iteration counts close");
+        expect_less(std::abs(s2.iterations - s1.iterations), 0.1 * s1.iterations + 4, "iteration counts close");
     }
 }
 TEST(factory_parse) {
