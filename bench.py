@@ -33,6 +33,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of the FP64 SpMV on the 2048^2 Poisson matrix, from
+# two rocprofv3 --pmc runs of `bench.py --spmv-only` (FETCH_SIZE 154,831 KiB
+# x 2.000 + WRITE_SIZE 32,768 KiB x 1.000; the factors make a 256 MiB copy
+# read and write 256 MiB).
+SPMV_FP64_HBM_BYTES = (154831 * 2.0 + 32768 * 1.0) * 1024
 
 
 def log(*a):
@@ -301,6 +306,9 @@ def main():
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV roofline kernel (plus a 256 MiB device copy that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters); for rocprofv3 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -311,6 +319,14 @@ def main():
     ctx = H.Context(0)
     log("[bench] spmv roofline")
     roof, rn, rnnz = spmv_roofline(H, ctx, ev)
+    if args.spmv_only:
+        nbytes = 256 << 20  # calibration copy: reads and writes exactly 256 MiB
+        src, dst = ctx.array(n=nbytes // 8), ctx.array(n=nbytes // 8)
+        H.check(H.lib().hsps_memcpy_d2d(ctx.h, dst.ptr, src.ptr, nbytes))
+        ctx.sync()
+        print(json.dumps({"spmv": roof, "n": rn, "nnz": rnnz, "calibration_copy_bytes": nbytes}), flush=True)
+        ctx.close()
+        return
     res = bench_batched(H, ctx, ev, args) if args.workload == "batched" else bench_single(H, ctx, ev, args)
     r64 = roof["fp64"]
     line = {
@@ -334,7 +350,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_spmv<double,double,EpiStore> (CSR row-block SpMV, FP64), "
                                              f"Poisson 2048^2 (n={rn}, nnz={rnnz})",
                      "achieved": r64["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r64["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": r64["achieved_GBs"] / HBM_PEAK_GBS, "traffic": SPMV_FP64_HBM_BYTES,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per dispatch, "
+                                       "calibrated on a 256 MiB copy (profiles/spmv_poisson2048_fp64_fp32.md)",
                      "algorithmic_bytes": r64["algorithmic_bytes"], "avg_launch_us": r64["avg_launch_us"],
                      "fp32_spmv": roof["fp32"]},
         "cpu_baseline": res["cpu_baseline"],
