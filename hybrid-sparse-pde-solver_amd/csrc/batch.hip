@@ -11,14 +11,18 @@
 //    only — no launches per iteration, vectors stay in L2;
 //  * large systems: cut into tiles of <= tile_rows rows (one workgroup per
 //    tile, tile_rows autotuned), grouped into waves by a working-set budget
-//    (default: one wave).  Per iteration five launches over the wave's tiles:
-//      K1  q = A p, p.q partials  (FLUSH state: FP64 true residual r = b - A x)
+//    (default: one wave).  Per iteration four launches over the wave's tiles:
+//      K1  p' = D^-1 r + beta p (two p buffers), q = A p', p'.q partials
+//          (FLUSH state: FP64 true residual r = b - A (x64 + x_lo))
 //      F1  per-system finish: alpha (or breakdown / reliable-update decisions)
-//      K2  x += alpha p, r -= alpha q, r.r / r.z partials
+//      K2  x += alpha p', r -= alpha q, r.r / r.z partials
+//          (after a FLUSH: x64 += x_lo)
 //      F2  per-system finish: rel, rho, beta, convergence
-//      K3  p = D^-1 r + beta p   (FLUSH state: x64 += x_lo)
 //    The chunk of iterations is captured once per wave in a hipGraph and
 //    replayed; the host polls the wave's states one chunk behind.
+// Systems on <= 8 diagonals are stored by diagonal (constant-coefficient
+// stencils as one value per diagonal + a presence byte per row), so their
+// iterations move ~34 B per row (FP32) instead of ~96 B with CSR.
 // The two classes run concurrently on two streams.
 // Mixed precision: FP32 matrix values and Krylov vectors, FP64 reductions and
 // scalars, x accumulated in FP64 through an FP32 low-order part flushed at
@@ -32,6 +36,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <type_traits>
 
 #include "device_common.hpp"
 
@@ -40,13 +45,20 @@ namespace hsps_dev {
 static constexpr double kEps = std::numeric_limits<double>::epsilon();
 constexpr int kSmallRows = 20000;   // persistent one-workgroup path up to this size
 constexpr int kSmallThreads = 1024;
+constexpr int kMaxDiags = 8;        // DIA storage when a system's nonzeros lie on <= 8 diagonals
 
-enum : int { BS_NORMAL = 0, BS_FLUSH = 1, BS_PUPDATE = 2 };
+// NORMAL: K1 updates p and forms q = A p, K2 updates x and r.
+// FLUSH: K1 replaces r by the FP64 true residual b - A (x64 + x_lo).
+// FLUSHED: after a FLUSH, K2 folds x_lo into x64; K1 then proceeds as NORMAL.
+enum : int { BS_NORMAL = 0, BS_FLUSH = 1, BS_FLUSHED = 2 };
 enum : int { BR_NONE = 0, BR_RELIABLE = 1, BR_CHECK = 2, BR_RESTART = 3 };
+// storage of a system inside the batch
+enum : int { SK_CSR = 0, SK_DIA = 1, SK_STENCIL = 2 };
 
 struct BState {
     double rho, rho_old, alpha, beta, bnorm, rel, rmax, true_rel;
     int iter, halt, done, flag, reason, reliable, restarts, conv;
+    int pcur;  // which of the two p buffers holds the current search direction
 };
 
 struct BArgs {
@@ -63,6 +75,25 @@ struct BArgs {
     const int* sys_t1;
     const int* sys_row0;  // internal order
     const int* sys_row1;
+    // Diagonal storage of systems whose nonzeros lie on <= kMaxDiags diagonals
+    // (column = row + off[s*kMaxDiags+d], offsets ascending):
+    //  SK_DIA:     value of diagonal d at row li of system s in
+    //              dia[base[s] + d * n_s + li] (zero where absent);
+    //  SK_STENCIL: every diagonal carries one value uv[s*kMaxDiags+d]; bit d of
+    //              umask[row] says whether the row holds that diagonal.
+    const int* dia_kind;       // SK_* per system
+    const int* dia_nd;         // diagonals per system
+    const int* dia_off;        // S * kMaxDiags offsets
+    const long long* dia_base;
+    const double* dia64;
+    const float* dia32;
+    const double* uv64;        // S * kMaxDiags
+    const float* uv32;
+    const unsigned char* umask;  // N
+    // Jacobi with one diagonal value for the whole system: D^-1 = cd[s]
+    const int* cd_ok;
+    const double* cd64;
+    const float* cd32;
     BState* st;
     double* partials;
     int T;  // total tiles
@@ -74,29 +105,334 @@ template <class T> struct Prec;
 template <> struct Prec<double> {
     static __device__ const double* val(const BArgs& B) { return B.v64; }
     static __device__ const double* dinv(const BArgs& B) { return B.dinv64; }
+    static __device__ const double* dia(const BArgs& B) { return B.dia64; }
+    static __device__ const double* uv(const BArgs& B) { return B.uv64; }
+    static __device__ const double* cd(const BArgs& B) { return B.cd64; }
 };
 template <> struct Prec<float> {
     static __device__ const float* val(const BArgs& B) { return B.v32; }
     static __device__ const float* dinv(const BArgs& B) { return B.dinv32; }
+    static __device__ const float* dia(const BArgs& B) { return B.dia32; }
+    static __device__ const float* uv(const BArgs& B) { return B.uv32; }
+    static __device__ const float* cd(const BArgs& B) { return B.cd32; }
 };
 
-// Row sums of a tile (rows r0..r1) of A*x, LDS-staged; f(row, sum) once per
-// row from one lane.  Same staging rule as the single-system SpMV.
-template <class Acc, class V, class X, class F>
+// Row access of a diagonal-stored system, loaded once per workgroup.  Rows
+// are read without column indices; every load is coalesced across rows.
+template <class T>
+struct Stencil {
+    int kind, nd, row0, n_s;
+    int halo;  // max |offset|
+    int off[kMaxDiags];  // constant-indexed copies (unrolled paths)
+    T uv[kMaxDiags];
+    const int* __restrict__ offg;  // the same in memory (loop over diagonals)
+    const T* __restrict__ uvg;
+    const T* __restrict__ dia;
+    const unsigned char* __restrict__ mask;
+    __device__ __forceinline__ Stencil(const BArgs& B, int s) {
+        kind = B.dia_kind[s];
+        nd = B.dia_nd[s];
+        row0 = B.sys_row0[s];
+        n_s = B.sys_row1[s] - row0;
+        dia = Prec<T>::dia(B) + B.dia_base[s];
+        mask = B.umask;
+        const T* u = Prec<T>::uv(B) + (size_t)s * kMaxDiags;
+        offg = B.dia_off + s * kMaxDiags;
+        uvg = u;
+        halo = 0;
+#pragma unroll
+        for (int d = 0; d < kMaxDiags; ++d) {
+            off[d] = B.dia_off[s * kMaxDiags + d];
+            uv[d] = u[d];
+            if (d < nd) halo = max(halo, abs(off[d]));
+        }
+    }
+    // f(integral_constant<ND>): ND = 5 unrolls the row (1D and 2D five-point
+    // stencils; padding diagonals contribute nothing); ND = 0 walks the
+    // diagonals in a loop (wider stencils), keeping the kernels' register
+    // budget at what the five-point case needs.
+    template <class F>
+    __device__ __forceinline__ void dispatch(F&& f) const {
+        if (nd <= 5) f(std::integral_constant<int, 5>{});
+        else f(std::integral_constant<int, 0>{});
+    }
+    // Row i of A x, x(j) = xv(j): sum over the row's diagonals in ascending
+    // column order of A(i, j) * x(j) with rounded products (no fma
+    // contraction), i.e. exactly the CSR loop sum += v * x in column order.
+    // Every x(j) is read unconditionally at an index clamped into the system
+    // and masked afterwards, so the loads of a row are all in flight at once.
+    template <int ND, class XF>
+    __device__ __forceinline__ T row(int i, XF&& xv) const {
+#pragma clang fp contract(off)
+        const int li = i - row0, last = row0 + n_s - 1;
+        if constexpr (ND == 0) {
+            const unsigned m = kind == SK_STENCIL ? mask[i] : 0u;
+            T sum = 0;
+            for (int d = 0; d < nd; ++d) {
+                const int od = offg[d];
+                const int lj = li + od;
+                if (kind == SK_STENCIL ? ((m >> d) & 1u) != 0 : (lj >= 0 && lj < n_s)) {
+                    const T a = kind == SK_STENCIL ? uvg[d] : dia[(size_t)d * n_s + li];
+                    const T t = a * (T)xv(i + od);
+                    sum = sum + t;
+                }
+            }
+            return sum;
+        } else {
+            T xs[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) xs[d] = (T)xv(min(max(i + off[d], row0), last));
+            T sum = 0;
+            if (kind == SK_STENCIL) {
+                const unsigned m = mask[i];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const T t = uv[d] * xs[d];
+                    sum = ((m >> d) & 1u) ? sum + t : sum;
+                }
+            } else {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const int lj = li + off[d];
+                    const bool ok = d < nd && lj >= 0 && lj < n_s;
+                    const T t = dia[(size_t)(ok ? d : 0) * n_s + li] * xs[d];
+                    sum = ok ? sum + t : sum;
+                }
+            }
+            return sum;
+        }
+    }
+};
+
+// Rows r0..r1 of A x for a diagonal-stored system, with x staged in LDS:
+// chunks of C = cap - 2 halo rows; for each chunk x(j) = xv(j) is evaluated
+// once per row of the chunk plus its halo (coalesced, four rows in flight per
+// thread) into lds, own(j, x_j) is called for the chunk's own rows, then
+// f(i, x_i, (A x)_i) once per row with the row read from LDS.  Callers check
+// that 2 halo + kBlock <= cap.
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+
+template <int ND, class T, class XF, class OWN, class F>
+__device__ __forceinline__ void stencil_staged(const Stencil<T>& A, int r0, int r1, lds_ptr<T> lds, int cap,
+                                               XF&& xv, OWN&& own, F&& f) {
+    constexpr int U = 4;
+    const int tid = threadIdx.x;
+    const int C = cap - 2 * A.halo;
+    const int lo = A.row0, hi = A.row0 + A.n_s;
+    for (int c0 = r0; c0 < r1; c0 += C) {
+        const int c1 = min(r1, c0 + C);
+        const int base = c0 - A.halo;
+        const int s0 = max(base, lo), s1 = min(c1 + A.halo, hi);
+        __syncthreads();  // the previous chunk's readers are done with lds
+        for (int j0 = s0 + tid; j0 < s1; j0 += U * kBlock) {
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = xv(min(j0 + u * kBlock, s1 - 1));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + u * kBlock;
+                if (j < s1) {
+                    lds[j - base] = v[u];
+                    if (j >= c0 && j < c1) own(j, v[u]);
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = c0 + tid; i < c1; i += kBlock)
+            f(i, lds[i - base], A.template row<ND>(i, [&](int j) { return lds[j - base]; }));
+    }
+}
+
+// ---- 16-byte groups: G = 16 / sizeof(T) consecutive rows per lane ---------------------------
+// The iteration kernels of five-point stencil systems move every vector in
+// 16-byte loads and stores (four FP32 or two FP64 rows per lane), stage the
+// x values a row's stencil needs in LDS once per chunk, and read the presence
+// bytes of a group as one word.  Work vectors and the presence bytes are
+// allocated with kPad rows of slack, so a group that runs past the last row
+// stays inside the allocation; rows outside [c0, c1) are neither written nor
+// reduced.
+template <class T>
+struct alignas(16) Pack {
+    static constexpr int G = 16 / (int)sizeof(T);
+    T e[G];
+};
+constexpr int kPad = 16;
+
+template <class T>
+__device__ __forceinline__ Pack<T> ld16(const T* p) { return *reinterpret_cast<const Pack<T>*>(p); }
+template <class T>
+__device__ __forceinline__ void st16(T* p, const Pack<T>& v) { *reinterpret_cast<Pack<T>*>(p) = v; }
+template <class T>
+__device__ __forceinline__ unsigned mask_word(const unsigned char* m, int g) {
+    if constexpr (Pack<T>::G == 4) return *reinterpret_cast<const unsigned*>(m + g);
+    else return *reinterpret_cast<const unsigned short*>(m + g);
+}
+
+// Row i of a five-point stencil system from x staged in LDS (lds[j - base]
+// holds x_j), presence bits mb: the same sum, in the same order and rounding,
+// as Stencil::row<5>.
+template <class T>
+__device__ __forceinline__ T row_lds(const Stencil<T>& A, int i, unsigned mb, lds_ptr<T> lds, int base) {
+#pragma clang fp contract(off)
+    const int last = A.row0 + A.n_s - 1;
+    T sum = 0;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        const T t = A.uv[d] * lds[min(max(i + A.off[d], A.row0), last) - base];
+        sum = ((mb >> d) & 1u) ? sum + t : sum;
+    }
+    return sum;
+}
+
+// Chunk geometry of the vectorised stencil paths: chunks of C rows (a
+// multiple of G) whose staged window [g0, g1) (groups aligned to G, rows
+// inside the system) holds every column the chunk's rows touch.
+template <class T>
+struct VecWin {
+    static constexpr int G = Pack<T>::G;
+    int H, C;
+    __device__ __forceinline__ VecWin(const Stencil<T>& A, int cap) {
+        H = (A.halo + G - 1) / G * G;
+        C = (cap - 2 * H - 2 * G) / G * G;
+    }
+    // usable when a chunk holds at least 1024 rows
+    __device__ __forceinline__ bool ok() const { return C >= 1024; }
+};
+
+// K1 rows [r0, r1) of a five-point stencil system: p_new = dv r + beta p_old
+// by groups into LDS (and into pn for the chunk's own rows), then
+// q = A p_new from LDS and pq += p_new . q.  q is not stored: K2 recomputes it.
+template <class T>
+__device__ __forceinline__ void k1_stencil_vec(const Stencil<T>& A, int r0, int r1, lds_ptr<T> lds, int cap,
+                                               const T* __restrict__ r, const T* __restrict__ po,
+                                               T* __restrict__ pn, T dv, double beta, double& pq) {
+    constexpr int G = Pack<T>::G;
+    const VecWin<T> W(A, cap);
+    const int tid = threadIdx.x;
+    const int lo = A.row0, hi = A.row0 + A.n_s;
+    for (int c0 = r0; c0 < r1; c0 += W.C) {
+        const int c1 = min(r1, c0 + W.C);
+        const int a0 = c0 & ~(G - 1);
+        const int base = a0 - W.H;
+        const int g0 = max(base, lo & ~(G - 1)), g1 = min(c1 + W.H, hi);
+        __syncthreads();  // the previous chunk's readers are done with lds
+        for (int g = g0 + tid * G; g < g1; g += kBlock * G) {
+            const Pack<T> rv = ld16(r + g), pv = ld16(po + g);
+            Pack<T> v;
+#pragma unroll
+            for (int e = 0; e < G; ++e) v.e[e] = (T)((double)(T)(dv * rv.e[e]) + beta * (double)pv.e[e]);
+#pragma unroll
+            for (int e = 0; e < G; ++e) lds[g - base + e] = v.e[e];
+            if (g >= c0 && g + G <= c1) {
+                st16(pn + g, v);
+            } else {
+#pragma unroll
+                for (int e = 0; e < G; ++e)
+                    if (g + e >= c0 && g + e < c1) pn[g + e] = v.e[e];
+            }
+        }
+        __syncthreads();
+        for (int g = a0 + tid * G; g < c1; g += kBlock * G) {
+            const unsigned mw = mask_word<T>(A.mask, g);
+#pragma unroll
+            for (int e = 0; e < G; ++e) {
+                const int i = g + e;
+                if (i >= c0 && i < c1) {
+                    const T sum = row_lds(A, i, (mw >> (8 * e)) & 0xffu, lds, base);
+                    pq += (double)lds[i - base] * (double)sum;
+                }
+            }
+        }
+    }
+}
+
+// K2 rows [r0, r1) of a five-point stencil system: q = A p recomputed from p
+// staged in LDS, then x += alpha p (x_lo in mixed mode), r -= alpha q, with
+// r.r and r.z (z = dv r) partials; own-row vectors by 16-byte groups.
+template <class T, bool MIXED>
+__device__ __forceinline__ void k2_stencil_vec(const Stencil<T>& A, int r0, int r1, lds_ptr<T> lds, int cap,
+                                               const T* __restrict__ p, T* __restrict__ r, double* __restrict__ x64,
+                                               float* __restrict__ xlo, T dv, double alpha, double (&red)[2]) {
+    constexpr int G = Pack<T>::G;
+    using X = std::conditional_t<MIXED, float, double>;  // the x part K2 updates
+    static_assert(Pack<X>::G == G, "x and the Krylov vectors group alike");
+    const VecWin<T> W(A, cap);
+    const int tid = threadIdx.x;
+    const int lo = A.row0, hi = A.row0 + A.n_s;
+    X* __restrict__ xs = MIXED ? (X*)xlo : (X*)x64;
+    for (int c0 = r0; c0 < r1; c0 += W.C) {
+        const int c1 = min(r1, c0 + W.C);
+        const int a0 = c0 & ~(G - 1);
+        const int base = a0 - W.H;
+        const int g0 = max(base, lo & ~(G - 1)), g1 = min(c1 + W.H, hi);
+        __syncthreads();
+        for (int g = g0 + tid * G; g < g1; g += kBlock * G) {
+            const Pack<T> v = ld16(p + g);
+#pragma unroll
+            for (int e = 0; e < G; ++e) lds[g - base + e] = v.e[e];
+        }
+        __syncthreads();
+        for (int g = a0 + tid * G; g < c1; g += kBlock * G) {
+            const bool full = g >= c0 && g + G <= c1;
+            const unsigned mw = mask_word<T>(A.mask, g);
+            const Pack<T> rv = ld16(r + g);
+            Pack<X> xv;
+            if (MIXED || full) {  // x_lo is padded; the caller's x64 only read in full groups
+                xv = ld16(xs + g);
+            } else {
+#pragma unroll
+                for (int e = 0; e < G; ++e) xv.e[e] = (g + e >= c0 && g + e < c1) ? xs[g + e] : X(0);
+            }
+            Pack<T> ro;
+            Pack<X> xo;
+#pragma unroll
+            for (int e = 0; e < G; ++e) {
+                const int i = g + e;
+                const T pi = lds[i - base];
+                const T qi = row_lds(A, i, (mw >> (8 * e)) & 0xffu, lds, base);
+                xo.e[e] = (X)((double)xv.e[e] + alpha * (double)pi);
+                const T ri = (T)((double)rv.e[e] - alpha * (double)qi);
+                ro.e[e] = ri;
+                if (i >= c0 && i < c1) {
+                    const double rd = (double)ri;
+                    red[0] += rd * rd;
+                    red[1] += rd * (double)(T)(dv * ri);
+                }
+            }
+            if (full) {
+                st16(r + g, ro);
+                st16(xs + g, xo);
+            } else {
+#pragma unroll
+                for (int e = 0; e < G; ++e)
+                    if (g + e >= c0 && g + e < c1) {
+                        r[g + e] = ro.e[e];
+                        xs[g + e] = xo.e[e];
+                    }
+            }
+        }
+    }
+}
+
+// Row sums of a tile (rows r0..r1) of A*x with x(j) = xv(j), LDS-staged;
+// f(row, sum) once per row from one lane.  Same staging rule as the
+// single-system SpMV.
+template <class Acc, class V, class XF, class P, class F>
 __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict__ rp, const int* __restrict__ ci,
-                                          const V* __restrict__ val, const X* __restrict__ x, Acc* prod, F&& f) {
+                                          const V* __restrict__ val, XF&& xv, P prod, F&& f) {
     const int tid = threadIdx.x;
     const int k0 = rp[r0], k1 = rp[r1];
     if (k1 - k0 > kLdsNnz) {  // one long row
         Acc s = 0;
-        for (int k = k0 + tid; k < k1; k += kBlock) s += (Acc)val[k] * (Acc)x[ci[k]];
+        for (int k = k0 + tid; k < k1; k += kBlock) s += (Acc)val[k] * (Acc)xv(ci[k]);
         double v1[1] = {(double)s};
         __shared__ double lred[kBlock / 64];
         block_sum<1>(v1, lred);
         if (tid == 0) f(r0, (Acc)v1[0]);
         return;
     }
-    for (int k = k0 + tid; k < k1; k += kBlock) prod[k - k0] = (Acc)val[k] * (Acc)x[ci[k]];
+    for (int k = k0 + tid; k < k1; k += kBlock) prod[k - k0] = (Acc)val[k] * (Acc)xv(ci[k]);
     __syncthreads();
     const int nrows = r1 - r0;
     int tpr = 1;
@@ -108,6 +444,21 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict_
         for (int k = a + l; k < b; k += tpr) s += prod[k];
         for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (l == 0) f(row, s);
+    }
+}
+
+// FP64 rows r0..r1 of A*x for system s: diagonal-stored systems row by row
+// (any tile size), CSR systems through the LDS-staged tile_spmv (tiles within
+// kLdsNnz nonzeros).  f(row, sum) once per row.
+template <class XF, class P, class F>
+__device__ __forceinline__ void tile_rows64(const BArgs& B, int s, int r0, int r1, XF&& xv, P prod, F&& f) {
+    if (B.dia_kind[s] != SK_CSR) {
+        const Stencil<double> A(B, s);
+        A.dispatch([&](auto nd) {
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) f(i, A.template row<nd()>(i, xv));
+        });
+    } else {
+        tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, xv, prod, f);
     }
 }
 
@@ -148,7 +499,7 @@ k_b_init1(BArgs B, int t0, const double* __restrict__ b, const double* __restric
     const int tile = t0 + blockIdx.x;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
     double red[2] = {0.0, 0.0};
-    tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, x, prod, [&](int i, double sum) {
+    tile_rows64(B, B.tile_sys[tile], r0, r1, [&](int j) { return x[j]; }, prod, [&](int i, double sum) {
         const double ri = b[i] - sum;
         r64[i] = ri;
         red[0] += ri * ri;
@@ -179,7 +530,7 @@ struct OpInit1 {
     }
 };
 
-// r = (T) r64, x_lo = 0, p = D^-1 r, r.z partials
+// r = (T) r64, x_lo = 0, p = D^-1 r (first p buffer), r.z partials
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
 k_b_init2(BArgs B, int t0, const double* __restrict__ r64, T* __restrict__ r, T* __restrict__ p,
@@ -206,35 +557,82 @@ struct OpInit2 {
     __device__ void operator()(BState& S, const double* tot, const BArgs&) const {
         S.rho = tot[0];
         S.rho_old = tot[0];
+        S.beta = 0.0;  // the first K1 forms p = z + 0 * z
+        S.pcur = 0;
     }
 };
 
 // ---- K1 / F1 -------------------------------------------------------------------------------
+// NORMAL / FLUSHED: p_new = D^-1 r + beta p_old into the other p buffer, and
+// q = A p_new with p.q partials.  The p update of cg_solver.cpp:173-176 is
+// evaluated on the fly for every column the tile's rows touch (the same
+// expression on the same inputs as the row's owner, so bit-identical), which
+// removes a pass over r and p per iteration.  Stencil-stored systems do not
+// store q: K2 recomputes it from p_new.
+// FLUSH: FP64 true residual r = b - A (x64 + x_lo).
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
-k_b_k1(BArgs B, int t0, const T* __restrict__ p, T* __restrict__ q, T* __restrict__ r, const double* __restrict__ b,
-       const double* __restrict__ x64) {
-    __shared__ double prod_d[kLdsNnz];
+k_b_k1(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
+       const double* __restrict__ b, const double* __restrict__ x64, const float* __restrict__ xlo) {
+    __shared__ __attribute__((aligned(16))) double prod_d[kLdsNnz];
+    // every access to prod_d goes through address-space-3 pointers
+    const lds_ptr<double> prod64 = (lds_ptr<double>)prod_d;
     const int tile = t0 + blockIdx.x;
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
     const int flag = S0->flag;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
+    const bool cdok = B.cd_ok[s] != 0;
+    const T cdv = Prec<T>::cd(B)[s];
+    const T* __restrict__ dinv = Prec<T>::dinv(B);
     double red[2] = {0.0, 0.0};
-    if (flag != BS_FLUSH) {
-        tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), p, reinterpret_cast<T*>(prod_d), [&](int i, T sum) {
-            q[i] = sum;
-            red[0] += (double)p[i] * (double)sum;
-        });
-    } else {  // x was flushed by the previous K3: FP64 true residual replaces r
-        const T* dinv = Prec<T>::dinv(B);
-        tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, x64, prod_d, [&](int i, double sum) {
+    if (flag == BS_FLUSH) {
+        auto xv = [&](int j) { return MIXED ? x64[j] + (double)xlo[j] : x64[j]; };
+        tile_rows64(B, s, r0, r1, xv, prod64, [&](int i, double sum) {
             const double rh = b[i] - sum;
             const T ri = (T)rh;
             r[i] = ri;
             red[0] += rh * rh;
-            red[1] += (double)ri * (double)(T)(dinv[i] * ri);
+            red[1] += (double)ri * (double)(T)((cdok ? cdv : dinv[i]) * ri);
+        });
+        tile_partial<2>(red, B, tile);
+        return;
+    }
+    const double beta = S0->beta;
+    const T* __restrict__ po = S0->pcur ? p1 : p0;
+    T* __restrict__ pn = S0->pcur ? p0 : p1;
+    auto pnew = [&](int j) -> T {
+        const T dj = cdok ? cdv : dinv[j];
+        return (T)((double)(T)(dj * r[j]) + beta * (double)po[j]);
+    };
+    const int kind = B.dia_kind[s];
+    if (kind != SK_CSR) {
+        const Stencil<T> A(B, s);
+        const lds_ptr<T> lds = (lds_ptr<T>)prod64;
+        constexpr int cap = kLdsNnz * (int)(sizeof(double) / sizeof(T));
+        auto out = [&](int i, T pi, T sum) {
+            if (kind == SK_DIA) q[i] = sum;
+            red[0] += (double)pi * (double)sum;
+        };
+        if (kind == SK_STENCIL && A.nd <= 5 && cdok && VecWin<T>(A, cap).ok()) {  // 16-byte groups
+            k1_stencil_vec(A, r0, r1, lds, cap, r, po, pn, cdv, beta, red[0]);
+        } else if (2 * A.halo + kBlock <= cap) {  // p_new staged in LDS, evaluated once per row
+            A.dispatch([&](auto nd) {
+                stencil_staged<nd()>(A, r0, r1, lds, cap, pnew, [&](int j, T v) { pn[j] = v; }, out);
+            });
+        } else {  // halo wider than the staging buffer: rows straight from memory
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+                const T pi = pnew(i);
+                pn[i] = pi;
+                out(i, pi, A.template row<0>(i, pnew));
+            }
+        }
+    } else {
+        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) pn[i] = pnew(i);
+        tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), pnew, (lds_ptr<T>)prod64, [&](int i, T sum) {
+            q[i] = sum;
+            red[0] += (double)pnew(i) * (double)sum;
         });
     }
     tile_partial<2>(red, B, tile);
@@ -254,6 +652,7 @@ struct OpF1 {
             } else {
                 S.alpha = S.rho / pq;
                 S.flag = BS_NORMAL;
+                S.pcur ^= 1;  // K1 wrote p_new into the other buffer
             }
             return;
         }
@@ -277,32 +676,63 @@ struct OpF1 {
             S.rho = tot[1];
             S.beta = S.rho / S.rho_old;
         }
-        S.flag = BS_PUPDATE;
+        S.flag = BS_FLUSHED;
         S.reason = BR_NONE;
     }
 };
 
 // ---- K2 / F2 -------------------------------------------------------------------------------
+// NORMAL: x += alpha p, r -= alpha q (q recomputed from p for stencil-stored
+// systems), r.r / r.z partials.  FLUSHED: x64 += x_lo, x_lo = 0.
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
-k_b_k2(BArgs B, int t0, const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ r,
-       double* __restrict__ x64, float* __restrict__ xlo) {
+k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, const T* __restrict__ q,
+       T* __restrict__ r, double* __restrict__ x64, float* __restrict__ xlo) {
+    __shared__ __attribute__((aligned(16))) double stage_d[kLdsNnz];
     const int tile = t0 + blockIdx.x;
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
-    if (S0->halt || S0->flag != BS_NORMAL) return;
-    const double alpha = S0->alpha;
+    if (S0->halt) return;
+    const int flag = S0->flag;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
-    const T* dinv = Prec<T>::dinv(B);
+    if (flag == BS_FLUSHED) {
+        if (MIXED)
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+                x64[i] += (double)xlo[i];
+                xlo[i] = 0.0f;
+            }
+        return;
+    }
+    if (flag != BS_NORMAL) return;
+    const double alpha = S0->alpha;
+    const T* __restrict__ p = S0->pcur ? p1 : p0;
+    const bool cdok = B.cd_ok[s] != 0;
+    const T cdv = Prec<T>::cd(B)[s];
+    const T* __restrict__ dinv = Prec<T>::dinv(B);
     double red[2] = {0.0, 0.0};
-    for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
-        if (MIXED) xlo[i] = (float)((double)xlo[i] + alpha * (double)p[i]);
-        else x64[i] = x64[i] + alpha * (double)p[i];
-        const T ri = (T)((double)r[i] - alpha * (double)q[i]);
+    auto body = [&](int i, T pi, T qi) {
+        if (MIXED) xlo[i] = (float)((double)xlo[i] + alpha * (double)pi);
+        else x64[i] = x64[i] + alpha * (double)pi;
+        const T ri = (T)((double)r[i] - alpha * (double)qi);
         r[i] = ri;
         const double rd = (double)ri;
         red[0] += rd * rd;
-        red[1] += rd * (double)(T)(dinv[i] * ri);
+        red[1] += rd * (double)(T)((cdok ? cdv : dinv[i]) * ri);
+    };
+    if (B.dia_kind[s] == SK_STENCIL) {
+        const Stencil<T> A(B, s);
+        const lds_ptr<T> lds = (lds_ptr<T>)(lds_ptr<double>)stage_d;
+        constexpr int cap = kLdsNnz * (int)(sizeof(double) / sizeof(T));
+        auto pv = [&](int j) { return p[j]; };
+        if (B.dia_kind[s] == SK_STENCIL && A.nd <= 5 && cdok && VecWin<T>(A, cap).ok()) {  // 16-byte groups
+            k2_stencil_vec<T, MIXED>(A, r0, r1, lds, cap, p, r, x64, xlo, cdv, alpha, red);
+        } else if (2 * A.halo + kBlock <= cap) {
+            A.dispatch([&](auto nd) { stencil_staged<nd()>(A, r0, r1, lds, cap, pv, [](int, T) {}, body); });
+        } else {
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], A.template row<0>(i, pv));
+        }
+    } else {
+        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], q[i]);
     }
     tile_partial<2>(red, B, tile);
 }
@@ -345,31 +775,6 @@ struct OpF2 {
     }
 };
 
-// ---- K3 ------------------------------------------------------------------------------------
-template <class T, bool MIXED>
-__global__ void __launch_bounds__(kBlock)
-k_b_k3(BArgs B, int t0, T* __restrict__ p, const T* __restrict__ r, double* __restrict__ x64,
-       float* __restrict__ xlo) {
-    const int tile = t0 + blockIdx.x;
-    const int s = B.tile_sys[tile];
-    const BState* S0 = &B.st[s];
-    if (S0->halt) return;
-    const int flag = S0->flag;
-    const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
-    if (flag == BS_FLUSH) {
-        if (MIXED)
-            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
-                x64[i] += (double)xlo[i];
-                xlo[i] = 0.0f;
-            }
-        return;
-    }
-    const double beta = S0->beta;
-    const T* dinv = Prec<T>::dinv(B);
-    for (int i = r0 + threadIdx.x; i < r1; i += kBlock)
-        p[i] = (T)((double)(T)(dinv[i] * r[i]) + beta * (double)p[i]);
-}
-
 // ---- final: x64 += x_lo and the FP64 true residual per (large) system ----------------------
 __global__ void k_b_flush_range(int a, int e, double* __restrict__ x64, float* __restrict__ xlo) {
     for (int i = a + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
@@ -384,7 +789,7 @@ k_b_final(BArgs B, int t0, const double* __restrict__ b, const double* __restric
     const int tile = t0 + blockIdx.x;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
     double red[1] = {0.0};
-    tile_spmv<double>(r0, r1, B.rp, B.ci, B.v64, x, prod, [&](int i, double sum) {
+    tile_rows64(B, B.tile_sys[tile], r0, r1, [&](int j) { return x[j]; }, prod, [&](int i, double sum) {
         const double ri = b[i] - sum;
         red[0] += ri * ri;
     });
@@ -434,6 +839,8 @@ k_b_small(BArgs B, int k0, const double* __restrict__ b, double* __restrict__ x6
     const int* __restrict__ ci = B.ci;
     const T* __restrict__ val = Prec<T>::val(B);
     const T* __restrict__ dinv = Prec<T>::dinv(B);
+    const bool is_dia = B.dia_kind[s] != SK_CSR;
+    const Stencil<T> A(B, s);
     // true residual into r (rounded), returns (|r64|^2, r.z)
     auto true_residual = [&](double (&o)[2]) {
         o[0] = o[1] = 0.0;
@@ -472,11 +879,21 @@ k_b_small(BArgs B, int k0, const double* __restrict__ b, double* __restrict__ x6
         while (S.iter < B.max_iter) {
             // q = A p ; p.q
             double v1[1] = {0.0};
-            for (int i = a + tid; i < e; i += kSmallThreads) {
-                T sum = 0;
-                for (int k = rp[i]; k < rp[i + 1]; ++k) sum += val[k] * p[ci[k]];
-                q[i] = sum;
-                v1[0] += (double)p[i] * (double)sum;
+            if (is_dia) {
+                A.dispatch([&](auto nd) {
+                    for (int i = a + tid; i < e; i += kSmallThreads) {
+                        const T sum = A.template row<nd()>(i, [&](int j) { return p[j]; });
+                        q[i] = sum;
+                        v1[0] += (double)p[i] * (double)sum;
+                    }
+                });
+            } else {
+                for (int i = a + tid; i < e; i += kSmallThreads) {
+                    T sum = 0;
+                    for (int k = rp[i]; k < rp[i + 1]; ++k) sum += val[k] * p[ci[k]];
+                    q[i] = sum;
+                    v1[0] += (double)p[i] * (double)sum;
+                }
             }
             wg_sum<1>(v1, red);
             int reason = BR_NONE;
@@ -584,14 +1001,18 @@ struct Batch {
     std::vector<int> h_rp;       // global row_ptr, N+1
     std::vector<int> order;      // internal position -> user system id (by nnz, descending)
     DevBuf rp, ci, v64, v32, dinv64, dinv32, sys_row0, sys_row1;
-    // tiled schedule (depends on tile_rows)
-    int tile_rows = 0;
+    DevBuf dia_kind, dia_nd, dia_off, dia_base, dia64, dia32, uv64, uv32, umask, cd_ok, cd64, cd32;
+    int n_kind[3] = {0, 0, 0};  // systems per SK_* storage
+    std::vector<int> h_kind;    // SK_* per internal system
+    // tiled schedule: rows per tile for CSR systems (0 = not tuned yet; also
+    // capped at kLdsNnz nonzeros) and for diagonal-stored systems
+    int tile_rows = 0, tile_rows_dia = 0;
     int T = 0;
     DevBuf tile_r0, tile_r1, tile_sys, sys_t0, sys_t1;
     std::vector<int> h_sys_t0, h_sys_t1;
     std::vector<std::pair<int, int>> waves;  // internal system ranges within [0, n_large)
     // per-solve work (reused)
-    DevBuf st, partials, r64, r, p, q, xlo;
+    DevBuf st, partials, r64, r, p, p1, q, xlo;
     int work_prec = -1;
     struct G { hipGraphExec_t exec = nullptr; int wave = -1, prec = -1, chunk = 0; };
     std::vector<G> graphs;
@@ -608,7 +1029,7 @@ struct Batch {
     }
 };
 
-static void build_tiles(Batch& B, int tile_rows) {
+static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     std::vector<int> r0, r1, sys;
     B.h_sys_t0.assign(B.S, 0);
     B.h_sys_t1.assign(B.S, 0);
@@ -620,7 +1041,9 @@ static void build_tiles(Batch& B, int tile_rows) {
         while (i < e) {
             int j = i;
             const int base = B.h_rp[i];
-            if (B.h_rp[i + 1] - base > kLdsNnz) {
+            if (B.h_kind[k] != SK_CSR) {
+                j = std::min(e, i + tile_rows_dia);
+            } else if (B.h_rp[i + 1] - base > kLdsNnz) {
                 j = i + 1;
             } else {
                 while (j < e && j - i < tile_rows && B.h_rp[j + 1] - base <= kLdsNnz) ++j;
@@ -644,6 +1067,7 @@ static void build_tiles(Batch& B, int tile_rows) {
     up(B.sys_t1, B.h_sys_t1);
     B.partials = DevBuf(B.ctx, sizeof(double) * 2 * (size_t)std::max(B.T, 1));
     B.tile_rows = tile_rows;
+    B.tile_rows_dia = tile_rows_dia;
     B.drop_graphs();
 }
 
@@ -668,6 +1092,117 @@ static void build_waves(Batch& B, size_t budget) {
     if (start < B.n_large) B.waves.push_back({start, B.n_large});
 }
 
+// Storage selection per system.  A system whose nonzeros lie on <= kMaxDiags
+// diagonals is diagonal-stored: SK_STENCIL when every diagonal carries a
+// single value and no row repeats a column (constant-coefficient stencils:
+// one value per diagonal plus a presence bit per row), else SK_DIA when the
+// padding stays under 50% (diagonal-major values, zero where absent); all
+// others stay CSR.  The iteration kernels of diagonal-stored systems read no
+// column indices or row pointers.  Independently, a system whose Jacobi
+// diagonal D^-1 is one value everywhere gets it as a per-system scalar.
+static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci, const double* v,
+                      const std::vector<double>& dinv) {
+    const int S = B.S;
+    std::vector<int> kind(S, SK_CSR), nd(S, 0), offs((size_t)S * kMaxDiags, 0), cdok(S, 0);
+    std::vector<long long> base(S, 0);
+    std::vector<double> uv((size_t)S * kMaxDiags, 0.0), cd(S, 0.0);
+    std::vector<unsigned char> mask((size_t)B.N + kPad, 0);
+    // HSPS_BATCH_STORAGE=csr keeps every system in CSR; =dia allows DIA but not stencil
+    const char* cap_env = std::getenv("HSPS_BATCH_STORAGE");
+    const int cap = !cap_env ? SK_STENCIL : !std::strcmp(cap_env, "csr") ? SK_CSR
+                                          : !std::strcmp(cap_env, "dia") ? SK_DIA : SK_STENCIL;
+    long long total = 0;
+    for (int k = 0; k < S; ++k) {
+        const int s = B.order[k];
+        const int a = row_off[s], e = row_off[s + 1];
+        if (e > a) {
+            bool same = true;
+            for (int i = a + 1; i < e && same; ++i) same = dinv[i] == dinv[a];
+            if (same) {
+                cdok[k] = 1;
+                cd[k] = dinv[a];
+            }
+        }
+        const long long nz = rp[e] - rp[a];
+        std::vector<int> o;
+        bool ok = e > a;
+        for (int i = a; i < e && ok; ++i)
+            for (int q = rp[i]; q < rp[i + 1] && ok; ++q) {
+                const int d = ci[q] - i;
+                if (std::find(o.begin(), o.end(), d) == o.end()) {
+                    if ((int)o.size() == kMaxDiags) ok = false;
+                    else o.push_back(d);
+                }
+            }
+        if (!ok || cap == SK_CSR) continue;
+        std::sort(o.begin(), o.end());
+        const int m = (int)o.size();
+        nd[k] = m;
+        for (int d = 0; d < m; ++d) offs[(size_t)k * kMaxDiags + d] = o[d];
+        // one value per diagonal, no repeated column in a row?
+        std::vector<char> seen(m, 0);
+        bool uniform = true;
+        for (int i = a; i < e && uniform; ++i) {
+            unsigned bits = 0;
+            for (int q = rp[i]; q < rp[i + 1] && uniform; ++q) {
+                const int d = (int)(std::lower_bound(o.begin(), o.end(), ci[q] - i) - o.begin());
+                if (bits & (1u << d)) uniform = false;
+                bits |= 1u << d;
+                double& u = uv[(size_t)k * kMaxDiags + d];
+                if (!seen[d]) { seen[d] = 1; u = v[q]; }
+                else if (!(v[q] == u) || std::signbit(v[q]) != std::signbit(u)) uniform = false;
+            }
+            mask[i] = (unsigned char)bits;
+        }
+        if (uniform && cap == SK_STENCIL && cdok[k]) {  // (one diagonal value: D^-1 is a scalar)
+            kind[k] = SK_STENCIL;
+            continue;
+        }
+        for (int i = a; i < e; ++i) mask[i] = 0;
+        for (int d = 0; d < kMaxDiags; ++d) uv[(size_t)k * kMaxDiags + d] = 0.0;
+        if ((long long)m * (e - a) * 2 > nz * 3) {
+            nd[k] = 0;
+            continue;
+        }
+        kind[k] = SK_DIA;
+        base[k] = total;
+        total += (long long)m * (e - a);
+    }
+    std::vector<double> d64((size_t)std::max<long long>(total, 1), 0.0);
+    for (int k = 0; k < S; ++k) {
+        ++B.n_kind[kind[k]];
+        if (kind[k] != SK_DIA) continue;
+        const int s = B.order[k];
+        const int a = row_off[s], e = row_off[s + 1], n_s = e - a;
+        const int* o = &offs[(size_t)k * kMaxDiags];
+        for (int i = a; i < e; ++i)
+            for (int q = rp[i]; q < rp[i + 1]; ++q) {
+                const int d = (int)(std::lower_bound(o, o + nd[k], ci[q] - i) - o);
+                d64[(size_t)base[k] + (size_t)d * n_s + (i - a)] += v[q];
+            }
+    }
+    auto up = [&](DevBuf& dst, const void* src, size_t bytes) {
+        dst = DevBuf(B.ctx, std::max<size_t>(bytes, 8));
+        if (bytes) HSPS_HIP(hipMemcpy(dst.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    B.h_kind = kind;
+    up(B.dia_kind, kind.data(), kind.size() * sizeof(int));
+    up(B.dia_nd, nd.data(), nd.size() * sizeof(int));
+    up(B.dia_off, offs.data(), offs.size() * sizeof(int));
+    up(B.dia_base, base.data(), base.size() * sizeof(long long));
+    up(B.dia64, d64.data(), (size_t)total * sizeof(double));
+    std::vector<float> d32(d64.begin(), d64.begin() + total);
+    up(B.dia32, d32.data(), (size_t)total * sizeof(float));
+    up(B.uv64, uv.data(), uv.size() * sizeof(double));
+    std::vector<float> u32(uv.begin(), uv.end());
+    up(B.uv32, u32.data(), u32.size() * sizeof(float));
+    up(B.umask, mask.data(), mask.size());
+    up(B.cd_ok, cdok.data(), cdok.size() * sizeof(int));
+    up(B.cd64, cd.data(), cd.size() * sizeof(double));
+    std::vector<float> c32(cd.begin(), cd.end());
+    up(B.cd32, c32.data(), c32.size() * sizeof(float));
+}
+
 static BArgs args_of(const Batch& B, const hsps_params& p) {
     BArgs a;
     a.rp = B.rp.as<int>();
@@ -683,6 +1218,18 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     a.sys_t1 = B.sys_t1.as<int>();
     a.sys_row0 = B.sys_row0.as<int>();
     a.sys_row1 = B.sys_row1.as<int>();
+    a.dia_kind = B.dia_kind.as<int>();
+    a.dia_nd = B.dia_nd.as<int>();
+    a.dia_off = B.dia_off.as<int>();
+    a.dia_base = B.dia_base.as<long long>();
+    a.dia64 = B.dia64.as<double>();
+    a.dia32 = B.dia32.as<float>();
+    a.uv64 = B.uv64.as<double>();
+    a.uv32 = B.uv32.as<float>();
+    a.umask = B.umask.as<unsigned char>();
+    a.cd_ok = B.cd_ok.as<int>();
+    a.cd64 = B.cd64.as<double>();
+    a.cd32 = B.cd32.as<float>();
     a.st = B.st.as<BState>();
     a.partials = B.partials.as<double>();
     a.T = B.T;
@@ -696,22 +1243,24 @@ template <class T, bool MIXED>
 static void enqueue_iters(const Batch& B, const BArgs& a, int k0, int k1, int t0, int nt, int iters, const double* b,
                           double* x, hipStream_t s) {
     T* r = B.r.as<T>();
-    T* p = B.p.as<T>();
+    T* p0 = B.p.as<T>();
+    T* p1 = B.p1.as<T>();
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
     const int ns = k1 - k0;
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, (const T*)p, q, r, b, (const double*)x);
+        hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, p0, p1, q, r, b, (const double*)x,
+                           (const float*)xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF1{});
-        hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, (const T*)p, (const T*)q, r, x, xl);
+        hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, (const T*)p0, (const T*)p1,
+                           (const T*)q, r, x, xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF2<MIXED>{});
-        hipLaunchKernelGGL((k_b_k3<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, p, (const T*)r, x, xl);
     }
     HSPS_CHECK_LAUNCH();
 }
 
 static void ensure_work(Batch& B, int prec) {
-    const size_t n = (size_t)std::max(B.N, 1);
+    const size_t n = (size_t)B.N + kPad;  // 16-byte groups may run past the last row
     if (!B.st.p) {
         B.st = DevBuf(B.ctx, sizeof(BState) * (size_t)std::max(B.S, 1));
         B.r64 = DevBuf(B.ctx, n * sizeof(double));
@@ -720,6 +1269,7 @@ static void ensure_work(Batch& B, int prec) {
         const size_t vb = prec == HSPS_MIXED ? 4 : 8;
         B.r = DevBuf(B.ctx, n * vb);
         B.p = DevBuf(B.ctx, n * vb);
+        B.p1 = DevBuf(B.ctx, n * vb);
         B.q = DevBuf(B.ctx, n * vb);
         B.xlo = DevBuf(B.ctx, prec == HSPS_MIXED ? n * sizeof(float) : 0);
         B.work_prec = prec;
@@ -862,33 +1412,48 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
     return host;
 }
 
-// Project autotuner: times a few chunks of the tiled waves for each tile size
-// on a scratch copy of x and keeps the fastest.
+// Tile-size autotuner: times a few chunks of the tiled waves for each
+// candidate on a scratch copy of x and keeps the fastest — first the tile of
+// diagonal-stored systems, then the CSR tile, each only when the batch has
+// large systems of that kind.
 static void autotune(Batch& B, const double* b, const double* x0, const hsps_params& p) {
+    bool has_csr = false, has_dia = false;
+    for (int k = 0; k < B.n_large; ++k) (B.h_kind[k] == SK_CSR ? has_csr : has_dia) = true;
+    int best_c = 512, best_d = 4096;
     if (B.n_large == 0) {
-        build_tiles(B, 256);
+        build_tiles(B, best_c, best_d);
         return;
     }
-    const int cands[] = {256, 512, 1024};
     hipStream_t s = B.ctx->stream;
     DevBuf xs(B.ctx, (size_t)std::max(B.N, 1) * sizeof(double));
     hsps_params q = p;
     q.check_every = 16;
     q.max_iter = 48;  // bounds the small-system kernel during tuning
-    int best = 256;
-    double best_t = 1e30;
-    for (int tr : cands) {
-        build_tiles(B, tr);
+    auto trial = [&](int c, int d) {
+        build_tiles(B, c, d);
         HSPS_HIP(hipMemcpyAsync(xs.p, x0, (size_t)B.N * sizeof(double), hipMemcpyDeviceToDevice, s));
         (void)run_batch(B, b, xs.as<double>(), q, false, 1);  // warm
         HSPS_HIP(hipMemcpyAsync(xs.p, x0, (size_t)B.N * sizeof(double), hipMemcpyDeviceToDevice, s));
         HSPS_HIP(hipStreamSynchronize(s));
         const auto t0 = std::chrono::steady_clock::now();
         (void)run_batch(B, b, xs.as<double>(), q, false, 2);
-        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (dt < best_t) { best_t = dt; best = tr; }
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    if (has_dia) {
+        double bt = 1e30;
+        for (int d : {1024, 2048, 4096, 8192}) {
+            const double t = trial(best_c, d);
+            if (t < bt) { bt = t; best_d = d; }
+        }
     }
-    build_tiles(B, best);
+    if (has_csr) {
+        double bt = 1e30;
+        for (int c : {256, 512, 1024}) {
+            const double t = trial(c, best_d);
+            if (t < bt) { bt = t; best_c = c; }
+        }
+    }
+    build_tiles(B, best_c, best_d);
 }
 
 }  // namespace hsps_dev
@@ -973,7 +1538,8 @@ int hsps_batch_create(hsps_ctx ctx, int S, const int* row_off, const int* rp, co
                 HSPS_HIP(hipMemcpy(B->sys_row0.p, r0.data(), sizeof(int) * S, hipMemcpyHostToDevice));
                 HSPS_HIP(hipMemcpy(B->sys_row1.p, r1.data(), sizeof(int) * S, hipMemcpyHostToDevice));
             }
-            build_tiles(*B, 256);
+            build_dia(*B, row_off, rp, ci, v, d64);
+            build_tiles(*B, 256, 1024);
             B->tile_rows = 0;  // not tuned yet
             // Wave working-set budget (MB).  Measured on MI355X (512 systems,
             // 1.9 TB of CG traffic): one wave beats MALL-sized (<=192 MB)
@@ -1014,7 +1580,8 @@ int hsps_batch_solve(hsps_batch B, const double* b, double* x, const hsps_params
         if (pp) p = *pp; else hsps_params_default(&p);
         B->ctx->set_current();
         if (p.batch_tile_rows > 0) {
-            if (p.batch_tile_rows != B->tile_rows) build_tiles(*B, std::min(p.batch_tile_rows, 4096));
+            const int tr = std::min(p.batch_tile_rows, 1 << 16);
+            if (tr != B->tile_rows || tr != B->tile_rows_dia) build_tiles(*B, tr, tr);
         } else if (B->tile_rows == 0) {
             ensure_work(*B, p.precision == HSPS_MIXED ? HSPS_MIXED : HSPS_FP64);
             autotune(*B, b, x, p);
@@ -1033,6 +1600,16 @@ int hsps_batch_solve(hsps_batch B, const double* b, double* x, const hsps_params
             n_conv += S.conv;
         }
         return n_conv == B->S ? 1 : 0;
+    });
+}
+
+int hsps_batch_storage(hsps_batch B, int* n_csr, int* n_dia, int* n_stencil) {
+    return guarded([&] {
+        if (!B) throw Error(HSPS_ERR_INVALID, "hsps_batch_storage: null batch");
+        if (n_csr) *n_csr = B->n_kind[SK_CSR];
+        if (n_dia) *n_dia = B->n_kind[SK_DIA];
+        if (n_stencil) *n_stencil = B->n_kind[SK_STENCIL];
+        return HSPS_OK;
     });
 }
 

@@ -91,6 +91,7 @@ SIGNATURES = {
     "hsps_batch_create": (_i, [_vp, _i, _ip, _ip, _ip, _dp, C.POINTER(_vp)]),
     "hsps_batch_destroy": (_i, [_vp]),
     "hsps_batch_solve": (_i, [_vp, _vp, _vp, _PP, _ip, _ip, _dp]),
+    "hsps_batch_storage": (_i, [_vp, _ip, _ip, _ip]),
 }
 
 
@@ -354,6 +355,12 @@ class Batch:
         rc = check(lib().hsps_batch_solve(self.h, b.ptr, x.ptr, C.byref(p), it.ctypes.data_as(_ip),
                                           cv.ctypes.data_as(_ip), tr.ctypes.data_as(_dp)))
         return bool(rc), it, cv, tr
+
+    def storage(self):
+        """Systems per storage kind chosen at creation: {"csr", "dia", "stencil"}."""
+        n = [C.c_int(), C.c_int(), C.c_int()]
+        check(lib().hsps_batch_storage(self.h, *[C.byref(c) for c in n]))
+        return {"csr": n[0].value, "dia": n[1].value, "stencil": n[2].value}
 
 
 def solve(A, b, x, solver="cg", M=None, params=None):

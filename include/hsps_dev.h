@@ -66,7 +66,8 @@ typedef struct hsps_params {
     double reliable_delta; /* mixed CG: recompute the FP64 true residual when
                               |r| drops below delta * max|r| since the last one (0.1) */
     int check_every;       /* host polls the device convergence flag every k iterations (32) */
-    int batch_tile_rows;   /* batched solves: rows per workgroup tile; 0 = autotune on first solve (0) */
+    int batch_tile_rows;   /* batched solves: rows per workgroup tile (CSR tiles also stop at 2048
+                              nonzeros); 0 = autotune on first solve (0) */
 } hsps_params;
 
 /* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64
@@ -182,6 +183,10 @@ int hsps_batch_destroy(hsps_batch B);
  * h_iters/h_converged/h_true_res: per-system results (may be NULL). */
 int hsps_batch_solve(hsps_batch B, const double* d_b, double* d_x, const hsps_params* p,
                      int* h_iters, int* h_converged, double* h_true_res);
+/* Storage chosen per system at creation: CSR, diagonal-major (nonzeros on
+ * <= 8 diagonals) or stencil (one value per diagonal + a presence bit per
+ * row).  HSPS_BATCH_STORAGE=csr|dia in the environment caps the choice. */
+int hsps_batch_storage(hsps_batch B, int* n_csr, int* n_dia, int* n_stencil);
 
 #ifdef __cplusplus
 }

@@ -96,3 +96,119 @@ def test_batch_rejects_bad_input(H, ctx):
         H.Batch(ctx, np.array([0, 16, 32], np.int32), np.concatenate([rp[:-1], rp + rp[-1]]).astype(np.int32),
                 np.concatenate([ci_bad, ci + 16]), np.concatenate([v, v]))
     assert e.value.code == -1
+
+
+def varcoef_2d(H, n, seed):
+    """5-point Poisson with a row-varying diagonal 4 + u_i, u_i in [0, 1):
+    nonzeros on 5 diagonals, values not constant along them (DIA storage)."""
+    rp, ci, v = H.poisson_2d(n)
+    v = v.copy()
+    u = np.abs(O.uniform(seed, n * n))
+    rows = np.repeat(np.arange(n * n), np.diff(rp))
+    v[ci == rows] += u
+    return rp, ci, v
+
+
+def graph_spd(n, seed, deg=3):
+    """Laplacian of a pseudo-random graph plus 0.05 I: columns scattered over
+    the whole system (CSR storage)."""
+    rng = np.random.default_rng(seed)
+    a = np.repeat(np.arange(n), deg)
+    b = rng.integers(0, n, n * deg)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    i = np.concatenate([a, b])
+    j = np.concatenate([b, a])
+    import scipy.sparse as sp
+    L = sp.coo_matrix((-np.ones(len(i)), (i, j)), shape=(n, n)).tocsr()
+    L.sum_duplicates()
+    d = -np.asarray(L.sum(axis=1)).ravel() + 0.05
+    L = (L + sp.diags(d)).tocsr()
+    L.sort_indices()
+    return L.indptr.astype(np.int32), L.indices.astype(np.int32), L.data.astype(np.float64)
+
+
+def csr_of(M):
+    import scipy.sparse as sp
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64)
+
+
+def poisson_3d(n):
+    """7-point Poisson on an n^3 grid: seven diagonals, offsets up to n^2."""
+    import scipy.sparse as sp
+    T = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    I = sp.identity(n)
+    return csr_of(sp.kron(sp.kron(T, I), I) + sp.kron(sp.kron(I, T), I) + sp.kron(sp.kron(I, I), T))
+
+
+def wide_stencil(n, w):
+    """Five diagonals at offsets 0, +-1, +-w (w wider than any LDS staging
+    window): 4 on the diagonal, -1 off."""
+    import scipy.sparse as sp
+    return csr_of(sp.diags([-np.ones(n - w), -np.ones(n - 1), 4 * np.ones(n), -np.ones(n - 1), -np.ones(n - w)],
+                           [-w, -1, 0, 1, w]))
+
+
+def tiled_systems(H):
+    """Systems above the persistent-kernel size (tiled path) and below it,
+    one of each storage kind, plus stencils with seven diagonals and with a
+    halo wider than the kernels' LDS staging window."""
+    arrs = [H.poisson_2d(160), varcoef_2d(H, 150, 5), graph_spd(24000, 1), H.anisotropic_2d(170, 0.02),
+            H.poisson_2d(30), graph_spd(500, 2), varcoef_2d(H, 40, 6), poisson_3d(30), wide_stencil(30000, 2000)]
+    kinds = ["stencil", "dia", "csr", "stencil", "stencil", "csr", "dia", "stencil", "stencil"]
+    oracles = [O.Csr.from_csr(len(rp) - 1, len(rp) - 1, rp, ci, v) for rp, ci, v in arrs]
+    return arrs, kinds, oracles
+
+
+@pytest.mark.parametrize("storage", ["auto", "dia", "csr"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_tiled_storage_kinds(H, ctx, monkeypatch, storage, precision):
+    """Tiled path on every storage kind, each capped by HSPS_BATCH_STORAGE,
+    against the oracle's single-system PCG+Jacobi."""
+    arrs, kinds, oracles = tiled_systems(H)
+    if storage != "auto":
+        monkeypatch.setenv("HSPS_BATCH_STORAGE", storage)
+    B = H.Batch.from_systems(ctx, arrs)
+    cap = {"auto": 2, "dia": 1, "csr": 0}[storage]
+    rank = {"csr": 0, "dia": 1, "stencil": 2}
+    want = {"csr": 0, "dia": 0, "stencil": 0}
+    for k in kinds:
+        want[["csr", "dia", "stencil"][min(rank[k], cap)]] += 1
+    assert B.storage() == want
+    bs = [A.spmv(O.uniform(300 + s, A.rows)) for s, A in enumerate(oracles)]
+    bd, xd = ctx.array(np.concatenate(bs)), ctx.array(n=B.n)
+    p = H.default_params(tol=1e-8, max_iter=5000, precision=precision)
+    ok, iters, conv, tr = B.solve(bd, xd, p)
+    x = xd.download()
+    assert ok and conv.all(), (iters, conv, tr)
+    for s, A in enumerate(oracles):
+        xs_dev = x[B.row_off[s]:B.row_off[s + 1]]
+        rel = np.linalg.norm(bs[s] - A.spmv(xs_dev)) / np.linalg.norm(bs[s])
+        assert rel < 1e-8 and abs(rel - tr[s]) <= 1e-3 * rel + 1e-16, (s, rel, tr[s])
+        xo, st = O.solve(A, bs[s], "cg", "jacobi", tol=1e-8, max_iter=5000)
+        assert np.linalg.norm(xs_dev - xo) <= 1e-6 * np.linalg.norm(xo), s
+        slack = 0.05 if precision == "fp64" else 0.10
+        assert abs(iters[s] - st.iterations) <= max(1, slack * st.iterations), (s, iters[s], st.iterations)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_tiled_alone_equals_in_mix(H, ctx, precision):
+    """Tiles never straddle systems and every per-system reduction runs over
+    the system's own tiles in a fixed order: a tiled system's iterates are
+    bit-identical alone and inside a batch."""
+    arrs, _, oracles = tiled_systems(H)
+    b_all = [A.spmv(O.uniform(11 + s, A.rows)) for s, A in enumerate(oracles)]
+    p = H.default_params(tol=1e-8, max_iter=5000, precision=precision, batch_tile_rows=512)
+    B = H.Batch.from_systems(ctx, arrs)
+    bd, xd = ctx.array(np.concatenate(b_all)), ctx.array(n=B.n)
+    ok, it_mix, _, _ = B.solve(bd, xd, p)
+    x_mix = xd.download()
+    assert ok
+    for s in [0, 1, 2]:
+        Bs = H.Batch.from_systems(ctx, [arrs[s]])
+        bs, xs = ctx.array(b_all[s]), ctx.array(n=len(b_all[s]))
+        ok1, it1, _, _ = Bs.solve(bs, xs, p)
+        assert ok1 and it1[0] == it_mix[s]
+        np.testing.assert_array_equal(xs.download(), x_mix[B.row_off[s]:B.row_off[s + 1]])
