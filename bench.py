@@ -12,7 +12,11 @@ inputs resident in HBM.
                  Krylov vectors, FP64 reductions, FP64 reliable-update
                  refinement), all 512 systems in one batch
   fp64_baseline  solves/s of the plain FP64 device PCG, one system at a time
-  roofline       the CSR SpMV kernel (FP64) on the 2048^2 Poisson matrix
+  roofline       the batched iteration at full load (K1 + K2 + the two
+                 per-system finish kernels over every tiled row): 34 B of
+                 algorithmic traffic per row (stencil storage, FP32), timed as
+                 (t(96 iterations) - t(32 iterations)) / 64
+  spmv_roofline  the CSR SpMV kernel (FP64) on the 2048^2 Poisson matrix
                  (n = 4,194,304; a working set above the 256 MB Infinity
                  Cache), timed with HIP events on the context's stream;
                  algorithmic bytes B = nnz*(8+4) + 4*(n+1) + 8*n + 8*n
@@ -38,6 +42,11 @@ HBM_PEAK_GBS = 8000.0
 # x 2.000 + WRITE_SIZE 32,768 KiB x 1.000; the factors make a 256 MiB copy
 # read and write 256 MiB).
 SPMV_FP64_HBM_BYTES = (154831 * 2.0 + 32768 * 1.0) * 1024
+# HBM bytes per row-iteration of the stencil-stored FP32 batched iteration
+# (k_b_k1 + k_b_k2), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --only-batched --tile-rows 2048` summed over every K1/K2 dispatch
+# of one solve, divided by its row-iterations (sum of rows x iterations).
+BATCHED_HBM_BYTES_PER_ROW_ITER = 39.0
 
 
 def log(*a):
@@ -144,7 +153,8 @@ def bench_batched(H, ctx, ev, args):
     xd = ctx.array(n=N)
 
     def batched(precision, steps, warmup):
-        p = H.default_params(tol=1e-8, max_iter=20000, precision=precision, check_every=32)
+        p = H.default_params(tol=1e-8, max_iter=20000, precision=precision, check_every=32,
+                             batch_tile_rows=args.tile_rows)
         for _ in range(warmup):
             H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
             B.solve(bd, xd, p)
@@ -165,6 +175,42 @@ def bench_batched(H, ctx, ev, args):
     log("[bench] batched mixed")
     t_mx, it_mx, tr_mx, err_mx = batched("mixed", args.steps, args.warmup)
     log(f"[bench] batched mixed {np.mean(t_mx):.3f}s")
+    storage = B.storage()
+    sizes = np.diff(row_off).astype(np.float64)
+    row_iters = float(np.dot(sizes, it_mx.astype(np.float64)))
+    # algorithmic bytes of one row-iteration of the stencil-stored FP32 path:
+    # K1 reads r, p_old, the presence byte and writes p_new (13 B); K2 reads
+    # p, the presence byte, r, x_lo and writes r, x_lo (21 B)
+    per_row = 34 if storage["stencil"] == len(systems) else None
+    # Full-load iteration: every tiled system (rows > 20,000; the smaller ones run in the persistent
+    # one-workgroup kernel on the second stream) is still iterating during the first 96 iterations,
+    # so (t(96) - t(32)) / 64 is one iteration (K1, F1, K2, F2) over all tiled rows.
+    def capped(iters):
+        p = H.default_params(tol=1e-8, max_iter=iters, precision="mixed", check_every=32,
+                             batch_tile_rows=args.tile_rows)
+        ts = []
+        for _ in range(3):
+            H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
+            ctx.sync()
+            t0 = time.perf_counter()
+            B.solve(bd, xd, p)
+            ctx.sync()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+    tiled_rows = float(sizes[sizes > 20000].sum())
+    t_it = (capped(96) - capped(32)) / 64
+    batched_roof = {"bytes_per_row_iteration": per_row, "storage": storage, "tiled_rows": tiled_rows,
+                    "full_load_iteration_us": 1e6 * t_it,
+                    "full_load_achieved_GBs": per_row * tiled_rows / t_it / 1e9 if per_row else None,
+                    "row_iterations": row_iters,
+                    "solve_average_GBs": per_row * row_iters / float(np.mean(t_mx)) / 1e9 if per_row else None}
+    if args.only_batched:
+        for A in mats:
+            A.close()
+        B.close()
+        return {"workload": "batched_only", "value": len(systems) / float(np.mean(t_mx)),
+                "ms_per_step": 1e3 * float(np.mean(t_mx)), "config": {}, "fp64_baseline": None,
+                "speedup_vs_fp64": None, "cpu_baseline": None, "batched_iteration": batched_roof}
     t_b64, it_b64, tr_b64, _ = batched("fp64", 1, 1)
     log(f"[bench] batched fp64 {np.mean(t_b64):.3f}s")
 
@@ -231,6 +277,7 @@ def bench_batched(H, ctx, ev, args):
                   "iteration_ratio_vs_fp64": float(it_mx.sum() / iters64.sum())},
         "speedup_vs_fp64": value / fp64_rate,
         "cpu_baseline": cpu,
+        "batched_iteration": batched_roof,
     }
     for A in mats:
         A.close()
@@ -306,6 +353,9 @@ def main():
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=0, help="batched tile rows (0: autotuned on the first solve)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="batched mixed solves only (no FP64 or CPU baselines); for rocprofv3 runs")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV roofline kernel (plus a 256 MiB device copy that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters); for rocprofv3 runs")
@@ -329,6 +379,32 @@ def main():
         return
     res = bench_batched(H, ctx, ev, args) if args.workload == "batched" else bench_single(H, ctx, ev, args)
     r64 = roof["fp64"]
+    spmv_roof = {"bound": "hbm", "kernel": "k_spmv<double,double,EpiStore> (CSR row-block SpMV, FP64), "
+                                           f"Poisson 2048^2 (n={rn}, nnz={rnnz})",
+                 "achieved": r64["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": r64["achieved_GBs"] / HBM_PEAK_GBS, "traffic": SPMV_FP64_HBM_BYTES,
+                 "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per dispatch, "
+                                   "calibrated on a 256 MiB copy (profiles/spmv_poisson2048_fp64_fp32.md)",
+                 "algorithmic_bytes": r64["algorithmic_bytes"], "avg_launch_us": r64["avg_launch_us"],
+                 "fp32_spmv": roof["fp32"]}
+    bi = res.get("batched_iteration")
+    if bi and bi.get("full_load_achieved_GBs"):
+        alg = bi["bytes_per_row_iteration"] * bi["tiled_rows"]
+        roofline = {"bound": "hbm",
+                    "kernel": "batched PCG iteration at full load: k_b_k1<float> + k_b_k2<float> + the two "
+                              f"per-system finish kernels over {bi['tiled_rows']:.0f} stencil-stored rows",
+                    "achieved": bi["full_load_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": bi["full_load_achieved_GBs"] / HBM_PEAK_GBS,
+                    "traffic": BATCHED_HBM_BYTES_PER_ROW_ITER * bi["tiled_rows"],
+                    "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 summed over the k_b_k1 / "
+                                      "k_b_k2 dispatches of a batched solve, per row-iteration "
+                                      f"({BATCHED_HBM_BYTES_PER_ROW_ITER:.1f} B vs {bi['bytes_per_row_iteration']} B "
+                                      "algorithmic; profiles/batched512_pcg_jacobi_kernels.md)",
+                    "algorithmic_bytes": alg, "avg_launch_us": bi["full_load_iteration_us"],
+                    "per": "one iteration (one launch each of K1, F1, K2, F2)"}
+        res["spmv_roofline"] = spmv_roof
+    else:
+        roofline = spmv_roof
     line = {
         "metric": "solves/s @ rel-res<=1e-8 (batched mixed vs plain FP64), p50/p99 latency",
         "value": res["value"],
@@ -347,14 +423,7 @@ def main():
         "speedup_vs_fp64": res["speedup_vs_fp64"],
         "detail": {k: v for k, v in res.items() if k not in ("value", "ms_per_step", "config", "fp64_baseline",
                                                              "speedup_vs_fp64", "cpu_baseline")},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<double,double,EpiStore> (CSR row-block SpMV, FP64), "
-                                             f"Poisson 2048^2 (n={rn}, nnz={rnnz})",
-                     "achieved": r64["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r64["achieved_GBs"] / HBM_PEAK_GBS, "traffic": SPMV_FP64_HBM_BYTES,
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per dispatch, "
-                                       "calibrated on a 256 MiB copy (profiles/spmv_poisson2048_fp64_fp32.md)",
-                     "algorithmic_bytes": r64["algorithmic_bytes"], "avg_launch_us": r64["avg_launch_us"],
-                     "fp32_spmv": roof["fp32"]},
+        "roofline": roofline,
         "cpu_baseline": res["cpu_baseline"],
     }
     print(json.dumps(line), flush=True)
