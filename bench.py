@@ -344,12 +344,98 @@ def bench_single(H, ctx, ev, args):
     }
 
 
+def serve_specs(count, nmax, seed=77):
+    """Request mix: n log-uniform in [1e3, nmax] (grid side rounded), one third
+    each Poisson, anisotropic (eps log-uniform in [1e-3, 1]) and
+    convection-diffusion with Pe = |beta|/eps log-uniform in [1, 1000] at a
+    uniform angle (|beta| = 1, eps = 1/Pe)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        side = max(32, int(round(np.sqrt(np.exp(rng.uniform(np.log(1e3), np.log(nmax)))))))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            out.append(("poisson", side, 0.0, 0.0, 0.0))
+        elif kind == 1:
+            out.append(("aniso", side, float(np.exp(rng.uniform(np.log(1e-3), 0.0))), 0.0, 0.0))
+        else:
+            pe = float(np.exp(rng.uniform(0.0, np.log(1000.0))))
+            th = float(rng.uniform(0.0, 2 * np.pi))
+            out.append(("convdiff", side, 1.0 / pe, float(np.cos(th)), float(np.sin(th))))
+    return out
+
+
+def bench_serve(H, ctx, args):
+    specs = serve_specs(args.requests, args.serve_nmax)
+    log(f"[bench] serve: building {len(specs)} requests")
+    reqs = []
+    for k, (kind, side, a, bx, by) in enumerate(specs):
+        if kind == "poisson":
+            rp, ci, v = H.poisson_2d(side)
+        elif kind == "aniso":
+            rp, ci, v = H.anisotropic_2d(side, a)
+        else:
+            rp, ci, v = H.convection_diffusion_2d(side, a, bx, by)
+        import scipy.sparse as sp
+        n = side * side
+        b = sp.csr_matrix((v, ci, rp), shape=(n, n)) @ H.uniform_pm1(9000 + k, n)
+        reqs.append((rp, ci, v, b))
+    # mixed precision; ILU(0) at MODERATE (ILUT's dense row scan is O(n^2))
+    p = H.default_params(tol=1e-8, max_iter=5000, precision="mixed", drop_tol=0.0)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(reqs)))
+    srv = H.Server(workers=args.workers, hbm_budget=args.hbm_budget, batch_max=64, batch_window_s=0.002)
+    # warm-up: one request of each kind through every code path
+    for k in range(min(3, len(reqs))):
+        srv.wait(srv.submit(*reqs[k], p))
+    log(f"[bench] serve: {len(reqs)} requests at {args.rate}/s, {args.workers} workers")
+    t0 = time.perf_counter()
+    tickets = []
+    for (rp, ci, v, b), ta in zip(reqs, arrivals):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append(srv.submit(rp, ci, v, b, p))
+    results = [srv.wait(t) for t in tickets]
+    makespan = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([r[2].latency_s for r in results])
+    states = [int(r[2].state) for r in results]
+    ok = [r[0] for r in results]
+    value = len(results) / makespan
+    return {
+        "workload": f"serve_{len(reqs)}_requests",
+        "value": value,
+        "ms_per_step": 1e3 * makespan,
+        "config": {"workload": f"serving {len(reqs)} requests, Poisson arrivals at {args.rate}/s, n log-uniform in "
+                               f"[1e3, {args.serve_nmax:.0f}], Poisson / anisotropic / convection-diffusion (Pe "
+                               "1..1000), adaptive ladder, mixed precision, FP64 rel-res<=1e-8",
+                   "requests": len(reqs), "rate_per_s": args.rate, "workers": args.workers,
+                   "total_rows": int(sum(len(r[0]) - 1 for r in reqs))},
+        "fp64_baseline": None,
+        "speedup_vs_fp64": None,
+        "cpu_baseline": None,
+        "serve": {"p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                  "mean_ms": 1e3 * float(lat.mean()), "converged": int(sum(ok)),
+                  "max_true_residual": float(max(r[2].true_residual for r in results)),
+                  "states": {H.STATE_NAMES[s]: states.count(s) for s in sorted(set(states))},
+                  "batched": int(sum(r[2].batched for r in results)),
+                  "escalated": int(sum(r[2].escalations > 0 for r in results)),
+                  "makespan_s": makespan},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "single"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "single", "serve"], default="batched")
+    ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--workers", type=int, default=4, help="serve: concurrent solves")
+    ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: admission budget (bytes)")
+    ap.add_argument("--serve-nmax", type=float, default=2.5e5, help="serve: largest request (rows)")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
@@ -377,7 +463,12 @@ def main():
         print(json.dumps({"spmv": roof, "n": rn, "nnz": rnnz, "calibration_copy_bytes": nbytes}), flush=True)
         ctx.close()
         return
-    res = bench_batched(H, ctx, ev, args) if args.workload == "batched" else bench_single(H, ctx, ev, args)
+    if args.workload == "batched":
+        res = bench_batched(H, ctx, ev, args)
+    elif args.workload == "serve":
+        res = bench_serve(H, ctx, args)
+    else:
+        res = bench_single(H, ctx, ev, args)
     r64 = roof["fp64"]
     spmv_roof = {"bound": "hbm", "kernel": "k_spmv<double,double,EpiStore> (CSR row-block SpMV, FP64), "
                                            f"Poisson 2048^2 (n={rn}, nnz={rnnz})",

@@ -1609,7 +1609,7 @@ int hsps_batch_solve(hsps_batch B, const double* b, double* x, const hsps_params
         // HSPS_BATCH_NO_GRAPH=1 launches the iteration chunks directly (for
         // tracing tools that do not follow graph launches).
         static const bool no_graph = std::getenv("HSPS_BATCH_NO_GRAPH") != nullptr;
-        std::vector<BState> st = run_batch(*B, b, x, p, !no_graph);
+        std::vector<BState> st = run_batch(*B, b, x, p, !no_graph && p.batch_graphs != 0);
         int n_conv = 0;
         for (int k = 0; k < B->S; ++k) {
             const int s = B->order[k];

@@ -93,20 +93,21 @@ __global__ void k_f64_to_f32(int n, const double* __restrict__ x, float* __restr
 
 void Csr::upload_values() {
     ctx->set_current();
-    if (nnz > 0) {
-        HSPS_HIP(hipMemcpy(v64.p, h_v.data(), (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
-        if (v32.p) {
-            hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0, nullptr, nnz, v64.as<double>(),
-                               v32.as<float>());
-            HSPS_CHECK_LAUNCH();
-        }
-    }
+    hipStream_t s = ctx->stream;  // stream-ordered: no device-wide sync (other contexts keep running)
     std::vector<double> d(rows, 0.0);
     for (int i = 0; i < rows; ++i)
         for (int k = h_rp[i]; k < h_rp[i + 1]; ++k)
             if (h_ci[k] == i) { d[i] = h_v[k]; break; }
-    if (rows > 0) HSPS_HIP(hipMemcpy(diag64.p, d.data(), (size_t)rows * sizeof(double), hipMemcpyHostToDevice));
-    HSPS_HIP(hipDeviceSynchronize());
+    if (nnz > 0) {
+        HSPS_HIP(hipMemcpyAsync(v64.p, h_v.data(), (size_t)nnz * sizeof(double), hipMemcpyHostToDevice, s));
+        if (v32.p) {
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, v64.as<double>(),
+                               v32.as<float>());
+            HSPS_CHECK_LAUNCH();
+        }
+    }
+    if (rows > 0) HSPS_HIP(hipMemcpyAsync(diag64.p, d.data(), (size_t)rows * sizeof(double), hipMemcpyHostToDevice, s));
+    HSPS_HIP(hipStreamSynchronize(s));
 }
 
 static void validate_csr(int rows, int cols, int nnz, const int* rp, const int* ci) {
@@ -290,6 +291,7 @@ void hsps_params_default(hsps_params* p) {
     p->reliable_delta = 0.1;
     p->check_every = 32;
     p->batch_tile_rows = 0;
+    p->batch_graphs = 1;
 }
 
 int hsps_ctx_create(int device, hsps_ctx* out) {

@@ -426,7 +426,7 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
                 case HSPS_PRECOND_AMG: setup_amg(*M, *A, p); break;
                 default: throw Error(HSPS_ERR_INVALID, "SolverFactory: unknown PrecondType");
             }
-            HSPS_HIP(hipDeviceSynchronize());
+            HSPS_HIP(hipStreamSynchronize(A->ctx->stream));
         } catch (...) {
             delete M;
             throw;

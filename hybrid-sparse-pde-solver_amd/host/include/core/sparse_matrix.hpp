@@ -82,6 +82,9 @@ public:
 
     /// Device handle (created on demand; shared by copies until modified).
     hsps_csr device() const;
+    /// Drops this object's reference to the device copy (the next device use
+    /// uploads again); no reference counterpart.
+    void release_device() const { dev_.reset(); }
 
 private:
     Index rows_ = 0;

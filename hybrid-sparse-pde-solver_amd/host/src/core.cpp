@@ -26,15 +26,26 @@ DeviceContext& DeviceContext::instance() {
     return c;
 }
 
+static thread_local hsps_ctx tl_ctx = nullptr;  // ThreadDevice binding of this thread
+
+hsps_ctx DeviceContext::current() { return tl_ctx ? tl_ctx : instance().ctx(); }
+
+ThreadDevice::ThreadDevice(hsps_ctx c) : prev_(tl_ctx) { tl_ctx = c; }
+ThreadDevice::~ThreadDevice() { tl_ctx = prev_; }
+
 DeviceBuffer::DeviceBuffer(size_t bytes) : bytes_(bytes) {
-    if (bytes) dev_check(hsps_malloc(DeviceContext::instance().ctx(), bytes, &p_));
+    if (bytes) {
+        ctx_ = DeviceContext::current();
+        dev_check(hsps_malloc(ctx_, bytes, &p_));
+    }
 }
 DeviceBuffer::~DeviceBuffer() {
-    if (p_) hsps_free(DeviceContext::instance().ctx(), p_);
+    if (p_) hsps_free(ctx_, p_);
 }
 DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
     if (this != &o) {
-        if (p_) hsps_free(DeviceContext::instance().ctx(), p_);
+        if (p_) hsps_free(ctx_, p_);
+        ctx_ = o.ctx_;
         p_ = o.p_;
         bytes_ = o.bytes_;
         o.p_ = nullptr;
@@ -43,7 +54,7 @@ DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
     return *this;
 }
 
-static hsps_ctx ctx() { return DeviceContext::instance().ctx(); }
+static hsps_ctx ctx() { return DeviceContext::current(); }
 
 // ---- Vector ---------------------------------------------------------------------------
 Vector::Vector(Index n, Real val) : data_(static_cast<Size>(n), val) {}
@@ -65,7 +76,7 @@ Vector& Vector::operator=(const Vector& o) {
 
 void Vector::host_sync() const {
     if (!host_valid_) {
-        if (!data_.empty()) dev_check(hsps_memcpy_d2h(ctx(), data_.data(), dev_.get(), data_.size() * sizeof(Real)));
+        if (!data_.empty()) dev_check(hsps_memcpy_d2h(dev_.ctx(), data_.data(), dev_.get(), data_.size() * sizeof(Real)));
         host_valid_ = true;
     }
 }
@@ -80,7 +91,7 @@ void Vector::ensure_dev() const {
 const double* Vector::device_data() const {
     ensure_dev();
     if (!dev_valid_) {
-        if (!data_.empty()) dev_check(hsps_memcpy_h2d(ctx(), dev_.get(), data_.data(), data_.size() * sizeof(Real)));
+        if (!data_.empty()) dev_check(hsps_memcpy_h2d(dev_.ctx(), dev_.get(), data_.data(), data_.size() * sizeof(Real)));
         dev_valid_ = true;
     }
     return static_cast<const double*>(dev_.get());

@@ -68,6 +68,8 @@ typedef struct hsps_params {
     int check_every;       /* host polls the device convergence flag every k iterations (32) */
     int batch_tile_rows;   /* batched solves: rows per workgroup tile (CSR tiles also stop at 2048
                               nonzeros); 0 = autotune on first solve (0) */
+    int batch_graphs;      /* batched solves: replay the iteration chunks as hipGraphs (1); 0 launches
+                              them directly, for callers that solve from several threads at once */
 } hsps_params;
 
 /* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64

@@ -1,0 +1,56 @@
+/* =============================================================================
+ * hsps_serve.h — C ABI of the solve server (lib/libhsps.so).  No reference
+ * counterpart: the reference solves one system per call
+ * (adaptive_selector.cpp:298-394); this serves a stream of independent
+ * requests, each through that adaptive ladder, with HBM-budget admission,
+ * concurrent workers (one device context and stream each) and continuous
+ * batching of small EASY (PCG + Jacobi) requests into hsps_batch launches.
+ * Status codes and hsps_params are those of hsps_dev.h.
+ * ========================================================================== */
+#ifndef HSPS_SERVE_H
+#define HSPS_SERVE_H
+
+#include "hsps_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSPS_ERR_SERVE (-8) /* runtime failure inside the server (std::runtime_error) */
+
+typedef struct hsps_server_s* hsps_server;
+
+typedef struct {
+    int converged;        /* 1 if x meets tol in the FP64 true residual */
+    int iterations;       /* of the attempt that produced x */
+    int state;            /* ladder state of that attempt: 0 EASY, 1 MODERATE, 2 HARD */
+    int batched;          /* 1 if the EASY attempt ran inside a batched launch */
+    int escalations;      /* ladder steps taken after the first attempt */
+    double true_residual; /* ||b - A x|| / ||b|| in FP64 */
+    double queue_s;       /* submit -> admission */
+    double latency_s;     /* submit -> completion */
+} hsps_serve_result;
+
+/* workers: concurrent solves; hbm_budget_bytes: admitted requests' estimated
+ * device footprint; batch_max: requests per batched launch (1 disables
+ * batching); batch_rows_max: largest batched request; batch_window_s: longest
+ * wait of an EASY request for company. */
+int hsps_server_create(int device, int workers, double hbm_budget_bytes, int batch_max, int batch_rows_max,
+                       double batch_window_s, hsps_server* out);
+/* Finishes every submitted request, then stops. */
+int hsps_server_destroy(hsps_server S);
+/* Copies the n x n CSR system and b; returns at once with a ticket.  p may be
+ * NULL (defaults); p->precision selects FP64 or mixed. */
+int hsps_server_submit(hsps_server S, int n, const int* h_row_ptr, const int* h_col_idx, const double* h_values,
+                       const double* h_b, const hsps_params* p, long long* ticket);
+/* *done = 1 once the request has completed. */
+int hsps_server_poll(hsps_server S, long long ticket, int* done);
+/* Blocks until the request completes; x (n doubles, may be NULL) receives the
+ * solution; the ticket is then forgotten.  Returns 1 converged, 0 not. */
+int hsps_server_wait(hsps_server S, long long ticket, double* h_x, hsps_serve_result* res);
+const char* hsps_server_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSPS_SERVE_H */

@@ -1,0 +1,110 @@
+"""Solve server (hsps_server_*, include/hsps_serve.h) on the GPU.
+
+Each request goes through the adaptive ladder of adaptive_selector.cpp:298-394
+(initial state from the features, escalation on failure, warm start); EASY
+requests are solved in batched launches.  Bars: every request's ladder state
+and escalation count equal the FP64 oracle's adaptive solve of the same system;
+its x meets tol in the FP64 true residual and lies within 1e-6 relative of
+the oracle's x; a request's result is bit-identical whether it is served in a
+mix (batched with others, other solves running concurrently, a tight HBM
+budget) or alone.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def build(kind, n, a=0.0, b=0.0):
+    if kind == "poisson":
+        A = O.Csr.poisson(n)
+    elif kind == "aniso":
+        A = O.Csr.aniso(n, a)
+    else:
+        A = O.Csr.convdiff(n, a, b, b)
+    rp, ci, v = A.arrays()
+    return A, (rp, ci, v)
+
+
+# EASY (SPD, well conditioned), MODERATE / HARD (nonsymmetric convection)
+REQS = [("poisson", 40), ("poisson", 24), ("aniso", 30, 0.3), ("convdiff", 24, 1.0, 0.0),
+        ("convdiff", 32, 0.02, 1.0), ("poisson", 64), ("convdiff", 20, 0.05, 1.0), ("poisson", 17)]
+
+
+def systems(reqs=REQS):
+    out = []
+    for k, spec in enumerate(reqs):
+        A, arr = build(*spec)
+        out.append((A, arr, A.spmv(O.uniform(500 + k, A.rows))))
+    return out
+
+
+def serve(H, items, params, **kw):
+    srv = H.Server(**kw)
+    tickets = [srv.submit(*arr, b, params) for _, arr, b in items]
+    res = [srv.wait(t) for t in tickets]
+    srv.close()
+    return res
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_server_follows_the_oracle_ladder(H, precision):
+    items = systems()
+    p = H.default_params(tol=1e-8, max_iter=2000, precision=precision)
+    res = serve(H, items, p, workers=3, batch_window_s=0.02)
+    for (A, _, b), (ok, x, r) in zip(items, res):
+        xo, so = O.adaptive(A, b, tol=1e-8, max_iter=2000)
+        assert so.converged and ok and r.converged
+        assert (r.state, r.escalations) == (so.state, so.escalations), (H.STATE_NAMES[r.state], so.state)
+        rel = np.linalg.norm(b - A.spmv(x)) / np.linalg.norm(b)
+        assert rel <= 1e-8 and abs(rel - r.true_residual) <= 1e-3 * rel + 1e-16
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+        if r.state == 0:
+            assert r.batched  # EASY requests of this size are served by batched launches
+
+
+def test_server_request_independent_of_the_mix(H):
+    items = systems()
+    p = H.default_params(tol=1e-8, max_iter=2000, precision="mixed")
+    mix = serve(H, items, p, workers=3, batch_window_s=0.05)
+    tight = serve(H, items, p, workers=2, hbm_budget=1.0, batch_window_s=0.0)  # one request in flight
+    for k in [0, 2, 3, 4, 5]:
+        alone = serve(H, [items[k]], p, workers=1)[0]
+        for other in (mix[k], tight[k]):
+            assert other[2].iterations == alone[2].iterations and other[2].state == alone[2].state
+            np.testing.assert_array_equal(other[1], alone[1])
+
+
+def test_server_escalates_like_the_reference(H):
+    # Too few iterations for CG + Jacobi: the ladder climbs, warm-started.
+    # Poisson 20^2 with 15 iterations ends at MODERATE (the oracle needs 8
+    # there), 48^2 with 10 at HARD (4); the two iteration caps also keep the
+    # EASY attempts in separate batched launches.
+    items = systems([("poisson", 20), ("poisson", 48)])
+    caps = [15, 10]
+    srv = H.Server(workers=2, batch_window_s=0.02)
+    tickets = [srv.submit(*arr, b, H.default_params(tol=1e-8, max_iter=m, precision="fp64"))
+               for (_, arr, b), m in zip(items, caps)]
+    res = [srv.wait(t) for t in tickets]
+    srv.close()
+    for (A, _, b), m, (ok, x, r) in zip(items, caps, res):
+        xo, so = O.adaptive(A, b, tol=1e-8, max_iter=m)
+        assert so.converged and ok
+        assert r.escalations == so.escalations >= 1 and r.state == so.state and r.batched
+        assert np.linalg.norm(b - A.spmv(x)) <= 1e-8 * np.linalg.norm(b)
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_server_rejects_bad_requests(H):
+    srv = H.Server(workers=1)
+    rp = np.array([0, 1, 2], np.int32)
+    with pytest.raises(H.HspsError) as e:
+        srv.submit(rp, np.array([0, 5], np.int32), np.ones(2), np.ones(2))  # column outside the matrix
+    assert e.value.code == -1 and b"column" in H.serve_lib().hsps_server_last_error()
+    with pytest.raises(H.HspsError):
+        srv.submit(np.array([0, 1, 0], np.int32), np.array([0], np.int32), np.ones(1), np.ones(2))
+    ok, x, r = srv.wait(srv.submit(rp, np.array([0, 1], np.int32), np.array([2.0, 4.0]), np.array([1.0, 1.0])))
+    assert ok and np.allclose(x, [0.5, 0.25])  # and the server still serves
+    srv.close()
