@@ -214,27 +214,35 @@ def bench_batched(H, ctx, ev, args):
     t_b64, it_b64, tr_b64, _ = batched("fp64", 1, 1)
     log(f"[bench] batched fp64 {np.mean(t_b64):.3f}s")
 
-    # plain FP64 device PCG, one system at a time
-    p64 = H.default_params(tol=1e-8, max_iter=20000, precision="fp64", check_every=32)
-    precs = [H.Precond(A, "jacobi", p64) for A in mats]
-    iters64 = np.zeros(len(mats), int)
-    t_single = []
-    for rep in range(2):  # first pass warms caches/allocations, second is timed
-        total = 0.0
-        for s, A in enumerate(mats):
-            n = A.rows
-            bs = ctx.array(b[row_off[s]:row_off[s + 1]])
-            xs_d = ctx.array(n=n)
-            ctx.sync()
-            t0 = time.perf_counter()
-            ok, st = H.solve(A, bs, xs_d, "cg", precs[s], p64)
-            ctx.sync()
-            total += time.perf_counter() - t0
-            assert ok and st.true_residual <= 1e-8
-            iters64[s] = st.iterations
-        t_single.append(total)
-    fp64_time = t_single[-1]
-    log(f"[bench] one-at-a-time fp64 {fp64_time:.3f}s")
+    # plain FP64 device PCG, one system at a time (cg_engine=0: CSR row-block SpMV and
+    # BLAS-1 kernels per iteration); then the same through the one-system batched engine
+    def one_at_a_time(engine):
+        p64 = H.default_params(tol=1e-8, max_iter=20000, precision="fp64", check_every=32, cg_engine=engine)
+        precs = [H.Precond(A, "jacobi", p64) for A in mats]
+        iters = np.zeros(len(mats), int)
+        t_single = []
+        for rep in range(2):  # first pass warms caches/allocations, second is timed
+            total = 0.0
+            for s, A in enumerate(mats):
+                n = A.rows
+                bs = ctx.array(b[row_off[s]:row_off[s + 1]])
+                xs_d = ctx.array(n=n)
+                ctx.sync()
+                t0 = time.perf_counter()
+                ok, st = H.solve(A, bs, xs_d, "cg", precs[s], p64)
+                ctx.sync()
+                total += time.perf_counter() - t0
+                assert ok and st.true_residual <= 1e-8
+                iters[s] = st.iterations
+            t_single.append(total)
+        for M in precs:
+            M.close()
+        return t_single[-1], iters
+
+    fp64_time, iters64 = one_at_a_time(0)
+    log(f"[bench] one-at-a-time fp64 (plain kernels) {fp64_time:.3f}s")
+    fp64e_time, iters64e = one_at_a_time(1)
+    log(f"[bench] one-at-a-time fp64 (batched engine) {fp64e_time:.3f}s")
 
     # CPU oracle on a bounded sample, scaled by nnz x iterations
     cpu = None
@@ -271,6 +279,12 @@ def bench_batched(H, ctx, ev, args):
         "fp64_baseline": {"value": fp64_rate, "unit": "solves/s", "mode": "plain FP64 device PCG, one system at a time",
                           "total_iterations": int(iters64.sum()), "seconds": fp64_time},
         "batched_fp64": {"value": len(systems) / float(np.mean(t_b64)), "unit": "solves/s"},
+        "fp64_engine_one_at_a_time": {"value": len(systems) / fp64e_time, "unit": "solves/s",
+                                      "mode": "FP64 PCG+Jacobi, one system at a time, each run as a one-system "
+                                              "batch (stencil storage, fused kernels)",
+                                      "total_iterations": int(iters64e.sum()), "seconds": fp64e_time,
+                                      "speedup_of_batched_mixed": (len(systems) / float(np.mean(t_mx))) /
+                                                                  (len(systems) / fp64e_time)},
         "mixed": {"max_iterations": int(it_mx.max()), "total_iterations": int(it_mx.sum()),
                   "max_true_residual": float(tr_mx.max()), "max_rel_err_vs_xstar": err_mx,
                   "p50_ms": 1e3 * float(np.percentile(t_mx, 50)), "p99_ms": 1e3 * float(np.percentile(t_mx, 99)),
@@ -296,8 +310,8 @@ def bench_single(H, ctx, ev, args):
     A.spmv(x_dev, b_dev)
     b_host = b_dev.download()
 
-    def run(precision, steps, warmup):
-        p = H.default_params(tol=1e-8, max_iter=20000, precision=precision, check_every=64)
+    def run(precision, steps, warmup, engine=1):
+        p = H.default_params(tol=1e-8, max_iter=20000, precision=precision, check_every=64, cg_engine=engine)
         M = H.Precond(A, "jacobi", p)
         xd = ctx.array(n=n)
         for _ in range(warmup):
@@ -315,7 +329,8 @@ def bench_single(H, ctx, ev, args):
             stats.append(st)
         return times, stats
 
-    t64, s64 = run("fp64", max(1, args.steps), args.warmup)
+    t64, s64 = run("fp64", max(1, args.steps), args.warmup, engine=0)  # plain kernel sequence
+    t64e, s64e = run("fp64", max(1, args.steps), args.warmup)
     tmx, smx = run("mixed", args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
@@ -335,7 +350,10 @@ def bench_single(H, ctx, ev, args):
         "value": value,
         "ms_per_step": 1e3 * sum(tmx) / len(tmx),
         "config": {"workload": f"2-D Poisson {side}x{side}, PCG+Jacobi, FP64 rel-res<=1e-8", "n": n, "nnz": len(v)},
-        "fp64_baseline": {"value": len(t64) / sum(t64), "unit": "solves/s", "iterations": s64[0].iterations},
+        "fp64_baseline": {"value": len(t64) / sum(t64), "unit": "solves/s", "iterations": s64[0].iterations,
+                          "mode": "plain FP64 device PCG (CSR row-block SpMV + BLAS-1 kernels)"},
+        "fp64_engine": {"value": len(t64e) / sum(t64e), "unit": "solves/s", "iterations": s64e[0].iterations,
+                        "mode": "FP64 PCG as a one-system batch (stencil storage, fused kernels)"},
         "mixed": {"iterations": smx[0].iterations, "reliable_updates": smx[0].reliable_updates,
                   "true_residual": smx[0].true_residual, "p50_ms": 1e3 * float(np.percentile(tmx, 50)),
                   "p99_ms": 1e3 * float(np.percentile(tmx, 99))},

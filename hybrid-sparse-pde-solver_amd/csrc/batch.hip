@@ -47,6 +47,32 @@ constexpr int kSmallRows = 20000;   // persistent one-workgroup path up to this 
 constexpr int kSmallThreads = 1024;
 constexpr int kMaxDiags = 8;        // DIA storage when a system's nonzeros lie on <= 8 diagonals
 
+// LDS staging window of K1 / K2 (dynamic LDS, in doubles): 16 KB unless a
+// system's halo needs more (measured on MI355X with halos up to 512 rows, a
+// 32 KB window costs more in occupancy than it saves in halo re-reads); at
+// most 64 KB.  A chunk fills the 16 KB window, or takes 2 halo rows when the
+// halo is wider (as far as 64 KB allows).
+constexpr int kStageMinDoubles = 2048, kStageMaxDoubles = 8192;
+
+// Rows per LDS chunk of a diagonal-stored system with halo h (largest |offset|)
+// and element size es: a function of the system alone, so that its rows are
+// reduced in the same order whatever window a launch provides.
+__host__ __device__ constexpr int stage_halo(int h, int es) { return (h + 16 / es - 1) / (16 / es) * (16 / es); }
+__host__ __device__ constexpr int stage_chunk(int h, int es) {
+    const int G = 16 / es, hr = stage_halo(h, es);
+    const int fill = kStageMinDoubles * 8 / es - 2 * hr - 2 * G;  // what the 16 KB window leaves
+    const int room = kStageMaxDoubles * 8 / es - 2 * hr - 2 * G;  // what the 64 KB window leaves
+    const int want = 2 * hr < room ? 2 * hr : room;
+    const int c = (fill > want ? fill : want) / G * G;
+    return c > 0 ? c : 0;
+}
+// LDS elements the chunk and its halo window take (more than any window when
+// no chunk fits).
+__host__ __device__ constexpr int stage_need(int h, int es) {
+    return stage_chunk(h, es) >= 16 / es ? 2 * stage_halo(h, es) + 2 * (16 / es) + stage_chunk(h, es) : 1 << 30;
+}
+
+
 // NORMAL: K1 updates p and forms q = A p, K2 updates x and r.
 // FLUSH: K1 replaces r by the FP64 true residual b - A (x64 + x_lo).
 // FLUSHED: after a FLUSH, K2 folds x_lo into x64; K1 then proceeds as NORMAL.
@@ -59,6 +85,7 @@ struct BState {
     double rho, rho_old, alpha, beta, bnorm, rel, rmax, true_rel;
     int iter, halt, done, flag, reason, reliable, restarts, conv;
     int pcur;  // which of the two p buffers holds the current search direction
+    double r0;  // initial ||r|| (absolute)
 };
 
 struct BArgs {
@@ -94,6 +121,7 @@ struct BArgs {
     const int* cd_ok;
     const double* cd64;
     const float* cd32;
+    int stage_doubles;  // dynamic LDS of K1 / K2, in doubles (>= kLdsNnz for the CSR path)
     BState* st;
     double* partials;
     int T;  // total tiles
@@ -209,7 +237,7 @@ struct Stencil {
 // once per row of the chunk plus its halo (coalesced, four rows in flight per
 // thread) into lds, own(j, x_j) is called for the chunk's own rows, then
 // f(i, x_i, (A x)_i) once per row with the row read from LDS.  Callers check
-// that 2 halo + kBlock <= cap.
+// that stage_need(halo) <= cap.
 template <class T>
 using lds_ptr = __attribute__((address_space(3))) T*;
 
@@ -218,7 +246,7 @@ __device__ __forceinline__ void stencil_staged(const Stencil<T>& A, int r0, int 
                                                XF&& xv, OWN&& own, F&& f) {
     constexpr int U = 4;
     const int tid = threadIdx.x;
-    const int C = cap - 2 * A.halo;
+    const int C = stage_chunk(A.halo, (int)sizeof(T));  // 2 halo + C <= stage_need <= cap (checked by the caller)
     const int lo = A.row0, hi = A.row0 + A.n_s;
     for (int c0 = r0; c0 < r1; c0 += C) {
         const int c1 = min(r1, c0 + C);
@@ -258,9 +286,6 @@ struct alignas(16) Pack {
     T e[G];
 };
 constexpr int kPad = 16;
-// LDS staging window of the stencil paths, in doubles (16 KB: measured on
-// MI355X, a 32 KB window costs more in occupancy than it saves in halo).
-constexpr int kStageDoubles = 2048;
 
 template <class T>
 __device__ __forceinline__ Pack<T> ld16(const T* p) { return *reinterpret_cast<const Pack<T>*>(p); }
@@ -295,12 +320,13 @@ template <class T>
 struct VecWin {
     static constexpr int G = Pack<T>::G;
     int H, C;
+    bool fits;
     __device__ __forceinline__ VecWin(const Stencil<T>& A, int cap) {
-        H = (A.halo + G - 1) / G * G;
-        C = (cap - 2 * H - 2 * G) / G * G;
+        H = stage_halo(A.halo, (int)sizeof(T));
+        C = stage_chunk(A.halo, (int)sizeof(T));
+        fits = stage_need(A.halo, (int)sizeof(T)) <= cap;
     }
-    // usable when a chunk holds at least 1024 rows
-    __device__ __forceinline__ bool ok() const { return C >= 1024; }
+    __device__ __forceinline__ bool ok() const { return fits; }
 };
 
 // K1 rows [r0, r1) of a five-point stencil system: p_new = dv r + beta p_old
@@ -535,6 +561,7 @@ struct OpInit1 {
         S.bnorm = sqrt(tot[1]);
         const double r0n = sqrt(tot[0]);
         S.rmax = r0n;
+        S.r0 = r0n;
         if (S.bnorm < kEps) {  // cg_solver.cpp:60-66
             S.done = S.conv = S.halt = 1;
             S.rel = r0n;
@@ -594,9 +621,9 @@ template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
 k_b_k1(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
        const double* __restrict__ b, const double* __restrict__ x64, const float* __restrict__ xlo) {
-    __shared__ __attribute__((aligned(16))) double prod_d[kStageDoubles > kLdsNnz ? kStageDoubles : kLdsNnz];
-    // every access to prod_d goes through address-space-3 pointers
-    const lds_ptr<double> prod64 = (lds_ptr<double>)prod_d;
+    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
+    // every access to the staging buffer goes through address-space-3 pointers
+    const lds_ptr<double> prod64 = (lds_ptr<double>)stage_lds;
     const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
@@ -630,14 +657,14 @@ k_b_k1(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ 
     if (kind != SK_CSR) {
         const Stencil<T> A(B, s);
         const lds_ptr<T> lds = (lds_ptr<T>)prod64;
-        constexpr int cap = kStageDoubles * (int)(sizeof(double) / sizeof(T));
+        const int cap = B.stage_doubles * (int)(sizeof(double) / sizeof(T));
         auto out = [&](int i, T pi, T sum) {
             if (kind == SK_DIA) q[i] = sum;
             red[0] += (double)pi * (double)sum;
         };
         if (kind == SK_STENCIL && A.nd <= 5 && cdok && VecWin<T>(A, cap).ok()) {  // 16-byte groups
             k1_stencil_vec(A, r0, r1, lds, cap, r, po, pn, cdv, beta, red[0]);
-        } else if (2 * A.halo + kBlock <= cap) {  // p_new staged in LDS, evaluated once per row
+        } else if (stage_need(A.halo, (int)sizeof(T)) <= cap) {  // p_new staged in LDS, evaluated once per row
             A.dispatch([&](auto nd) {
                 stencil_staged<nd()>(A, r0, r1, lds, cap, pnew, [&](int j, T v) { pn[j] = v; }, out);
             });
@@ -708,7 +735,7 @@ template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
 k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, const T* __restrict__ q,
        T* __restrict__ r, double* __restrict__ x64, float* __restrict__ xlo) {
-    __shared__ __attribute__((aligned(16))) double stage_d[kStageDoubles];
+    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
     const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
@@ -741,12 +768,12 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, cons
     };
     if (B.dia_kind[s] == SK_STENCIL) {
         const Stencil<T> A(B, s);
-        const lds_ptr<T> lds = (lds_ptr<T>)(lds_ptr<double>)stage_d;
-        constexpr int cap = kStageDoubles * (int)(sizeof(double) / sizeof(T));
+        const lds_ptr<T> lds = (lds_ptr<T>)(lds_ptr<double>)stage_lds;
+        const int cap = B.stage_doubles * (int)(sizeof(double) / sizeof(T));
         auto pv = [&](int j) { return p[j]; };
         if (B.dia_kind[s] == SK_STENCIL && A.nd <= 5 && cdok && VecWin<T>(A, cap).ok()) {  // 16-byte groups
             k2_stencil_vec<T, MIXED>(A, r0, r1, lds, cap, p, r, x64, xlo, cdv, alpha, red);
-        } else if (2 * A.halo + kBlock <= cap) {
+        } else if (stage_need(A.halo, (int)sizeof(T)) <= cap) {
             A.dispatch([&](auto nd) { stencil_staged<nd()>(A, r0, r1, lds, cap, pv, [](int, T) {}, body); });
         } else {
             for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], A.template row<0>(i, pv));
@@ -764,13 +791,9 @@ struct OpF2 {
         const double rn = sqrt(tot[0]);
         S.rel = rn / S.bnorm;
         S.iter += 1;
-        if (S.rel < B.tol) {  // cg_solver.cpp:143
-            if (MIXED) {
-                S.flag = BS_FLUSH;
-                S.reason = BR_CHECK;
-            } else {
-                S.done = S.conv = S.halt = 1;
-            }
+        if (S.rel < B.tol) {  // cg_solver.cpp:143, confirmed on the FP64 true residual
+            S.flag = BS_FLUSH;
+            S.reason = BR_CHECK;
             return;
         }
         if (MIXED) {
@@ -884,6 +907,7 @@ k_b_small(BArgs B, int k0, const double* __restrict__ b, double* __restrict__ x6
     true_residual(o);
     const double r0n = sqrt(o[0]);
     S.rmax = r0n;
+    S.r0 = r0n;
     S.rel = S.bnorm < kEps ? r0n : r0n / S.bnorm;
     S.true_rel = S.rel;
     if (S.bnorm < kEps || S.rel < B.tol) {
@@ -938,8 +962,7 @@ k_b_small(BArgs B, int k0, const double* __restrict__ b, double* __restrict__ x6
                 S.rel = rn / S.bnorm;
                 S.iter += 1;
                 if (MIXED) S.rmax = fmax(S.rmax, rn);
-                if (S.rel < B.tol) {
-                    if (!MIXED) { S.done = S.conv = 1; break; }
+                if (S.rel < B.tol) {  // confirmed on the FP64 true residual below
                     reason = BR_CHECK;
                 } else if (MIXED && rn < B.delta * S.rmax) {
                     reason = BR_RELIABLE;
@@ -1024,6 +1047,7 @@ struct Batch {
     DevBuf dia_kind, dia_nd, dia_off, dia_base, dia64, dia32, uv64, uv32, umask, cd_ok, cd64, cd32;
     int n_kind[3] = {0, 0, 0};  // systems per SK_* storage
     std::vector<int> h_kind;    // SK_* per internal system
+    int stage_doubles[2] = {kStageMinDoubles, kStageMinDoubles};  // K1 / K2 LDS: [0] FP64, [1] FP32
     // tiled schedule: rows per tile for CSR systems (0 = not tuned yet; also
     // capped at kLdsNnz nonzeros) and for diagonal-stored systems
     int tile_rows = 0, tile_rows_dia = 0;
@@ -1036,6 +1060,8 @@ struct Batch {
     int work_prec = -1;
     struct G { hipGraphExec_t exec = nullptr; int wave = -1, prec = -1, chunk = 0; };
     std::vector<G> graphs;
+    const double* graph_b = nullptr;  // the b and x the captured kernels point at
+    const double* graph_x = nullptr;
     ~Batch() {
         drop_graphs();
         if (join) (void)hipEventDestroy(join);
@@ -1188,6 +1214,20 @@ static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci
         base[k] = total;
         total += (long long)m * (e - a);
     }
+    // LDS window per precision: the largest halo of a tiled diagonal-stored system
+    for (int prec = 0; prec < 2; ++prec) {
+        const int es = prec ? 4 : 8, G = 16 / es;
+        int need = kStageMinDoubles;
+        for (int k = 0; k < std::min(B.n_large, S); ++k) {
+            if (kind[k] == SK_CSR) continue;
+            int h = 0;
+            for (int d = 0; d < nd[k]; ++d) h = std::max(h, std::abs(offs[(size_t)k * kMaxDiags + d]));
+            (void)G;
+            const long long dbl = ((long long)stage_need(h, es) * es + 7) / 8;
+            if (dbl <= kStageMaxDoubles) need = (int)std::max<long long>(need, dbl);  // else: rows from memory
+        }
+        B.stage_doubles[prec] = need;
+    }
     std::vector<double> d64((size_t)std::max<long long>(total, 1), 0.0);
     for (int k = 0; k < S; ++k) {
         ++B.n_kind[kind[k]];
@@ -1250,6 +1290,7 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     a.cd_ok = B.cd_ok.as<int>();
     a.cd64 = B.cd64.as<double>();
     a.cd32 = B.cd32.as<float>();
+    a.stage_doubles = B.stage_doubles[p.precision == HSPS_MIXED ? 1 : 0];
     a.st = B.st.as<BState>();
     a.partials = B.partials.as<double>();
     a.T = B.T;
@@ -1269,10 +1310,11 @@ static void enqueue_iters(const Batch& B, const BArgs& a, int k0, int k1, int t0
     float* xl = B.xlo.as<float>();
     const int ns = k1 - k0;
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, p0, p1, q, r, b, (const double*)x,
+        const size_t lds = sizeof(double) * (size_t)a.stage_doubles;
+        hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, q, r, b, (const double*)x,
                            (const float*)xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF1{});
-        hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, t0, (const T*)p0, (const T*)p1,
+        hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0, (const T*)p1,
                            (const T*)q, r, x, xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF2<MIXED>{});
     }
@@ -1369,6 +1411,11 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
         HSPS_CHECK_LAUNCH();
     }
     const int chunk = std::max(1, std::min(p.check_every > 0 ? p.check_every : 32, 64));
+    if (use_graph && (b != B.graph_b || x != B.graph_x)) {  // captured kernels hold b and x
+        B.drop_graphs();
+        B.graph_b = b;
+        B.graph_x = x;
+    }
     std::vector<BState> host(B.S);
     BState* pin = nullptr;
     HSPS_HIP(hipHostMalloc((void**)&pin, sizeof(BState) * (size_t)std::max(B.S, 1) * 2, hipHostMallocDefault));
@@ -1476,6 +1523,119 @@ static void autotune(Batch& B, const double* b, const double* x0, const hsps_par
     build_tiles(B, best_c, best_d);
 }
 
+// Validates the concatenated systems and builds B on ctx: storage per
+// system, tiles (CSR 256 rows, diagonal-stored 1024; retuned on the first
+// solve), waves.
+static void fill_batch(Batch& B, Ctx* ctx, int S, const int* row_off, const int* rp, const int* ci, const double* v) {
+    if (!ctx || S < 0 || !row_off || !rp) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: bad argument");
+    const int N = row_off[S];
+    if (row_off[0] != 0 || N < 0) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: row_off[0] must be 0");
+    const long long nnz = rp[N];
+    if (rp[0] != 0 || nnz < 0 || (nnz > 0 && (!ci || !v))) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: bad row_ptr");
+    std::vector<double> d64(N);
+    for (int s = 0; s < S; ++s) {
+        if (row_off[s + 1] < row_off[s]) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: row_off not monotone");
+        for (int i = row_off[s]; i < row_off[s + 1]; ++i) {
+            if (rp[i + 1] < rp[i]) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: row_ptr not monotone");
+            double d = 0.0;
+            for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                if (ci[k] < row_off[s] || ci[k] >= row_off[s + 1])
+                    throw Error(HSPS_ERR_INVALID, "hsps_batch_create: column outside its system's block (row " +
+                                                      std::to_string(i) + ")");
+                if (ci[k] == i) d = v[k];
+            }
+            if (std::abs(d) < kEps)  // jacobi.cpp:28-30
+                throw Error(HSPS_ERR_ZERO_DIAG, "JacobiPreconditioner: zero diagonal at row " + std::to_string(i));
+            d64[i] = 1.0 / d;
+        }
+    }
+    B.ctx = ctx;
+    B.S = S;
+    B.N = N;
+    B.nnz = nnz;
+    B.h_row_off.assign(row_off, row_off + S + 1);
+    B.h_rp.assign(rp, rp + N + 1);
+    ctx->set_current();
+    HSPS_HIP(hipStreamCreateWithFlags(&B.aux, hipStreamNonBlocking));
+    HSPS_HIP(hipEventCreateWithFlags(&B.join, hipEventDisableTiming));
+    HSPS_HIP(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+    B.rp = DevBuf(ctx, sizeof(int) * (size_t)(N + 1));
+    B.ci = DevBuf(ctx, sizeof(int) * (size_t)std::max<long long>(nnz, 1));
+    B.v64 = DevBuf(ctx, sizeof(double) * (size_t)std::max<long long>(nnz, 1));
+    B.v32 = DevBuf(ctx, sizeof(float) * (size_t)std::max<long long>(nnz, 1));
+    B.dinv64 = DevBuf(ctx, sizeof(double) * (size_t)std::max(N, 1));
+    B.dinv32 = DevBuf(ctx, sizeof(float) * (size_t)std::max(N, 1));
+    HSPS_HIP(hipMemcpy(B.rp.p, rp, sizeof(int) * (size_t)(N + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        HSPS_HIP(hipMemcpy(B.ci.p, ci, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
+        HSPS_HIP(hipMemcpy(B.v64.p, v, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
+        std::vector<float> f(v, v + nnz);
+        HSPS_HIP(hipMemcpy(B.v32.p, f.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice));
+    }
+    if (N) {
+        HSPS_HIP(hipMemcpy(B.dinv64.p, d64.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice));
+        std::vector<float> f(d64.begin(), d64.end());
+        HSPS_HIP(hipMemcpy(B.dinv32.p, f.data(), sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+    }
+    B.order.resize(S);
+    std::iota(B.order.begin(), B.order.end(), 0);
+    std::stable_sort(B.order.begin(), B.order.end(), [&](int a, int b) {
+        return rp[row_off[a + 1]] - rp[row_off[a]] > rp[row_off[b + 1]] - rp[row_off[b]];
+    });
+    B.n_large = 0;
+    for (int k = 0; k < S; ++k)
+        if (row_off[B.order[k] + 1] - row_off[B.order[k]] > kSmallRows) B.n_large = k + 1;
+    std::vector<int> r0(S), r1(S);
+    for (int k = 0; k < S; ++k) {
+        r0[k] = row_off[B.order[k]];
+        r1[k] = row_off[B.order[k] + 1];
+    }
+    B.sys_row0 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
+    B.sys_row1 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
+    if (S) {
+        HSPS_HIP(hipMemcpy(B.sys_row0.p, r0.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+        HSPS_HIP(hipMemcpy(B.sys_row1.p, r1.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+    }
+    build_dia(B, row_off, rp, ci, v, d64);
+    build_tiles(B, 256, 1024);
+    B.tile_rows = 0;  // not tuned yet
+    // Wave working-set budget (MB).  Measured on MI355X (512 systems,
+    // 1.9 TB of CG traffic): one wave beats MALL-sized (<=192 MB)
+    // waves — the fixed per-iteration launch cost of extra waves
+    // outweighs the cache hits — so the default is a single wave.
+    const char* wmb = std::getenv("HSPS_BATCH_WAVE_MB");
+    const size_t budget_mb = wmb ? (size_t)std::max(8, std::atoi(wmb)) : ((size_t)1 << 20);
+    build_waves(B, budget_mb << 20);
+}
+
+bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResult* out) {
+    if (A.solo_use == 0) return false;
+    if (!A.solo) {
+        auto B = std::make_shared<Batch>();
+        const int row_off[2] = {0, A.rows};
+        fill_batch(*B, A.ctx, 1, row_off, A.h_rp.data(), A.h_ci.data(), A.h_v.data());
+        // a large CSR-stored system: the plain kernel sequence (row-block SpMV) serves it better
+        if (B->n_large > 0 && B->h_kind[0] == SK_CSR) {
+            A.solo_use = 0;
+            return false;
+        }
+        build_tiles(*B, 512, 2048);  // fixed tiles: no autotuning trials inside a single solve
+        A.solo = B;
+        A.solo_use = 1;
+    }
+    static const bool no_graph = std::getenv("HSPS_BATCH_NO_GRAPH") != nullptr;
+    const std::vector<BState> st = run_batch(*A.solo, b, x, p, !no_graph && p.batch_graphs != 0);
+    const BState& S = st[0];
+    out->iterations = S.iter;
+    out->converged = S.conv;
+    out->restarts = S.restarts;
+    out->reliable_updates = S.reliable;
+    out->initial_residual = S.r0;
+    out->final_residual = S.rel;
+    out->true_residual = S.true_rel;
+    return true;
+}
+
 }  // namespace hsps_dev
 
 using namespace hsps_dev;
@@ -1487,87 +1647,10 @@ extern "C" {
 int hsps_batch_create(hsps_ctx ctx, int S, const int* row_off, const int* rp, const int* ci, const double* v,
                       hsps_batch* out) {
     return guarded([&] {
-        if (!ctx || !out || S < 0 || !row_off || !rp) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: bad argument");
-        const int N = row_off[S];
-        if (row_off[0] != 0 || N < 0) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: row_off[0] must be 0");
-        const long long nnz = rp[N];
-        if (rp[0] != 0 || nnz < 0 || (nnz > 0 && (!ci || !v))) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: bad row_ptr");
-        std::vector<double> d64(N);
-        for (int s = 0; s < S; ++s) {
-            if (row_off[s + 1] < row_off[s]) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: row_off not monotone");
-            for (int i = row_off[s]; i < row_off[s + 1]; ++i) {
-                if (rp[i + 1] < rp[i]) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: row_ptr not monotone");
-                double d = 0.0;
-                for (int k = rp[i]; k < rp[i + 1]; ++k) {
-                    if (ci[k] < row_off[s] || ci[k] >= row_off[s + 1])
-                        throw Error(HSPS_ERR_INVALID, "hsps_batch_create: column outside its system's block (row " +
-                                                          std::to_string(i) + ")");
-                    if (ci[k] == i) d = v[k];
-                }
-                if (std::abs(d) < kEps)  // jacobi.cpp:28-30
-                    throw Error(HSPS_ERR_ZERO_DIAG, "JacobiPreconditioner: zero diagonal at row " + std::to_string(i));
-                d64[i] = 1.0 / d;
-            }
-        }
+        if (!out) throw Error(HSPS_ERR_INVALID, "hsps_batch_create: bad argument");
         auto* B = new hsps_batch_s;
         try {
-            B->ctx = ctx;
-            B->S = S;
-            B->N = N;
-            B->nnz = nnz;
-            B->h_row_off.assign(row_off, row_off + S + 1);
-            B->h_rp.assign(rp, rp + N + 1);
-            ctx->set_current();
-            HSPS_HIP(hipStreamCreateWithFlags(&B->aux, hipStreamNonBlocking));
-            HSPS_HIP(hipEventCreateWithFlags(&B->join, hipEventDisableTiming));
-            HSPS_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
-            B->rp = DevBuf(ctx, sizeof(int) * (size_t)(N + 1));
-            B->ci = DevBuf(ctx, sizeof(int) * (size_t)std::max<long long>(nnz, 1));
-            B->v64 = DevBuf(ctx, sizeof(double) * (size_t)std::max<long long>(nnz, 1));
-            B->v32 = DevBuf(ctx, sizeof(float) * (size_t)std::max<long long>(nnz, 1));
-            B->dinv64 = DevBuf(ctx, sizeof(double) * (size_t)std::max(N, 1));
-            B->dinv32 = DevBuf(ctx, sizeof(float) * (size_t)std::max(N, 1));
-            HSPS_HIP(hipMemcpy(B->rp.p, rp, sizeof(int) * (size_t)(N + 1), hipMemcpyHostToDevice));
-            if (nnz) {
-                HSPS_HIP(hipMemcpy(B->ci.p, ci, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
-                HSPS_HIP(hipMemcpy(B->v64.p, v, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
-                std::vector<float> f(v, v + nnz);
-                HSPS_HIP(hipMemcpy(B->v32.p, f.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice));
-            }
-            if (N) {
-                HSPS_HIP(hipMemcpy(B->dinv64.p, d64.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice));
-                std::vector<float> f(d64.begin(), d64.end());
-                HSPS_HIP(hipMemcpy(B->dinv32.p, f.data(), sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
-            }
-            B->order.resize(S);
-            std::iota(B->order.begin(), B->order.end(), 0);
-            std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) {
-                return rp[row_off[a + 1]] - rp[row_off[a]] > rp[row_off[b + 1]] - rp[row_off[b]];
-            });
-            B->n_large = 0;
-            for (int k = 0; k < S; ++k)
-                if (row_off[B->order[k] + 1] - row_off[B->order[k]] > kSmallRows) B->n_large = k + 1;
-            std::vector<int> r0(S), r1(S);
-            for (int k = 0; k < S; ++k) {
-                r0[k] = row_off[B->order[k]];
-                r1[k] = row_off[B->order[k] + 1];
-            }
-            B->sys_row0 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
-            B->sys_row1 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
-            if (S) {
-                HSPS_HIP(hipMemcpy(B->sys_row0.p, r0.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-                HSPS_HIP(hipMemcpy(B->sys_row1.p, r1.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-            }
-            build_dia(*B, row_off, rp, ci, v, d64);
-            build_tiles(*B, 256, 1024);
-            B->tile_rows = 0;  // not tuned yet
-            // Wave working-set budget (MB).  Measured on MI355X (512 systems,
-            // 1.9 TB of CG traffic): one wave beats MALL-sized (<=192 MB)
-            // waves — the fixed per-iteration launch cost of extra waves
-            // outweighs the cache hits — so the default is a single wave.
-            const char* wmb = std::getenv("HSPS_BATCH_WAVE_MB");
-            const size_t budget_mb = wmb ? (size_t)std::max(8, std::atoi(wmb)) : ((size_t)1 << 20);
-            build_waves(*B, budget_mb << 20);
+            fill_batch(*B, ctx, S, row_off, rp, ci, v);
         } catch (...) {
             delete B;
             throw;

@@ -93,6 +93,8 @@ __global__ void k_f64_to_f32(int n, const double* __restrict__ x, float* __restr
 
 void Csr::upload_values() {
     ctx->set_current();
+    solo.reset();  // a one-system batch holds its own copy of the values
+    solo_use = -1;
     hipStream_t s = ctx->stream;  // stream-ordered: no device-wide sync (other contexts keep running)
     std::vector<double> d(rows, 0.0);
     for (int i = 0; i < rows; ++i)
@@ -292,6 +294,7 @@ void hsps_params_default(hsps_params* p) {
     p->check_every = 32;
     p->batch_tile_rows = 0;
     p->batch_graphs = 1;
+    p->cg_engine = 1;
 }
 
 int hsps_ctx_create(int device, hsps_ctx* out) {

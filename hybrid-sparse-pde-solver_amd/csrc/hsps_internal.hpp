@@ -92,6 +92,8 @@ constexpr int kMaxBlockRows = 256;
 constexpr int kMaxRedBlocks = 2048;  // cap for grid-stride reduction kernels
 
 // Device CSR: int32 pattern, FP64 values + FP32 copy, diagonal, row blocks.
+struct Batch;  // batch.hip
+
 struct Csr {
     Ctx* ctx = nullptr;
     int rows = 0, cols = 0, nnz = 0;
@@ -99,9 +101,23 @@ struct Csr {
     std::vector<double> h_v;
     DevBuf rp, ci, v64, v32, diag64, blocks;
     int n_blocks = 0;
+    // this matrix as a one-system batch (built on the first PCG + Jacobi
+    // solve that uses the batched engine; dropped when the values change)
+    std::shared_ptr<Batch> solo;
+    int solo_use = -1;  // -1 not decided, 0 plain kernels, 1 batched engine
     void upload_values();
     void build_schedule();
 };
+
+// PCG + Jacobi (omega = 1) on one system through the batched engine
+// (batch.hip): diagonal / stencil storage and the fused iteration kernels.
+struct SoloResult {
+    int iterations = 0, converged = 0, restarts = 0, reliable_updates = 0;
+    double initial_residual = 0, final_residual = 0, true_residual = 0;
+};
+// Returns false (nothing run) for a large CSR-stored system: the plain
+// kernel sequence serves those better.
+bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResult* out);
 
 // Row-block partition: consecutive rows, <= kMaxBlockRows rows and <=
 // kLdsNnz nonzeros per block; a longer row gets a block of its own.

@@ -788,6 +788,26 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const 
         hsps_params p;
         if (pp) p = *pp; else hsps_params_default(&p);
         A->ctx->set_current();
+        if (p.cg_engine && M && M->kind == HSPS_PRECOND_JACOBI && M->jacobi_omega == 1.0) {
+            const double t0 = now_s();
+            SoloResult r;
+            if (solo_cg(*A, b, x, p, &r)) {
+                hsps_stats s{};
+                s.iterations = r.iterations;
+                s.converged = r.converged;
+                s.restarts = r.restarts;
+                s.reliable_updates = r.reliable_updates;
+                s.initial_residual = r.initial_residual;
+                s.final_residual = r.final_residual;
+                s.true_residual = r.true_residual;
+                s.solve_time_s = now_s() - t0;
+                const long long it = r.iterations, n = A->rows;
+                const long long vb = p.precision == HSPS_MIXED ? 4 : 8;
+                finish_stats(&s, it * (2LL * A->nnz + 10LL * n), it * (spmv_bytes(n, A->nnz) + 7LL * n * vb));
+                if (st) *st = s;
+                return r.converged;
+            }
+        }
         return p.precision == HSPS_MIXED ? run_cg<float>(*A, M, b, x, p, st) : run_cg<double>(*A, M, b, x, p, st);
     });
 }

@@ -159,6 +159,7 @@ static void setup_jacobi(Precond& M, const Csr& A, double omega) {
     }
     M.dinv64 = upload(M.ctx, d.data(), d.size() * sizeof(double));
     M.dinv32 = upload_f32(M.ctx, d);
+    M.jacobi_omega = omega;
 }
 
 static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {

@@ -44,6 +44,7 @@ struct Precond {
     const Csr* A0 = nullptr;  // the operator it was built for (owned by the caller)
     // Jacobi
     DevBuf dinv64, dinv32;
+    double jacobi_omega = 1.0;
     // ILU
     std::unique_ptr<IluFactor> ilu;
     // AMG

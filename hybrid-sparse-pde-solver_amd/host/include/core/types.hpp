@@ -110,6 +110,7 @@ struct SolverParams {
     Precision precision = Precision::FP64;
     Real reliable_delta = 0.1;
     int check_every = 32;
+    bool device_graphs = true;  ///< replay fused iteration chunks as hipGraphs (off when threads share the device)
 };
 
 }  // namespace hsps

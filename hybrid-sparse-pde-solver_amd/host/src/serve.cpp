@@ -65,6 +65,7 @@ SolveServer::Ticket SolveServer::submit(SparseMatrix A, std::vector<Real> b, con
     r->A = std::move(A);
     r->b = std::move(b);
     r->p = params;
+    r->p.device_graphs = false;  // workers share the device: no stream capture
     r->t_submit = Clock::now();
     Ticket id;
     {

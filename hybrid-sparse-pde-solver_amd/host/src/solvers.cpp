@@ -33,6 +33,7 @@ hsps_params to_device_params(const SolverParams& p) {
     d.precision = p.precision == Precision::MIXED ? HSPS_MIXED : HSPS_FP64;
     d.reliable_delta = p.reliable_delta;
     d.check_every = p.check_every;
+    d.batch_graphs = p.device_graphs ? 1 : 0;
     return d;
 }
 

@@ -70,6 +70,9 @@ typedef struct hsps_params {
                               nonzeros); 0 = autotune on first solve (0) */
     int batch_graphs;      /* batched solves: replay the iteration chunks as hipGraphs (1); 0 launches
                               them directly, for callers that solve from several threads at once */
+    int cg_engine;         /* PCG + Jacobi (omega 1) on one system: 1 = run it as a one-system batch
+                              (diagonal / stencil storage, fused kernels) when the matrix is
+                              diagonal-stored or small (1); 0 = the plain kernel sequence */
 } hsps_params;
 
 /* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64

@@ -212,3 +212,23 @@ def test_batch_tiled_alone_equals_in_mix(H, ctx, precision):
         ok1, it1, _, _ = Bs.solve(bs, xs, p)
         assert ok1 and it1[0] == it_mix[s]
         np.testing.assert_array_equal(xs.download(), x_mix[B.row_off[s]:B.row_off[s + 1]])
+
+
+def test_batch_solves_with_new_vectors_each_call(H, ctx):
+    """The iteration chunks are replayed as hipGraphs whose kernels hold the
+    b and x pointers: a later solve on other vectors must not replay them."""
+    arrs, _, oracles = tiled_systems(H)
+    arrs, oracles = arrs[:2], oracles[:2]
+    B = H.Batch.from_systems(ctx, arrs)
+    p = H.default_params(tol=1e-8, max_iter=5000, precision="mixed", batch_tile_rows=1024)
+    outs = []
+    for seed in (1, 2, 3):
+        bs = [A.spmv(O.uniform(seed * 10 + s, A.rows)) for s, A in enumerate(oracles)]
+        bd, xd = ctx.array(np.concatenate(bs)), ctx.array(n=B.n)
+        ok, it, cv, tr = B.solve(bd, xd, p)
+        x = xd.download()
+        assert ok and (tr <= 1e-8).all()
+        for s, A in enumerate(oracles):
+            xs = x[B.row_off[s]:B.row_off[s + 1]]
+            assert np.linalg.norm(bs[s] - A.spmv(xs)) <= 1e-8 * np.linalg.norm(bs[s])
+        outs.append((bd, xd))  # keep the buffers alive: each call gets new addresses

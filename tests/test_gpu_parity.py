@@ -224,3 +224,39 @@ def test_errors_as_status_codes(H, ctx):
     with pytest.raises(H.HspsError) as e:
         H.Precond(Ad, "ilu", H.default_params(drop_tol=0.0))
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cg_engine_matches_plain_kernels(H, ctx, precision):
+    """PCG + Jacobi on one system runs as a one-system batch (diagonal /
+    stencil storage, fused kernels) unless it is a large CSR-stored matrix.
+    Both paths meet the oracle's bars; the CSR fallback is the plain path
+    itself (bit-identical)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    n = 30000
+    i = np.repeat(np.arange(n), 3)
+    j = rng.integers(0, n, 3 * n)
+    G = sp.coo_matrix((-np.ones(3 * n), (i, j)), shape=(n, n)).tocsr()
+    G = G + G.T
+    G.setdiag(0)
+    G.eliminate_zeros()
+    L = (G + sp.diags(-np.asarray(G.sum(axis=1)).ravel() + 0.05)).tocsr()
+    L.sort_indices()
+    cases = [O.Csr.poisson(200), O.Csr.aniso(150, 0.05), O.Csr.poisson(40),
+             O.Csr.from_csr(n, n, L.indptr.astype(np.int32), L.indices.astype(np.int32), L.data)]
+    for k, A in enumerate(cases):
+        b = A.spmv(O.uniform(40 + k, A.rows))
+        xo, so = O.solve(A, b, "cg", "jacobi", tol=1e-8, max_iter=20000)
+        res = {}
+        for engine in (0, 1):
+            ok, st, x = _solve_dev(H, ctx, A, b, "cg", "jacobi", tol=1e-8, max_iter=20000, precision=precision,
+                                   cg_engine=engine)
+            assert ok and _rel(A, b, x) < 1e-8 and abs(st.true_residual - _rel(A, b, x)) <= 1e-3 * st.true_residual
+            assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), (k, engine)
+            bound = max(1, 0.05 * so.iterations) if precision == "fp64" else max(2, 0.10 * so.iterations)
+            assert abs(st.iterations - so.iterations) <= bound, (k, engine, st.iterations, so.iterations)
+            res[engine] = (st.iterations, x)
+        if k == 3:  # large and CSR-stored: the plain kernels run either way
+            assert res[0][0] == res[1][0]
+            np.testing.assert_array_equal(res[0][1], res[1][1])
