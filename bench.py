@@ -362,6 +362,59 @@ def bench_single(H, ctx, ev, args):
     }
 
 
+def bench_convdiff(H, ctx, args):
+    """Convection-diffusion 1024^2 (n = 1,048,576), Pe = |beta|/eps = 100 with
+    |beta| = 1 at a seeded angle, FGMRES(30) + ILU(0) to FP64 rel-res <= 1e-10:
+    mixed (FP32 inner, FP64 restarts) vs FP64."""
+    side = args.grid if args.grid != 2048 else 1024
+    n = side * side
+    th = float(np.random.default_rng(31).uniform(0.0, 2 * np.pi))
+    eps = 1.0 / 100.0
+    rp, ci, v = H.convection_diffusion_2d(side, eps, float(np.cos(th)), float(np.sin(th)))
+    A = H.Csr(ctx, n, n, rp, ci, v)
+    xd_star = ctx.array(H.uniform_pm1(77, n))
+    b_dev = ctx.array(n=n)
+    A.spmv(xd_star, b_dev)
+
+    def run(precision, steps, warmup):
+        # mixed: GMRES-based iterative refinement with an inner tolerance of 1e-5 per FP32 cycle
+        p = H.default_params(tol=1e-10, max_iter=5000, restart_size=30, drop_tol=0.0, precision=precision,
+                             fgmres_inner_rel=1e-5 if precision == "mixed" else 0.0)
+        M = H.Precond(A, "ilu", p)
+        xd = ctx.array(n=n)
+        times, stats = [], []
+        for k in range(warmup + steps):
+            H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
+            ctx.sync()
+            t0 = time.perf_counter()
+            ok, st = H.solve(A, b_dev, xd, "fgmres", M, p)
+            ctx.sync()
+            if k >= warmup:
+                times.append(time.perf_counter() - t0)
+                assert ok and st.true_residual <= 1e-10, (precision, st.true_residual)
+                stats.append(st)
+        M.close()
+        return times, stats
+
+    t64, s64 = run("fp64", max(1, args.steps), args.warmup)
+    tmx, smx = run("mixed", args.steps, args.warmup)
+    value = len(tmx) / sum(tmx)
+    return {
+        "workload": f"convdiff{side}_pe100_fgmres30_ilu0",
+        "value": value,
+        "ms_per_step": 1e3 * sum(tmx) / len(tmx),
+        "config": {"workload": f"convection-diffusion {side}x{side}, Pe 100 (|beta| 1, angle {th:.3f}), "
+                               "FGMRES(30) + ILU(0), FP64 rel-res<=1e-10; mixed: FP32 cycles to a 1e-5 "
+                               "residual reduction, FP64 restarts", "n": n, "nnz": len(v)},
+        "fp64_baseline": {"value": len(t64) / sum(t64), "unit": "solves/s", "iterations": s64[0].iterations,
+                          "mode": "FP64 FGMRES(30) + ILU(0) on the device"},
+        "mixed": {"iterations": smx[0].iterations, "restarts": smx[0].restarts,
+                  "true_residual": smx[0].true_residual},
+        "speedup_vs_fp64": value / (len(t64) / sum(t64)),
+        "cpu_baseline": None,
+    }
+
+
 def serve_specs(count, nmax, seed=77):
     """Request mix: n log-uniform in [1e3, nmax] (grid side rounded), one third
     each Poisson, anisotropic (eps log-uniform in [1e-3, 1]) and
@@ -448,7 +501,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "single", "serve"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "single", "convdiff", "serve"], default="batched")
     ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--workers", type=int, default=4, help="serve: concurrent solves")
@@ -485,6 +538,8 @@ def main():
         res = bench_batched(H, ctx, ev, args)
     elif args.workload == "serve":
         res = bench_serve(H, ctx, args)
+    elif args.workload == "convdiff":
+        res = bench_convdiff(H, ctx, args)
     else:
         res = bench_single(H, ctx, ev, args)
     r64 = roof["fp64"]

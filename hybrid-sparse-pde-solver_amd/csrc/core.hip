@@ -295,6 +295,7 @@ void hsps_params_default(hsps_params* p) {
     p->batch_tile_rows = 0;
     p->batch_graphs = 1;
     p->cg_engine = 1;
+    p->fgmres_inner_rel = 0.0;
 }
 
 int hsps_ctx_create(int device, hsps_ctx* out) {

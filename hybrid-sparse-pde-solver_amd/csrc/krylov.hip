@@ -26,6 +26,7 @@
 // =============================================================================
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -48,6 +49,7 @@ struct CgState {
 
 struct GmState {
     double beta2, beta, res, tol_abs, inv_h;
+    double inner_rel;  // mixed: a cycle also ends once res < inner_rel * beta (0 in FP64)
     int jdone, cycle_done, m, pad;
 };
 
@@ -345,7 +347,7 @@ struct OpGivens {
         gm_rot(g[j], g[j + 1], cs[j], sn[j]);
         st->res = fabs(g[j + 1]);
         st->jdone = j + 1;
-        if (st->res < st->tol_abs) st->cycle_done = 1;
+        if (st->res < st->tol_abs || st->res < st->inner_rel * st->beta) st->cycle_done = 1;
     }
 };
 
@@ -688,6 +690,7 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     GmState init{};
     init.tol_abs = tol_abs;
     init.m = m;
+    init.inner_rel = MIXED ? std::max(0.0, p.fgmres_inner_rel) : 0.0;
     HSPS_HIP(hipMemcpyAsync(d, &init, sizeof init, hipMemcpyHostToDevice, s));
     HSPS_HIP(hipStreamSynchronize(s));
 

@@ -106,6 +106,127 @@ k_tri_upper(int n_levels, const int* __restrict__ lptr, const int* __restrict__ 
     }
 }
 
+// The same sweeps over the level-ordered ELL structure (TriEll): each row's
+// dependencies and values load independently of one another, a dependency on
+// the previous level reads the LDS copy that level left, and the sums run in
+// the same order as above (ascending columns), so the results are the same
+// bits.  LDS holds two levels of kTriLds values.
+constexpr int kTriLds = 2048;
+constexpr int kTriNone = std::numeric_limits<int>::max();
+//
+// ALL_LDS (every dependency is on the previous level, so the sweep never
+// reads its own global output): the barrier between levels orders LDS only —
+// the output stores stay in flight — and each lane's first row of the levels
+// kTriAhead ahead is loaded into a register ring (structure, row id, the
+// input taken in level order from `in_pos`, 1/U(i,i)) while the current level
+// computes, so a level costs LDS latency and a barrier rather than an HBM
+// round trip.
+constexpr int kTriAhead = 3;
+template <int K, class V, class T, bool UPPER>
+__global__ void __launch_bounds__(kTriThreads)
+k_tri_ell(int n_levels, const int* __restrict__ lptr, const int* __restrict__ order, const int* __restrict__ dep,
+          const V* __restrict__ val, const T* __restrict__ invd, const T* __restrict__ in, T* out, int use_lds,
+          const int* skip) {
+    __shared__ T buf[2 * kTriLds];
+    if (skip && *skip) return;
+    for (int lev = 0; lev < n_levels; ++lev) {
+        const int a = lptr[lev], b = lptr[lev + 1];
+        T* cur = buf + (lev & 1) * kTriLds;
+        const T* prev = buf + ((lev & 1) ^ 1) * kTriLds;
+        for (int idx = a + (int)threadIdx.x; idx < b; idx += kTriThreads) {
+            int d[K];
+            V v[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                d[k] = dep[(size_t)idx * K + k];
+                v[k] = val[(size_t)idx * K + k];
+            }
+            const int i = order[idx];
+            T s = in[i];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (d[k] == kTriNone) continue;
+                const T yj = d[k] < 0 ? prev[-d[k] - 1] : out[d[k]];
+                s -= (T)v[k] * yj;
+            }
+            if (UPPER) s = s * invd[idx];
+            out[i] = s;
+            if (use_lds) cur[idx - a] = s;
+        }
+        __syncthreads();
+    }
+}
+
+template <int K, class V, class T>
+struct TriRow {
+    int d[K];
+    V v[K];
+    int i;
+    T x, dv;
+};
+
+template <int K, class V, class T, bool UPPER>
+__global__ void __launch_bounds__(kTriThreads)
+k_tri_ell_lds(int n_levels, const int* __restrict__ lptr, const int* __restrict__ order, const int* __restrict__ dep,
+              const V* __restrict__ val, const T* __restrict__ invd, const T* __restrict__ in_pos, T* out,
+              const int* skip) {
+    __shared__ T buf[2 * kTriLds];
+    if (skip && *skip) return;
+    const int tid = threadIdx.x;
+    TriRow<K, V, T> ring[kTriAhead];
+    auto load = [&](TriRow<K, V, T>& R, int idx) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            R.d[k] = dep[(size_t)idx * K + k];
+            R.v[k] = val[(size_t)idx * K + k];
+        }
+        R.i = order[idx];
+        R.x = in_pos[idx];
+        if (UPPER) R.dv = invd[idx];
+    };
+    auto solve_row = [&](const TriRow<K, V, T>& R, const T* prev, T* cur, int slot) {
+        T s = R.x;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (R.d[k] != kTriNone) s -= (T)R.v[k] * prev[-R.d[k] - 1];
+        if (UPPER) s = s * R.dv;
+        out[R.i] = s;
+        cur[slot] = s;
+    };
+#pragma unroll
+    for (int r = 0; r < kTriAhead; ++r)
+        if (r < n_levels && lptr[r] + tid < lptr[r + 1]) load(ring[r], lptr[r] + tid);
+    for (int lev0 = 0; lev0 < n_levels; lev0 += kTriAhead) {
+#pragma unroll
+        for (int r = 0; r < kTriAhead; ++r) {  // constant ring slots
+            const int lev = lev0 + r;
+            if (lev >= n_levels) break;
+            const int a = lptr[lev], b = lptr[lev + 1];
+            T* cur = buf + (lev & 1) * kTriLds;
+            const T* prev = buf + ((lev & 1) ^ 1) * kTriLds;
+            if (a + tid < b) solve_row(ring[r], prev, cur, tid);
+            for (int idx = a + tid + kTriThreads; idx < b; idx += kTriThreads) {  // levels wider than the workgroup
+                TriRow<K, V, T> R;
+                load(R, idx);
+                solve_row(R, prev, cur, idx - a);
+            }
+            const int nl = lev + kTriAhead;
+            if (nl < n_levels && lptr[nl] + tid < lptr[nl + 1]) load(ring[r], lptr[nl] + tid);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        }
+    }
+}
+
+// in_pos[p] = in[order[p]]: a sweep's input in level order.
+template <class T>
+__global__ void k_tri_gather(int n, const int* __restrict__ order, const T* __restrict__ in, T* __restrict__ in_pos,
+                             const int* skip) {
+    if (skip && *skip) return;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) in_pos[p] = in[order[p]];
+}
+
 // ---- helpers ---------------------------------------------------------------------
 static DevBuf upload(Ctx* c, const void* h, size_t bytes) {
     DevBuf b(c, bytes);
@@ -141,6 +262,7 @@ static TriSchedule make_schedule(Ctx* c, const std::vector<int>& lev, int n_leve
     }
     std::vector<int> order(n), pos(s.h_level_ptr.begin(), s.h_level_ptr.end() - 1);
     for (int i = 0; i < n; ++i) order[pos[lev[i]]++] = i;
+    s.h_order = order;
     s.order = upload(c, order.data(), order.size() * sizeof(int));
     s.level_ptr = upload(c, s.h_level_ptr.data(), s.h_level_ptr.size() * sizeof(int));
     return s;
@@ -160,6 +282,52 @@ static void setup_jacobi(Precond& M, const Csr& A, double omega) {
     M.dinv64 = upload(M.ctx, d.data(), d.size() * sizeof(double));
     M.dinv32 = upload_f32(M.ctx, d);
     M.jacobi_omega = omega;
+}
+
+// Level-ordered ELL of the entries a sweep uses (lower: j < i; upper: j > i),
+// ascending columns per row as the CSR scan visits them.
+static TriEll make_tri_ell(Ctx* c, const hsps_host::HostCsr& F, const std::vector<int>& lev, const TriSchedule& S,
+                           bool lower, const std::vector<double>& invd) {
+    TriEll E;
+    const int n = F.rows;
+    int maxc = 0;
+    for (int i = 0; i < n; ++i) {
+        int cnt = 0;
+        for (int k = F.rp[i]; k < F.rp[i + 1]; ++k) cnt += lower ? F.ci[k] < i : F.ci[k] > i;
+        maxc = std::max(maxc, cnt);
+    }
+    if (maxc > 8) return E;  // the CSR sweep serves such factors
+    const int K = maxc <= 2 ? 2 : maxc <= 4 ? 4 : 8;
+    E.K = K;
+    E.lds = S.max_level_width <= kTriLds;
+    std::vector<int> pos_of(n);
+    for (int p = 0; p < n; ++p) pos_of[S.h_order[p]] = p;
+    std::vector<int> dep((size_t)n * K, kTriNone);
+    std::vector<double> val((size_t)n * K, 0.0), ip(lower ? 0 : n);
+    bool all_lds = E.lds;
+    for (int p = 0; p < n; ++p) {
+        const int i = S.h_order[p], li = lev[i];
+        int c = 0;
+        for (int k = F.rp[i]; k < F.rp[i + 1]; ++k) {
+            const int j = F.ci[k];
+            if (lower ? j >= i : j <= i) continue;
+            const bool from_lds = E.lds && li > 0 && lev[j] == li - 1;
+            dep[(size_t)p * K + c] = from_lds ? -(1 + pos_of[j] - S.h_level_ptr[li - 1]) : j;
+            all_lds = all_lds && from_lds;
+            val[(size_t)p * K + c] = F.v[k];
+            ++c;
+        }
+        if (!lower) ip[p] = invd[i];
+    }
+    E.all_lds = all_lds;
+    E.dep = upload(c, dep.data(), dep.size() * sizeof(int));
+    E.val64 = upload(c, val.data(), val.size() * sizeof(double));
+    E.val32 = upload_f32(c, val);
+    if (!lower) {
+        E.invd64 = upload(c, ip.data(), ip.size() * sizeof(double));
+        E.invd32 = upload_f32(c, ip);
+    }
+    return E;
 }
 
 static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {
@@ -187,12 +355,52 @@ static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {
     auto up = hsps_host::tri_levels(F, false, &nl_up);
     f->fwd = make_schedule(M.ctx, lo, nl_lo);
     f->bwd = make_schedule(M.ctx, up, nl_up);
+    f->fwd_ell = make_tri_ell(M.ctx, F, lo, f->fwd, true, invd);
+    f->bwd_ell = make_tri_ell(M.ctx, F, up, f->bwd, false, invd);
     f->inv_diag64 = upload(M.ctx, invd.data(), invd.size() * sizeof(double));
     f->inv_diag32 = upload_f32(M.ctx, invd);
     f->tmp64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
     f->tmp32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
+    f->pos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
+    f->pos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
     f->lu = to_dev(M.ctx, std::move(F));
     M.ilu = std::move(f);
+}
+
+template <int K, class T, bool UPPER>
+static void tri_ell_launch(const TriSchedule& S, const TriEll& E, const T* in, T* out, T* in_pos, hipStream_t s,
+                           const int* skip) {
+    const bool dbl = std::is_same<T, double>::value;
+    const T* invd = UPPER ? (dbl ? (const T*)E.invd64.p : (const T*)E.invd32.p) : nullptr;
+    const int n = S.n_levels ? S.h_level_ptr.back() : 0;
+    if (E.all_lds) {
+        hipLaunchKernelGGL(k_tri_gather<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, S.order.as<int>(), in, in_pos,
+                           skip);
+        if (dbl)
+            hipLaunchKernelGGL((k_tri_ell_lds<K, double, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s, S.n_levels,
+                               S.level_ptr.as<int>(), S.order.as<int>(), E.dep.as<int>(), E.val64.as<double>(), invd,
+                               (const T*)in_pos, out, skip);
+        else
+            hipLaunchKernelGGL((k_tri_ell_lds<K, float, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s, S.n_levels,
+                               S.level_ptr.as<int>(), S.order.as<int>(), E.dep.as<int>(), E.val32.as<float>(), invd,
+                               (const T*)in_pos, out, skip);
+    } else if (dbl) {
+        hipLaunchKernelGGL((k_tri_ell<K, double, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s, S.n_levels,
+                           S.level_ptr.as<int>(), S.order.as<int>(), E.dep.as<int>(), E.val64.as<double>(), invd, in,
+                           out, E.lds, skip);
+    } else {
+        hipLaunchKernelGGL((k_tri_ell<K, float, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s, S.n_levels,
+                           S.level_ptr.as<int>(), S.order.as<int>(), E.dep.as<int>(), E.val32.as<float>(), invd, in,
+                           out, E.lds, skip);
+    }
+    HSPS_CHECK_LAUNCH();
+}
+template <class T, bool UPPER>
+static void tri_ell_sweep(const TriSchedule& S, const TriEll& E, const T* in, T* out, T* in_pos, hipStream_t s,
+                          const int* skip) {
+    if (E.K == 2) tri_ell_launch<2, T, UPPER>(S, E, in, out, in_pos, s, skip);
+    else if (E.K == 4) tri_ell_launch<4, T, UPPER>(S, E, in, out, in_pos, s, skip);
+    else tri_ell_launch<8, T, UPPER>(S, E, in, out, in_pos, s, skip);
 }
 
 static void alloc_level_work(Ctx* c, AmgLevelDev& L, int n) {
@@ -361,7 +569,11 @@ static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int
             const bool dbl = std::is_same<T, double>::value;
             T* y = dbl ? (T*)f.tmp64.p : (T*)f.tmp32.p;
             const T* invd = dbl ? (const T*)f.inv_diag64.p : (const T*)f.inv_diag32.p;
-            if (dbl) {
+            if (f.fwd_ell.K && f.bwd_ell.K) {
+                T* ip = dbl ? (T*)f.pos64.p : (T*)f.pos32.p;
+                tri_ell_sweep<T, false>(f.fwd, f.fwd_ell, r, y, ip, s, skip);
+                tri_ell_sweep<T, true>(f.bwd, f.bwd_ell, (const T*)y, z, ip, s, skip);
+            } else if (dbl) {
                 hipLaunchKernelGGL((k_tri_lower<double, T>), dim3(1), dim3(kTriThreads), 0, s, f.fwd.n_levels,
                                    f.fwd.level_ptr.as<int>(), f.fwd.order.as<int>(), f.lu->rp.as<int>(),
                                    f.lu->ci.as<int>(), f.lu->v64.as<double>(), r, y, skip);

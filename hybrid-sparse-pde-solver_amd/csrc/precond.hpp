@@ -15,9 +15,23 @@ namespace hsps_dev {
 struct TriSchedule {
     int n_levels = 0;
     std::vector<int> h_level_ptr;  // n_levels+1 offsets into order
+    std::vector<int> h_order;
     DevBuf order;                  // rows sorted by level
     DevBuf level_ptr;
     int max_level_width = 0;
+};
+
+// The off-diagonal entries a sweep uses, in level order, K per row (ELL,
+// padded): the sweep then reads a row's structure with independent loads, and
+// a dependency on the previous level (kept in LDS by the sweep) as an LDS
+// index.  K = 0 when some row has more than 8 such entries.
+struct TriEll {
+    int K = 0;
+    int lds = 0;      // every level fits the LDS window
+    int all_lds = 0;  // every dependency is on the previous level (read from LDS)
+    DevBuf dep;       // per position and slot: row j (>= 0), -(1 + index in the previous level), or INT_MAX
+    DevBuf val64, val32;
+    DevBuf invd64, invd32;  // backward sweep: 1 / U(i,i) per position
 };
 
 struct IluFactor {
@@ -25,7 +39,9 @@ struct IluFactor {
     DevBuf diag_pos;          // position of U(i,i) in lu (or -1)
     DevBuf inv_diag64, inv_diag32;
     TriSchedule fwd, bwd;
+    TriEll fwd_ell, bwd_ell;
     DevBuf tmp64, tmp32;      // y of L y = r
+    DevBuf pos64, pos32;      // a sweep's input in level order
 };
 
 struct AmgLevelDev {

@@ -111,6 +111,7 @@ struct SolverParams {
     Real reliable_delta = 0.1;
     int check_every = 32;
     bool device_graphs = true;  ///< replay fused iteration chunks as hipGraphs (off when threads share the device)
+    Real fgmres_inner_rel = 0.0;  ///< mixed FGMRES: FP32 cycle ends at this residual reduction (0: at tol)
 };
 
 }  // namespace hsps

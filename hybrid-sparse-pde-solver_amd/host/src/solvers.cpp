@@ -34,6 +34,7 @@ hsps_params to_device_params(const SolverParams& p) {
     d.reliable_delta = p.reliable_delta;
     d.check_every = p.check_every;
     d.batch_graphs = p.device_graphs ? 1 : 0;
+    d.fgmres_inner_rel = p.fgmres_inner_rel;
     return d;
 }
 

@@ -25,7 +25,7 @@ class Params(C.Structure):
                 ("amg_strong", C.c_double), ("amg_smooth_pre", C.c_int), ("amg_smooth_post", C.c_int),
                 ("amg_omega", C.c_double), ("jacobi_omega", C.c_double), ("precision", C.c_int),
                 ("reliable_delta", C.c_double), ("check_every", C.c_int), ("batch_tile_rows", C.c_int),
-                ("batch_graphs", C.c_int), ("cg_engine", C.c_int)]
+                ("batch_graphs", C.c_int), ("cg_engine", C.c_int), ("fgmres_inner_rel", C.c_double)]
 
 
 class Stats(C.Structure):

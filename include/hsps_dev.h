@@ -73,6 +73,11 @@ typedef struct hsps_params {
     int cg_engine;         /* PCG + Jacobi (omega 1) on one system: 1 = run it as a one-system batch
                               (diagonal / stencil storage, fused kernels) when the matrix is
                               diagonal-stored or small (1); 0 = the plain kernel sequence */
+    double fgmres_inner_rel; /* mixed FGMRES: an FP32 cycle also ends once its residual estimate is
+                              below fgmres_inner_rel x the cycle's starting residual, and the FP64
+                              restart refines (the inner tolerance of GMRES-based iterative
+                              refinement; pays off for tolerances FP32 cannot reach in one cycle,
+                              e.g. 1e-5 for tol 1e-10); 0 = cycles run to tol (0) */
 } hsps_params;
 
 /* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64

@@ -260,3 +260,21 @@ def test_cg_engine_matches_plain_kernels(H, ctx, precision):
         if k == 3:  # large and CSR-stored: the plain kernels run either way
             assert res[0][0] == res[1][0]
             np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+def test_mixed_fgmres_inner_tolerance(H, ctx):
+    """GMRES-based refinement to 1e-10: an FP32 cycle that stops at a 1e-5
+    residual reduction (hsps_params.fgmres_inner_rel) and lets the FP64
+    restart refine needs no more iterations than cycles run to tol, and stays
+    close to the FP64 oracle's count."""
+    A = O.Csr.convdiff(256, 0.05, 0.6, 0.8)  # the FP64 oracle needs 26 iterations
+    b = A.spmv(O.uniform(9, A.rows))
+    xo, so = O.solve(A, b, "fgmres", "ilu", tol=1e-10, max_iter=2000, restart_size=30, drop_tol=0.0)
+    runs = {}
+    for inner in (0.0, 1e-5):
+        ok, st, x = _solve_dev(H, ctx, A, b, "fgmres", "ilu", tol=1e-10, max_iter=2000, restart_size=30,
+                               drop_tol=0.0, precision="mixed", fgmres_inner_rel=inner)
+        assert ok and _rel(A, b, x) < 1e-10 and st.true_residual < 1e-10, inner
+        runs[inner] = st.iterations
+    assert runs[1e-5] <= runs[0.0], runs
+    assert abs(runs[1e-5] - so.iterations) <= max(2, 0.15 * so.iterations), (runs, so.iterations)
