@@ -44,9 +44,10 @@ HBM_PEAK_GBS = 8000.0
 SPMV_FP64_HBM_BYTES = (154831 * 2.0 + 32768 * 1.0) * 1024
 # HBM bytes per row-iteration of the stencil-stored FP32 batched iteration
 # (k_b_k1 + k_b_k2), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
-# `bench.py --only-batched --tile-rows 2048` summed over every K1/K2 dispatch
-# of one solve, divided by its row-iterations (sum of rows x iterations).
-BATCHED_HBM_BYTES_PER_ROW_ITER = 39.0
+# `bench.py --only-batched --steps 1 --warmup 0 --tile-rows 2048`, summed over
+# every K1/K2 dispatch of the run and divided by its row-iterations (sum of
+# rows x iterations; profiles/batched512_pcg_jacobi_kernels.md).
+BATCHED_HBM_BYTES_PER_ROW_ITER = 35.7
 
 
 def log(*a):
