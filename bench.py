@@ -416,6 +416,60 @@ def bench_convdiff(H, ctx, args):
     }
 
 
+def bench_amg(H, ctx, args):
+    """Poisson 2048^2 (n = 4,194,304): FGMRES(30) + SA-AMG (theta 0.25, omega 0.67
+    as the reference, V(2,2)) to FP64 rel-res <= 1e-8, FP64 vs mixed; the
+    hierarchy setup (host aggregation / smoothed P / Galerkin, device upload)
+    is timed separately."""
+    side = args.grid
+    n = side * side
+    rp, ci, v = H.poisson_2d(side)
+    A = H.Csr(ctx, n, n, rp, ci, v)
+    xs = ctx.array(H.uniform_pm1(42, n))
+    b_dev = ctx.array(n=n)
+    A.spmv(xs, b_dev)
+
+    def run(precision, steps, warmup):
+        p = H.default_params(tol=1e-8, max_iter=2000, restart_size=30, precision=precision)
+        t0 = time.perf_counter()
+        M = H.Precond(A, "amg", p)
+        setup = time.perf_counter() - t0
+        xd = ctx.array(n=n)
+        times, stats = [], []
+        for k in range(warmup + steps):
+            H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
+            ctx.sync()
+            t0 = time.perf_counter()
+            ok, st = H.solve(A, b_dev, xd, "fgmres", M, p)
+            ctx.sync()
+            if k >= warmup:
+                times.append(time.perf_counter() - t0)
+                assert ok and st.true_residual <= 1e-8, (precision, st.true_residual)
+                stats.append(st)
+        levels = M.amg_levels()
+        prof = M.amg_setup_profile()
+        M.close()
+        return times, stats, (setup, prof), levels
+
+    t64, s64, su64, lv = run("fp64", max(1, args.steps), args.warmup)
+    tmx, smx, sumx, _ = run("mixed", args.steps, args.warmup)
+    value = len(tmx) / sum(tmx)
+    return {
+        "workload": f"poisson{side}_fgmres30_amg",
+        "value": value,
+        "ms_per_step": 1e3 * sum(tmx) / len(tmx),
+        "config": {"workload": f"2-D Poisson {side}x{side}, FGMRES(30) + SA-AMG (theta 0.25, omega 0.67, V(2,2)), "
+                               "FP64 rel-res<=1e-8", "n": n, "nnz": len(v), "amg_levels": lv},
+        "fp64_baseline": {"value": len(t64) / sum(t64), "unit": "solves/s", "iterations": s64[0].iterations,
+                          "setup_s": su64[0]},
+        "mixed": {"iterations": smx[0].iterations, "restarts": smx[0].restarts, "setup_s": sumx[0],
+                  "true_residual": smx[0].true_residual},
+        "amg_setup_stages_s": sumx[1],
+        "speedup_vs_fp64": value / (len(t64) / sum(t64)),
+        "cpu_baseline": None,
+    }
+
+
 def serve_specs(count, nmax, seed=77):
     """Request mix: n log-uniform in [1e3, nmax] (grid side rounded), one third
     each Poisson, anisotropic (eps log-uniform in [1e-3, 1]) and
@@ -502,7 +556,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "single", "convdiff", "serve"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "single", "amg", "convdiff", "serve"], default="batched")
     ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--workers", type=int, default=4, help="serve: concurrent solves")
@@ -541,6 +595,8 @@ def main():
         res = bench_serve(H, ctx, args)
     elif args.workload == "convdiff":
         res = bench_convdiff(H, ctx, args)
+    elif args.workload == "amg":
+        res = bench_amg(H, ctx, args)
     else:
         res = bench_single(H, ctx, ev, args)
     r64 = roof["fp64"]

@@ -121,6 +121,52 @@ static void validate_csr(int rows, int cols, int nnz, const int* rp, const int* 
         if (ci[k] < 0 || ci[k] >= cols) throw Error(HSPS_ERR_INVALID, "csr: column index out of range at entry " + std::to_string(k));
 }
 
+// Raw diagonal: the first entry with column == row, 0 when absent.
+__global__ void k_raw_diag(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                           double* __restrict__ d) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) {
+                a = v[k];
+                break;
+            }
+        d[i] = a;
+    }
+}
+
+std::unique_ptr<Csr> csr_adopt(Ctx* ctx, int rows, int cols, DevBuf&& rp, DevBuf&& ci, DevBuf&& v64,
+                               std::vector<int>&& h_rp, bool spmv_ready) {
+    auto A = std::make_unique<Csr>();
+    A->ctx = ctx;
+    A->rows = rows;
+    A->cols = cols;
+    A->nnz = h_rp.empty() ? 0 : h_rp[rows];
+    A->h_rp = std::move(h_rp);
+    A->rp = std::move(rp);
+    A->ci = std::move(ci);
+    A->v64 = std::move(v64);
+    if (spmv_ready) {
+        ctx->set_current();
+        hipStream_t s = ctx->stream;
+        A->v32 = DevBuf(ctx, (size_t)A->nnz * sizeof(float));
+        A->diag64 = DevBuf(ctx, (size_t)rows * sizeof(double));
+        if (A->nnz) {
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(A->nnz)), dim3(kBlock), 0, s, A->nnz, A->v64.as<double>(),
+                               A->v32.as<float>());
+            HSPS_CHECK_LAUNCH();
+        }
+        if (rows) {
+            hipLaunchKernelGGL(k_raw_diag, dim3(grid_for(rows, 1)), dim3(kBlock), 0, s, rows, A->rp.as<int>(),
+                               A->ci.as<int>(), A->v64.as<double>(), A->diag64.as<double>());
+            HSPS_CHECK_LAUNCH();
+        }
+        HSPS_HIP(hipStreamSynchronize(s));
+        A->build_schedule();
+    }
+    return A;
+}
+
 std::unique_ptr<Csr> make_csr(Ctx* ctx, int rows, int cols, std::vector<int> rp, std::vector<int> ci,
                               std::vector<double> v, bool want_f32) {
     validate_csr(rows, cols, (int)v.size(), rp.data(), ci.data());

@@ -41,6 +41,22 @@ std::vector<int> aggregate(const HostCsr& A, double theta, int* n_coarse) {
     return agg;
 }
 
+std::vector<int> aggregate_strong(int n, const std::vector<int>& srp, const std::vector<int>& sci, int* n_coarse) {
+    std::vector<int> agg(n, -1);
+    int nagg = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        agg[i] = nagg;
+        for (int k = srp[i]; k < srp[i + 1]; ++k) {
+            const int j = sci[k];
+            if (agg[j] < 0) agg[j] = nagg;
+        }
+        ++nagg;
+    }
+    if (n_coarse) *n_coarse = nagg;
+    return agg;
+}
+
 std::vector<double> inv_diagonal(const HostCsr& A) {
     std::vector<double> d(A.rows, 0.0);
 #pragma omp parallel for schedule(static)

@@ -21,6 +21,10 @@ struct HostCsr {
 // Greedy aggregation (amg.cpp:30-67): strength |a_ij| >= theta * max_{l!=i}|a_il|.
 std::vector<int> aggregate(const HostCsr& A, double theta, int* n_coarse);
 
+// The same greedy sweep over a precomputed strength pattern (srp/sci: the
+// strong neighbours j != i of each row, in row order).
+std::vector<int> aggregate_strong(int n, const std::vector<int>& srp, const std::vector<int>& sci, int* n_coarse);
+
 // Jacobi-smoothed tentative prolongation, one sweep, diagonal skipped,
 // |v| <= 1e-14 dropped (amg.cpp:73-178), built sparsely from the aggregate map.
 HostCsr smoothed_prolongation(const HostCsr& A, const std::vector<int>& agg, int n_coarse, double omega);

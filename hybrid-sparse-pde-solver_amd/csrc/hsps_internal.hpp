@@ -123,9 +123,14 @@ bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResul
 // kLdsNnz nonzeros per block; a longer row gets a block of its own.
 std::vector<int> make_row_blocks(const int* rp, int rows, int max_rows, int max_nnz);
 
-// Wrap a device CSR from host arrays (used by AMG levels / ILU factors).
+// Wrap a device CSR from host arrays (used by ILU factors).
 std::unique_ptr<Csr> make_csr(Ctx* ctx, int rows, int cols, std::vector<int> rp, std::vector<int> ci,
                               std::vector<double> v, bool want_f32 = true);
+// Take over a CSR built on the device (AMG levels): h_rp is the row pointer
+// already copied to the host; h_ci / h_v stay empty.  spmv_ready adds the
+// FP32 values, the diagonal and the row-block schedule.
+std::unique_ptr<Csr> csr_adopt(Ctx* ctx, int rows, int cols, DevBuf&& rp, DevBuf&& ci, DevBuf&& v64,
+                               std::vector<int>&& h_rp, bool spmv_ready);
 
 // ---- launchers implemented in the .hip files ---------------------------------
 // SpMV variants (spmv.hip)

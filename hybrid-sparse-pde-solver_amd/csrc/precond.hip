@@ -414,52 +414,43 @@ static void alloc_level_work(Ctx* c, AmgLevelDev& L, int n) {
     L.res32 = DevBuf(c, (size_t)n * 4);
 }
 
+// Host copy of a CSR: the arrays kept at creation, or a download for levels
+// built on the device.
+static hsps_host::HostCsr host_copy(const Csr& A) {
+    if (A.h_ci.size() == (size_t)A.nnz && A.h_v.size() == (size_t)A.nnz) return to_host(A);
+    hsps_host::HostCsr h;
+    h.rows = A.rows;
+    h.cols = A.cols;
+    h.rp = A.h_rp;
+    h.ci.resize(A.nnz);
+    h.v.resize(A.nnz);
+    A.ctx->set_current();
+    if (A.nnz) {
+        HSPS_HIP(hipMemcpyAsync(h.ci.data(), A.ci.p, (size_t)A.nnz * sizeof(int), hipMemcpyDeviceToHost, A.ctx->stream));
+        HSPS_HIP(hipMemcpyAsync(h.v.data(), A.v64.p, (size_t)A.nnz * sizeof(double), hipMemcpyDeviceToHost,
+                                A.ctx->stream));
+    }
+    HSPS_HIP(hipStreamSynchronize(A.ctx->stream));
+    return h;
+}
+
+// Hierarchy on the device (amg_setup.hip), then the coarsest level's dense
+// operator on the host (amg.cpp:260-291).
 static void setup_amg(Precond& M, const Csr& A, const hsps_params& p) {
-    const int coarse_limit = 50;  // amg.hpp:99
     M.amg_omega = p.amg_omega;
     M.pre = p.amg_smooth_pre;
     M.post = p.amg_smooth_post;
-    M.amg.clear();
-    hsps_host::HostCsr Af = to_host(A);
-    std::vector<hsps_host::HostCsr> hostA;
-    // level 0 (amg.cpp:189-196)
-    M.amg.emplace_back();
-    {
-        auto d = hsps_host::inv_diagonal(Af);
-        M.amg[0].dinv64 = upload(M.ctx, d.data(), d.size() * 8);
-        M.amg[0].dinv32 = upload_f32(M.ctx, d);
-    }
-    hostA.push_back(std::move(Af));
-    for (int lv = 0; lv < p.amg_levels - 1; ++lv) {  // amg.cpp:198-234
-        const hsps_host::HostCsr& F = hostA.back();
-        if (F.rows <= coarse_limit) break;
-        int nc = 0;
-        std::vector<int> agg = hsps_host::aggregate(F, p.amg_strong, &nc);
-        if (nc >= F.rows) break;
-        hsps_host::HostCsr P = hsps_host::smoothed_prolongation(F, agg, nc, M.amg_omega);
-        hsps_host::HostCsr R = hsps_host::transpose(P);
-        hsps_host::HostCsr Ac = hsps_host::spgemm(R, hsps_host::spgemm(F, P));
-        AmgLevelDev& cur = M.amg.back();
-        cur.agg = std::move(agg);
-        cur.P = to_dev(M.ctx, std::move(P));
-        cur.R = to_dev(M.ctx, std::move(R));
-        M.amg.emplace_back();
-        auto d = hsps_host::inv_diagonal(Ac);
-        M.amg.back().dinv64 = upload(M.ctx, d.data(), d.size() * 8);
-        M.amg.back().dinv32 = upload_f32(M.ctx, d);
-        hostA.push_back(std::move(Ac));
-    }
+    M.amg_times = AmgSetupTimes{};
+    amg_setup_device(M, A, p, &M.amg_times);
     const int L = (int)M.amg.size();
-    for (int lv = 1; lv < L; ++lv) {
-        hsps_host::HostCsr h = hostA[lv];
-        M.amg[lv].A = to_dev(M.ctx, std::move(h));
-    }
-    for (int lv = 0; lv < L; ++lv) alloc_level_work(M.ctx, M.amg[lv], hostA[lv].rows);
-    // coarsest-level dense operator (amg.cpp:260-291)
-    auto G = hsps_host::coarse_operator(hostA.back());
-    M.coarse_n = hostA.back().rows;
+    for (int lv = 0; lv < L; ++lv) alloc_level_work(M.ctx, M.amg[lv], lv == 0 ? A.rows : M.amg[lv].A->rows);
+    const auto t0 = std::chrono::steady_clock::now();
+    const hsps_host::HostCsr Hc = host_copy(L == 1 ? A : *M.amg[L - 1].A);
+    auto G = hsps_host::coarse_operator(Hc);
+    M.coarse_n = Hc.rows;
     M.coarse_inv64 = upload(M.ctx, G.data(), G.size() * 8);
     M.coarse_inv32 = upload_f32(M.ctx, G);
+    M.amg_times.coarse = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // ---- apply -------------------------------------------------------------------------
@@ -704,6 +695,14 @@ int hsps_amg_aggregates(hsps_precond M, int lv, int* out) {
     });
 }
 
+int hsps_amg_setup_profile(hsps_precond M, double* out) {
+    if (!M || M->kind != HSPS_PRECOND_AMG || !out) return HSPS_ERR_INVALID;
+    const AmgSetupTimes& t = M->amg_times;
+    const double v[6] = {t.strength, t.aggregate, t.prolongation, t.transpose, t.galerkin, t.coarse};
+    std::copy(v, v + 6, out);
+    return HSPS_OK;
+}
+
 int hsps_amg_level_matrix(hsps_precond M, int lv, int which, int* rp, int* ci, double* v) {
     return guarded([&] {
         if (!M || M->kind != HSPS_PRECOND_AMG || lv < 0 || lv >= (int)M->amg.size())
@@ -711,9 +710,15 @@ int hsps_amg_level_matrix(hsps_precond M, int lv, int which, int* rp, int* ci, d
         if (which != 0 && lv + 1 >= (int)M->amg.size())
             throw Error(HSPS_ERR_INVALID, "hsps_amg_level_matrix: coarsest level has no P/R");
         const Csr& A = which == 0 ? amg_A(*M, lv) : which == 1 ? *M->amg[lv].P : *M->amg[lv].R;
-        std::copy(A.h_rp.begin(), A.h_rp.end(), rp);
-        std::copy(A.h_ci.begin(), A.h_ci.end(), ci);
-        std::copy(A.h_v.begin(), A.h_v.end(), v);
+        if (!rp) throw Error(HSPS_ERR_INVALID, "hsps_amg_level_matrix: null row_ptr");
+        if (!ci || !v) {  // row pointer only (sizes the other arrays)
+            std::copy(A.h_rp.begin(), A.h_rp.end(), rp);
+            return HSPS_OK;
+        }
+        const hsps_host::HostCsr h = host_copy(A);
+        std::copy(h.rp.begin(), h.rp.end(), rp);
+        std::copy(h.ci.begin(), h.ci.end(), ci);
+        std::copy(h.v.begin(), h.v.end(), v);
         return HSPS_OK;
     });
 }

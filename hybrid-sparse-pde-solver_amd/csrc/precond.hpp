@@ -53,6 +53,11 @@ struct AmgLevelDev {
     DevBuf x32, x32b, rhs32, res32;
 };
 
+// Wall time of the AMG setup stages, seconds.
+struct AmgSetupTimes {
+    double strength = 0, aggregate = 0, prolongation = 0, transpose = 0, galerkin = 0, coarse = 0;
+};
+
 struct Precond {
     int kind = HSPS_PRECOND_NONE;
     Ctx* ctx = nullptr;
@@ -69,7 +74,11 @@ struct Precond {
     int pre = 2, post = 2;
     int coarse_n = 0;
     DevBuf coarse_inv64, coarse_inv32;  // dense inverse of the coarsest operator (row-major)
+    AmgSetupTimes amg_times;
 };
+
+// Builds M.amg (levels, P, R, D^-1) for A on the device (amg_setup.hip).
+void amg_setup_device(Precond& M, const Csr& A, const hsps_params& p, AmgSetupTimes* times);
 
 }  // namespace hsps_dev
 

@@ -86,6 +86,7 @@ SIGNATURES = {
     "hsps_amg_level_info": (_i, [_vp, _i, _ip, _ip]),
     "hsps_amg_aggregates": (_i, [_vp, _i, _ip]),
     "hsps_amg_level_matrix": (_i, [_vp, _i, _i, _ip, _ip, _dp]),
+    "hsps_amg_setup_profile": (_i, [_vp, _dp]),
     "hsps_cg_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_features": (_i, [_vp, _dp]),
@@ -276,6 +277,12 @@ class Precond:
         check(lib().hsps_amg_aggregates(self.h, lv, out.ctypes.data_as(_ip)))
         return out
 
+    def amg_setup_profile(self):
+        """Seconds per AMG setup stage."""
+        out = np.zeros(6, np.float64)
+        check(lib().hsps_amg_setup_profile(self.h, out.ctypes.data_as(_dp)))
+        return dict(zip(["strength", "aggregate", "prolongation", "transpose", "galerkin", "coarse"], out.tolist()))
+
     def amg_level_matrix(self, lv, which="A"):
         w = {"A": 0, "P": 1, "R": 2}[which]
         rows, nnz = self.amg_level_info(lv)
@@ -284,10 +291,10 @@ class Precond:
         else:
             nc, _ = self.amg_level_info(lv + 1)
             nr = rows if w == 1 else nc
-            # P: rows x nc ; R: nc x rows.  nnz(P) = nnz(R) is unknown here:
-            # size generously (P has at most rows*nc entries; use a loose bound).
-            nnz = rows * 32
         rp = np.zeros(nr + 1, np.int32)
+        # row pointer first (NULL column / value arrays): sizes the rest
+        check(lib().hsps_amg_level_matrix(self.h, lv, w, rp.ctypes.data_as(_ip), None, None))
+        nnz = int(rp[-1])
         ci = np.zeros(max(nnz, 1), np.int32)
         v = np.zeros(max(nnz, 1), np.float64)
         check(lib().hsps_amg_level_matrix(self.h, lv, w, rp.ctypes.data_as(_ip), ci.ctypes.data_as(_ip),

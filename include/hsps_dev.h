@@ -160,10 +160,16 @@ int hsps_precond_apply_f64(hsps_precond M, const double* d_r, double* d_z);
 int hsps_amg_levels(hsps_precond M);
 int hsps_amg_level_info(hsps_precond M, int lv, int* rows, int* nnz);
 int hsps_amg_aggregates(hsps_precond M, int lv, int* h_out);
-/* Copy level lv's operator (which=0 A, 1 P, 2 R) to host CSR arrays sized by
- * hsps_amg_level_info / the P,R shapes (rows(lv) x rows(lv+1)). */
+/* Copy level lv's operator (which=0 A, 1 P, 2 R) to host CSR arrays: the row
+ * pointer has rows+1 entries (A, P: rows(lv); R: rows(lv+1)); with
+ * h_col_idx or h_values NULL only the row pointer is written, whose last
+ * entry sizes the other two. */
 int hsps_amg_level_matrix(hsps_precond M, int lv, int which, int* h_row_ptr, int* h_col_idx,
                           double* h_values);
+/* Wall seconds of the AMG setup stages (no reference counterpart; the
+ * reference reports only the total): out[0..5] = strength pattern, host
+ * aggregation, smoothed P, R = P^T, Galerkin products, coarsest operator. */
+int hsps_amg_setup_profile(hsps_precond M, double* h_out6);
 
 /* ---- Krylov solves (SolverBase::solve, solver_base.hpp:32-35;
  *      CGSolver::solve cg_solver.cpp:37-191; FGMRESSolver::solve

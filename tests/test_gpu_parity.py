@@ -174,6 +174,90 @@ def test_amg_hierarchy_bit_exact(H, ctx, golden):
                 np.testing.assert_array_equal(M.amg_aggregates(lv), oa.aggregates(lv))
 
 
+def _arrow(side, stride=3, c=0.05):
+    """Poisson side^2 plus a hub row/column coupled to every stride-th node: a
+    P row and a Galerkin row reaching thousands of coarse columns, and an R
+    row (the hub's aggregate) with thousands of entries."""
+    rp, ci, v = O.Csr.poisson(side).arrays()
+    n = side * side
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    link = np.arange(0, n, stride)
+    r = np.concatenate([rows, np.full(len(link), n), link, [n]])
+    cc = np.concatenate([ci, link, np.full(len(link), n), [n]])
+    vv = np.concatenate([v, np.full(len(link), -c), np.full(len(link), -c), [1.0 + c * len(link)]])
+    return O.Csr.from_coo(n + 1, n + 1, r, cc, vv)
+
+
+def _hub_row(side, stride, diag=1e16):
+    """Poisson side^2 plus one row (no column) coupled to every stride-th node
+    with a huge diagonal: its smoothed-P entries fall under the 1e-14 drop, but
+    its row of A P and of the Galerkin product reach thousands of columns of a
+    coarse space wider than one LDS window."""
+    rp, ci, v = O.Csr.poisson(side).arrays()
+    n = side * side
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    link = np.arange(0, n, stride)
+    r = np.concatenate([rows, np.full(len(link), n), [n]])
+    cc = np.concatenate([ci, link, [n]])
+    vv = np.concatenate([v, np.full(len(link), -1.0), [diag]])
+    return O.Csr.from_coo(n + 1, n + 1, r, cc, vv)
+
+
+def _graph_laplacian(rng, n, max_deg):
+    """Random weighted graph Laplacian + I: rows of 2..max_deg entries."""
+    r, c, w = [], [], []
+    for i in range(n):
+        for j in rng.choice(n, size=int(rng.integers(1, max_deg // 2)), replace=False):
+            if j != i:
+                r.append(i); c.append(int(j)); w.append(float(rng.uniform(0.1, 1.0)))
+    r, c, w = np.array(r), np.array(c), np.array(w)
+    rr = np.concatenate([r, c])
+    cc = np.concatenate([c, r])
+    ww = np.concatenate([-w, -w])
+    deg = np.bincount(rr, weights=-ww, minlength=n)
+    rr = np.concatenate([rr, np.arange(n)])
+    cc = np.concatenate([cc, np.arange(n)])
+    ww = np.concatenate([ww, deg + 1.0])
+    return O.Csr.from_coo(n, n, rr, cc, ww)
+
+
+@pytest.mark.parametrize("case", ["poisson100", "poisson64_t05", "aniso64", "convdiff64", "graph4000", "arrow60", "hub130"])
+def test_amg_device_hierarchy_bitwise(H, ctx, case):
+    """Every level's A, P and R built on the device equal the oracle's
+    restatement of the reference's setup bit for bit (aggregates included)."""
+    theta = 0.5 if case.endswith("_t05") else 0.25
+    if case.startswith("poisson"):
+        A = O.Csr.poisson(int(case[7:].split("_")[0]))
+    elif case == "aniso64":
+        A = O.Csr.aniso(64, 0.01)
+    elif case == "convdiff64":
+        A = O.Csr.convdiff(64, 0.01, CV, SV)
+    elif case == "graph4000":
+        A = _graph_laplacian(np.random.default_rng(3), 4000, 160)
+    elif case == "arrow60":
+        A = _arrow(60, stride=4)
+    else:
+        A = _hub_row(130, 13)
+    M = H.Precond(dev_csr(H, ctx, A), "amg", H.default_params(amg_strong=theta))
+    oa = O.Amg(A, amg_strong=theta)
+    L = M.amg_levels()
+    assert L == oa.levels(), (case, L, oa.levels())
+    for lv in range(L):
+        whats = (["A"] if lv else []) + (["P", "R"] if lv + 1 < L else [])
+        for w in whats:
+            rp, ci, v = M.amg_level_matrix(lv, w)
+            orp, oci, ov = oa.matrix(lv, w).arrays()
+            msg = f"{case} level {lv} {w}"
+            np.testing.assert_array_equal(rp, orp, err_msg=msg)
+            np.testing.assert_array_equal(ci, oci, err_msg=msg)
+            np.testing.assert_array_equal(v.view(np.int64), np.ascontiguousarray(ov, np.float64).view(np.int64),
+                                          err_msg=msg)
+        if lv + 1 < L:
+            np.testing.assert_array_equal(M.amg_aggregates(lv), oa.aggregates(lv), err_msg=f"{case} agg {lv}")
+    prof = M.amg_setup_profile()
+    assert all(t >= 0 for t in prof.values()) and prof["galerkin"] > 0
+
+
 def test_amg_vcycle_matches_oracle(H, ctx):
     A = O.Csr.poisson(48)
     Ad = dev_csr(H, ctx, A)
