@@ -14,33 +14,6 @@ namespace hsps_host {
 
 static constexpr double kEps = std::numeric_limits<double>::epsilon();
 
-std::vector<int> aggregate(const HostCsr& A, double theta, int* n_coarse) {
-    const int n = A.rows;
-    std::vector<double> row_max(n, 0.0);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
-        double m = 0.0;
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (A.ci[k] != i) m = std::max(m, std::abs(A.v[k]));
-        row_max[i] = m;
-    }
-    // Sequential greedy sweep in row order: the decisions depend on it.
-    std::vector<int> agg(n, -1);
-    int nagg = 0;
-    for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0) continue;
-        agg[i] = nagg;
-        const double thr = theta * row_max[i];
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const int j = A.ci[k];
-            if (j != i && agg[j] < 0 && std::abs(A.v[k]) >= thr) agg[j] = nagg;
-        }
-        ++nagg;
-    }
-    if (n_coarse) *n_coarse = nagg;
-    return agg;
-}
-
 std::vector<int> aggregate_strong(int n, const std::vector<int>& srp, const std::vector<int>& sci, int* n_coarse) {
     std::vector<int> agg(n, -1);
     int nagg = 0;
@@ -55,133 +28,6 @@ std::vector<int> aggregate_strong(int n, const std::vector<int>& srp, const std:
     }
     if (n_coarse) *n_coarse = nagg;
     return agg;
-}
-
-std::vector<double> inv_diagonal(const HostCsr& A) {
-    std::vector<double> d(A.rows, 0.0);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < A.rows; ++i) {
-        double a = 0.0;
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (A.ci[k] == i) { a = A.v[k]; break; }
-        d[i] = std::abs(a) > kEps ? 1.0 / a : 0.0;
-    }
-    return d;
-}
-
-HostCsr smoothed_prolongation(const HostCsr& A, const std::vector<int>& agg, int nc, double omega) {
-    const int n = A.rows;
-    // omega / a_ii, 0 when |a_ii| <= eps (amg.cpp:120-127)
-    std::vector<double> w(n, 0.0);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i)
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (A.ci[k] == i) { w[i] = std::abs(A.v[k]) > kEps ? omega / A.v[k] : 0.0; break; }
-    // Row i touches at most (row nnz) distinct aggregates: a tiny sorted list.
-    std::vector<std::vector<std::pair<int, double>>> rows(n);
-#pragma omp parallel for schedule(dynamic, 256)
-    for (int i = 0; i < n; ++i) {
-        auto& row = rows[i];
-        row.emplace_back(agg[i], 1.0);
-        const double wi = w[i];
-        if (!(std::abs(wi) < kEps)) {
-            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-                const int j = A.ci[k];
-                if (j == i) continue;
-                const int c = agg[j];
-                const double t = wi * A.v[k];
-                size_t q = 0;
-                while (q < row.size() && row[q].first != c) ++q;
-                if (q == row.size()) row.emplace_back(c, 0.0);
-                row[q].second -= t;
-            }
-        }
-        std::sort(row.begin(), row.end());
-        row.erase(std::remove_if(row.begin(), row.end(), [](const auto& e) { return !(std::abs(e.second) > 1e-14); }),
-                  row.end());
-    }
-    HostCsr P;
-    P.rows = n;
-    P.cols = nc;
-    P.rp.assign(n + 1, 0);
-    for (int i = 0; i < n; ++i) P.rp[i + 1] = P.rp[i] + (int)rows[i].size();
-    P.ci.resize(P.rp[n]);
-    P.v.resize(P.rp[n]);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
-        int p = P.rp[i];
-        for (auto& e : rows[i]) { P.ci[p] = e.first; P.v[p] = e.second; ++p; }
-    }
-    return P;
-}
-
-HostCsr transpose(const HostCsr& A) {
-    HostCsr T;
-    T.rows = A.cols;
-    T.cols = A.rows;
-    T.rp.assign(A.cols + 1, 0);
-    for (int k = 0; k < A.nnz(); ++k) T.rp[A.ci[k] + 1]++;
-    for (int j = 0; j < A.cols; ++j) T.rp[j + 1] += T.rp[j];
-    T.ci.resize(A.nnz());
-    T.v.resize(A.nnz());
-    std::vector<int> pos(T.rp.begin(), T.rp.end() - 1);
-    for (int i = 0; i < A.rows; ++i)
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const int p = pos[A.ci[k]]++;
-            T.ci[p] = i;
-            T.v[p] = A.v[k];
-        }
-    return T;
-}
-
-HostCsr spgemm(const HostCsr& A, const HostCsr& B) {
-    if (A.cols != B.rows) throw std::runtime_error("mat_mat_product: inner dimensions mismatch");
-    const int m = A.rows, n = B.cols;
-    std::vector<int> cnt(m + 1, 0);
-    std::vector<std::vector<int>> cols(m);
-    std::vector<std::vector<double>> vals(m);
-#pragma omp parallel
-    {
-        std::vector<double> acc(n, 0.0);
-        std::vector<char> seen(n, 0);
-        std::vector<int> touched;
-#pragma omp for schedule(dynamic, 64)
-        for (int i = 0; i < m; ++i) {
-            for (int pa = A.rp[i]; pa < A.rp[i + 1]; ++pa) {
-                const int j = A.ci[pa];
-                const double av = A.v[pa];
-                for (int pb = B.rp[j]; pb < B.rp[j + 1]; ++pb) {
-                    const int l = B.ci[pb];
-                    if (!seen[l]) { seen[l] = 1; touched.push_back(l); }
-                    acc[l] = std::fma(av, B.v[pb], acc[l]);
-                }
-            }
-            std::sort(touched.begin(), touched.end());
-            auto& cc = cols[i];
-            auto& vv = vals[i];
-            cc.reserve(touched.size());
-            vv.reserve(touched.size());
-            for (int l : touched) {
-                if (std::abs(acc[l]) > kEps * kEps) { cc.push_back(l); vv.push_back(acc[l]); }
-                acc[l] = 0.0;
-                seen[l] = 0;
-            }
-            touched.clear();
-        }
-    }
-    HostCsr C;
-    C.rows = m;
-    C.cols = n;
-    C.rp.assign(m + 1, 0);
-    for (int i = 0; i < m; ++i) C.rp[i + 1] = C.rp[i] + (int)cols[i].size();
-    C.ci.resize(C.rp[m]);
-    C.v.resize(C.rp[m]);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < m; ++i) {
-        std::copy(cols[i].begin(), cols[i].end(), C.ci.begin() + C.rp[i]);
-        std::copy(vals[i].begin(), vals[i].end(), C.v.begin() + C.rp[i]);
-    }
-    return C;
 }
 
 HostCsr ilu0(const HostCsr& A) {

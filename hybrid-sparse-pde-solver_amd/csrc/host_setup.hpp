@@ -1,8 +1,8 @@
 // =============================================================================
 // host_setup.hpp — host-side setup stages of the solve path (compiled by the
-// host compiler with OpenMP): SA-AMG aggregation / prolongation / Galerkin,
-// ILU(0)/ILUT factorisation, triangular level schedules and the coarsest-
-// level dense operator.  Decisions (aggregates, ILU patterns, pivots) follow
+// host compiler with OpenMP): SA-AMG greedy aggregation, ILU(0)/ILUT
+// factorisation, triangular level schedules and the coarsest-level dense
+// operator.  Decisions (aggregates, ILU patterns, pivots) follow
 // the reference exactly; see each function's citation.
 // =============================================================================
 #pragma once
@@ -18,27 +18,11 @@ struct HostCsr {
     int nnz() const { return (int)v.size(); }
 };
 
-// Greedy aggregation (amg.cpp:30-67): strength |a_ij| >= theta * max_{l!=i}|a_il|.
-std::vector<int> aggregate(const HostCsr& A, double theta, int* n_coarse);
-
-// The same greedy sweep over a precomputed strength pattern (srp/sci: the
-// strong neighbours j != i of each row, in row order).
+// Greedy aggregation (amg.cpp:48-60) over a precomputed strength pattern
+// (srp/sci: the strong neighbours j != i of each row, |a_ij| >= theta *
+// max_{l!=i}|a_il|, in row order): each unassigned row in order seeds an
+// aggregate that its unassigned strong neighbours join.
 std::vector<int> aggregate_strong(int n, const std::vector<int>& srp, const std::vector<int>& sci, int* n_coarse);
-
-// Jacobi-smoothed tentative prolongation, one sweep, diagonal skipped,
-// |v| <= 1e-14 dropped (amg.cpp:73-178), built sparsely from the aggregate map.
-HostCsr smoothed_prolongation(const HostCsr& A, const std::vector<int>& agg, int n_coarse, double omega);
-
-// Transpose (sparse_matrix.cpp:138-165).
-HostCsr transpose(const HostCsr& A);
-
-// C = A*B, row-wise accumulation in (A-row, B-row) order with fused
-// multiply-add, ascending columns, |c| <= eps^2 dropped
-// (sparse_matrix.cpp:171-237).
-HostCsr spgemm(const HostCsr& A, const HostCsr& B);
-
-// 1/a_ii (0 where |a_ii| <= eps), amg.cpp:189-196 / :224-228.
-std::vector<double> inv_diagonal(const HostCsr& A);
 
 // ILU(0) on A's pattern (ilu.cpp:41-89) / ILUT(tau, p) (ilu.cpp:94-220).
 // Throws std::runtime_error on a near-zero pivot with the reference's message.
