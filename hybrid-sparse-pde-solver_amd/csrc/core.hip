@@ -342,6 +342,8 @@ void hsps_params_default(hsps_params* p) {
     p->batch_graphs = 1;
     p->cg_engine = 1;
     p->fgmres_inner_rel = 0.0;
+    p->sstep_k = 1;
+    p->sstep_cond_tol = 1e12;
 }
 
 int hsps_ctx_create(int device, hsps_ctx* out) {

@@ -161,6 +161,10 @@ void fill_zero_f64(int n, double* x, hipStream_t s, const int* skip = nullptr);
 void fill_zero_f32(int n, float* x, hipStream_t s, const int* skip = nullptr);
 int reduction_blocks(int n);
 
+// s-step CG and its Lanczos spectral interval (sstep.hip)
+void spectral_interval_dev(const Csr& A, int steps, double* lo, double* hi);
+int sstep_cg_dev(const Csr& A, const double* b, double* x, const hsps_params& p, hsps_stats* out);
+
 // Preconditioner object (precond.hip)
 struct Precond;
 void precond_apply_f64(const Precond& M, const double* r, double* z, hipStream_t s, const int* skip);

@@ -351,6 +351,71 @@ struct OpGivens {
     }
 };
 
+// ---- pipelined FGMRES: classical Gram-Schmidt with one re-orthogonalisation
+// (CGS2), each pass two launches whatever j is: the dots of w against all of
+// V_0..V_j at once (blockIdx.y = i), then w -= sum_i h_i V_i in index order
+// (the second pass also returns |w|^2 partials).  H(:, j) = h1 + h2.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_cgs_dots(int n, const T* __restrict__ w, const T* __restrict__ V, size_t ld, const int* skip, double* partials) {
+    if (*skip) return;
+    const T* vi = V + (size_t)blockIdx.y * ld;
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) red[0] += (double)w[i] * (double)vi[i];
+    block_partial<1>(red, partials + (size_t)blockIdx.y * gridDim.x);
+}
+
+// out[i * stride] = sum of partials[i * g .. i * g + g) in a fixed order, one
+// wave per i.
+__global__ void __launch_bounds__(kBlock)
+k_cgs_finish(int nv, int g, const double* __restrict__ partials, double* out, int stride, const int* skip) {
+    if (*skip) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = wv; i < nv; i += kBlock / 64) {
+        double a = 0.0;
+        for (int b = lane; b < g; b += 64) a += partials[(size_t)i * g + b];
+        a = wave_sum(a);
+        if (lane == 0) out[(size_t)i * stride] = a;
+    }
+}
+
+template <class T, bool NORM>
+__global__ void __launch_bounds__(kBlock)
+k_cgs_update(int n, T* __restrict__ w, const T* __restrict__ V, size_t ld, int nv, const double* h, int stride,
+             const int* skip, double* partials) {
+    if (*skip) return;
+    double red[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        double a = (double)w[i];
+        for (int c = 0; c < nv; ++c) a -= h[(size_t)c * stride] * (double)V[(size_t)c * ld + i];
+        const T wi = (T)a;
+        w[i] = wi;
+        if (NORM) red[0] += (double)wi * (double)wi;
+    }
+    if (NORM) block_partial<1>(red, partials);
+}
+
+// H(:, j) += h2; then as OpGivens, except that a breakdown (|w| <= eps) ends
+// the cycle before column j is rotated (pipelined_fgmres_solver.cpp:128-136).
+struct OpGivensPipe {
+    OpGivens base;
+    const double* h2;
+    __device__ bool skip() const { return base.skip(); }
+    __device__ void operator()(const double* tot) const {
+        const int j = base.j, m = base.st->m;
+        for (int i = 0; i <= j; ++i) base.H[i * m + j] += h2[i];
+        const double hn = sqrt(tot[0]);
+        if (!(hn > kEps)) {
+            base.H[(j + 1) * m + j] = hn;
+            base.st->inv_h = 0.0;
+            base.st->jdone = j + 1;
+            base.st->cycle_done = 1;
+            return;
+        }
+        base(tot);
+    }
+};
+
 template <class T>
 __global__ void k_gm_scale(int n, T* __restrict__ v, const GmState* st) {
     if (st->cycle_done) return;
@@ -636,9 +701,12 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     return finish(conv);
 }
 
+// pipe: the pipelined variant (pipelined_fgmres_solver.cpp:56-243): CGS2
+// orthogonalisation, breakdown ends the cycle, (its + 1) SpMVs charged per
+// cycle.  Otherwise FGMRES with modified Gram-Schmidt (fgmres_solver.cpp).
 template <class T>
 static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p,
-                      hsps_stats* out) {
+                      hsps_stats* out, bool pipe = false) {
     constexpr bool MIXED = std::is_same<T, float>::value;
     Ctx* ctx = A.ctx;
     hipStream_t s = ctx->stream;
@@ -659,6 +727,8 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     DevBuf csb(ctx, sizeof(double) * m), snb(ctx, sizeof(double) * m), gb(ctx, sizeof(double) * (m + 1)),
         yb(ctx, sizeof(double) * m);
     DevBuf scal(ctx, 64);
+    DevBuf pipe_part(ctx, pipe ? sizeof(double) * (size_t)(m + 1) * (size_t)std::max(g, 1) : 0);
+    DevBuf h2b(ctx, pipe ? sizeof(double) * (size_t)(m + 1) : 0);
     Pinned pin(sizeof(GmState) * 4 + 64);
     GmState* hst = static_cast<GmState*>(pin.p);
     GmState* d = dst.as<GmState>();
@@ -715,16 +785,36 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
             T* w = Vk(j + 1);
             if (MIXED) spmv_f32(A, (const float*)Zk(j), (float*)w, s, skip);
             else spmv_f64(A, (const double*)Zk(j), (double*)w, s, skip);
-            for (int i = 0; i <= j; ++i) {
-                const double* hprev = i > 0 ? H + (size_t)(i - 1) * m + j : nullptr;
-                const T* vprev = i > 0 ? Vk(i - 1) : nullptr;
-                hipLaunchKernelGGL(k_mgs<T>, dim3(g), dim3(kBlock), 0, s, n, w, vprev, hprev, (const T*)Vk(i), skip, part);
-                launch_finish<1>(part, g, OpStoreH{H + (size_t)i * m + j, skip}, s);
+            const OpGivens giv{j, H, csb.as<double>(), snb.as<double>(), gb.as<double>(), d};
+            if (pipe) {
+                const int nv = j + 1;
+                double* pp = pipe_part.as<double>();
+                double* h2 = h2b.as<double>();
+                hipLaunchKernelGGL(k_cgs_dots<T>, dim3(g, nv), dim3(kBlock), 0, s, n, (const T*)w, (const T*)Vp, ld, skip,
+                                   pp);
+                hipLaunchKernelGGL(k_cgs_finish, dim3(1), dim3(kBlock), 0, s, nv, g, (const double*)pp, H + j, m, skip);
+                hipLaunchKernelGGL((k_cgs_update<T, false>), dim3(g), dim3(kBlock), 0, s, n, w, (const T*)Vp, ld, nv,
+                                   (const double*)(H + j), m, skip, pp);
+                hipLaunchKernelGGL(k_cgs_dots<T>, dim3(g, nv), dim3(kBlock), 0, s, n, (const T*)w, (const T*)Vp, ld, skip,
+                                   pp);
+                hipLaunchKernelGGL(k_cgs_finish, dim3(1), dim3(kBlock), 0, s, nv, g, (const double*)pp, h2, 1, skip);
+                hipLaunchKernelGGL((k_cgs_update<T, true>), dim3(g), dim3(kBlock), 0, s, n, w, (const T*)Vp, ld, nv,
+                                   (const double*)h2, 1, skip, part);
+                launch_finish<1>(part, g, OpGivensPipe{giv, h2}, s);
                 HSPS_CHECK_LAUNCH();
+            } else {
+                for (int i = 0; i <= j; ++i) {
+                    const double* hprev = i > 0 ? H + (size_t)(i - 1) * m + j : nullptr;
+                    const T* vprev = i > 0 ? Vk(i - 1) : nullptr;
+                    hipLaunchKernelGGL(k_mgs<T>, dim3(g), dim3(kBlock), 0, s, n, w, vprev, hprev, (const T*)Vk(i), skip,
+                                       part);
+                    launch_finish<1>(part, g, OpStoreH{H + (size_t)i * m + j, skip}, s);
+                    HSPS_CHECK_LAUNCH();
+                }
+                hipLaunchKernelGGL(k_mgs_last<T>, dim3(g), dim3(kBlock), 0, s, n, (const double*)(H + (size_t)j * m + j),
+                                   w, (const T*)Vk(j), (const GmState*)d, part);
+                launch_finish<1>(part, g, giv, s);
             }
-            hipLaunchKernelGGL(k_mgs_last<T>, dim3(g), dim3(kBlock), 0, s, n, (const double*)(H + (size_t)j * m + j), w,
-                               (const T*)Vk(j), (const GmState*)d, part);
-            launch_finish<1>(part, g, OpGivens{j, H, csb.as<double>(), snb.as<double>(), gb.as<double>(), d}, s);
             hipLaunchKernelGGL(k_gm_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, w, (const GmState*)d);
             HSPS_CHECK_LAUNCH();
             if ((j + 1) % steps_per_poll == 0 && j + 1 < cm) {
@@ -768,8 +858,9 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, s);
     if (MIXED) conv = conv && st.true_residual < p.tol;
     const size_t vb = MIXED ? 4 : 8;
-    finish_stats(&st, (long long)total * (2LL * A.nnz + 4LL * n),
-                 (long long)total * (spmv_bytes(n, A.nnz) + 4LL * n * (long long)vb));
+    const long long spmvs = (long long)total + (pipe ? cycles : 0);  // pipelined_fgmres_solver.cpp:224-227
+    finish_stats(&st, spmvs * 2LL * A.nnz + (long long)total * 4LL * n,
+                 spmvs * spmv_bytes(n, A.nnz) + (long long)total * 4LL * n * (long long)vb);
     return finish(conv);
 }
 
@@ -812,6 +903,39 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const 
             }
         }
         return p.precision == HSPS_MIXED ? run_cg<float>(*A, M, b, x, p, st) : run_cg<double>(*A, M, b, x, p, st);
+    });
+}
+
+int hsps_pipelined_fgmres_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const hsps_params* pp,
+                                hsps_stats* st) {
+    return guarded([&] {
+        check_solve_args(A, M, b, x);
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        A->ctx->set_current();
+        return p.precision == HSPS_MIXED ? run_fgmres<float>(*A, M, b, x, p, st, true)
+                                         : run_fgmres<double>(*A, M, b, x, p, st, true);
+    });
+}
+
+int hsps_sstep_cg_solve(hsps_csr A, const double* b, double* x, const hsps_params* pp, hsps_stats* st) {
+    return guarded([&] {
+        check_solve_args(A, nullptr, b, x);
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        if (p.precision != HSPS_FP64) throw Error(HSPS_ERR_UNSUPPORTED, "s-step CG runs in FP64 only");
+        A->ctx->set_current();
+        return sstep_cg_dev(*A, b, x, p, st);
+    });
+}
+
+int hsps_spectral_interval(hsps_csr A, int steps, double* lo, double* hi) {
+    return guarded([&] {
+        if (!A || !lo || !hi) throw Error(HSPS_ERR_INVALID, "hsps_spectral_interval: null argument");
+        if (A->rows != A->cols) throw Error(HSPS_ERR_SIZE, "spectral_interval: matrix must be square");
+        A->ctx->set_current();
+        spectral_interval_dev(*A, steps, lo, hi);
+        return HSPS_OK;
     });
 }
 

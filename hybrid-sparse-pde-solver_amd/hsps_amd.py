@@ -25,7 +25,8 @@ class Params(C.Structure):
                 ("amg_strong", C.c_double), ("amg_smooth_pre", C.c_int), ("amg_smooth_post", C.c_int),
                 ("amg_omega", C.c_double), ("jacobi_omega", C.c_double), ("precision", C.c_int),
                 ("reliable_delta", C.c_double), ("check_every", C.c_int), ("batch_tile_rows", C.c_int),
-                ("batch_graphs", C.c_int), ("cg_engine", C.c_int), ("fgmres_inner_rel", C.c_double)]
+                ("batch_graphs", C.c_int), ("cg_engine", C.c_int), ("fgmres_inner_rel", C.c_double),
+                ("sstep_k", C.c_int), ("sstep_cond_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -33,7 +34,8 @@ class Stats(C.Structure):
                 ("reliable_updates", C.c_int), ("initial_residual", C.c_double),
                 ("final_residual", C.c_double), ("true_residual", C.c_double),
                 ("solve_time_s", C.c_double), ("setup_time_s", C.c_double),
-                ("flop_count", C.c_longlong), ("mem_bytes", C.c_longlong), ("energy_joules", C.c_double)]
+                ("flop_count", C.c_longlong), ("mem_bytes", C.c_longlong), ("energy_joules", C.c_double),
+                ("sstep_k", C.c_int), ("comm_events", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -89,6 +91,9 @@ SIGNATURES = {
     "hsps_amg_setup_profile": (_i, [_vp, _dp]),
     "hsps_cg_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
+    "hsps_pipelined_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
+    "hsps_sstep_cg_solve": (_i, [_vp, _vp, _vp, _PP, _SP]),
+    "hsps_spectral_interval": (_i, [_vp, _i, _dp, _dp]),
     "hsps_features": (_i, [_vp, _dp]),
     "hsps_batch_create": (_i, [_vp, _i, _ip, _ip, _ip, _dp, C.POINTER(_vp)]),
     "hsps_batch_destroy": (_i, [_vp]),
@@ -372,12 +377,23 @@ class Batch:
 
 
 def solve(A, b, x, solver="cg", M=None, params=None):
-    """Device solve through hsps_cg_solve / hsps_fgmres_solve; b, x DeviceArrays."""
+    """Device solve (solver: cg, fgmres, pipelined_fgmres, sstep_cg); b, x
+    DeviceArrays.  s-step CG takes no preconditioner."""
     p = params or default_params()
     st = Stats()
-    f = lib().hsps_cg_solve if solver == "cg" else lib().hsps_fgmres_solve
+    if solver == "sstep_cg":
+        rc = check(lib().hsps_sstep_cg_solve(A.h, b.ptr, x.ptr, C.byref(p), C.byref(st)))
+        return bool(rc), st
+    f = {"cg": lib().hsps_cg_solve, "fgmres": lib().hsps_fgmres_solve,
+         "pipelined_fgmres": lib().hsps_pipelined_fgmres_solve}[solver]
     rc = check(f(A.h, M.h if M is not None else None, b.ptr, x.ptr, C.byref(p), C.byref(st)))
     return bool(rc), st
+
+
+def spectral_interval(A, steps=20):
+    lo, hi = C.c_double(), C.c_double()
+    check(lib().hsps_spectral_interval(A.h, steps, C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
 
 
 # ---- workload generators (host, numpy) -----------------------------------------

@@ -78,6 +78,8 @@ typedef struct hsps_params {
                               restart refines (the inner tolerance of GMRES-based iterative
                               refinement; pays off for tolerances FP32 cannot reach in one cycle,
                               e.g. 1e-5 for tol 1e-10); 0 = cycles run to tol (0) */
+    int sstep_k;           /* s-step CG batch size k (1, types.hpp:134) */
+    double sstep_cond_tol; /* s-step CG: halve k when the Gram diagonal ratio exceeds this (1e12) */
 } hsps_params;
 
 /* SolverStats, types.hpp:91-114 (fields on the solve path) plus the FP64
@@ -95,6 +97,8 @@ typedef struct hsps_stats {
     long long flop_count;  /* flop_model, energy_monitor.hpp:163-177 */
     long long mem_bytes;
     double energy_joules;  /* 2e-10 J/FLOP + 5e-9 J/B proxy (cg_solver.cpp:186-188) */
+    int sstep_k;           /* s-step CG: the batch size used (types.hpp:110) */
+    int comm_events;       /* s-step CG: global reductions, one per batch (types.hpp:112) */
 } hsps_stats;
 
 /* ---- errors / defaults ------------------------------------------------- */
@@ -180,6 +184,22 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x, co
                   hsps_stats* stats);
 int hsps_fgmres_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x,
                       const hsps_params* p, hsps_stats* stats);
+/* PipelinedFGMRESSolver::solve (pipelined_fgmres_solver.cpp:165-243): the
+ * same Krylov space as FGMRES; here each iteration orthogonalises with two
+ * fused classical Gram-Schmidt passes (a fixed number of launches and
+ * reductions whatever the iteration index), a breakdown ends the cycle, and
+ * (its + 1) SpMVs per cycle are charged to flop_count / mem_bytes. */
+int hsps_pipelined_fgmres_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x,
+                                const hsps_params* p, hsps_stats* stats);
+/* SStepCGSolver::solve (sstep_cg_solver.cpp:168-305) with k = p->sstep_k:
+ * per batch a normalised Newton-Chebyshev basis, its Gram matrix and B^T r in
+ * one fused pass, a host Cholesky solve (steepest-descent step when it
+ * fails), x += B y and the exact residual.  FP64; the reference applies no
+ * preconditioner, so none is taken. */
+int hsps_sstep_cg_solve(hsps_csr A, const double* d_b, double* d_x, const hsps_params* p, hsps_stats* stats);
+/* SparseMatrix::spectral_interval (sparse_matrix.cpp:451-479): Lanczos from
+ * q = 1/sqrt(n) for min(n, steps, 64) steps, Gershgorin bounds of T. */
+int hsps_spectral_interval(hsps_csr A, int steps, double* lo, double* hi);
 
 /* ---- matrix features for the adaptive ladder
  *      (AdaptiveSelector::extract_features, adaptive_selector.cpp:38-220).

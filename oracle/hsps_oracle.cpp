@@ -731,6 +731,276 @@ bool fgmres_solve(const Csr& A, const Real* b, Real* x, const Precond* M, const 
     return st.converged;
 }
 
+// ---- spectral_interval: sparse_matrix.cpp:451-479 ---------------------------------
+// K = min(n, steps) Lanczos steps from q = 1/sqrt(n); Gershgorin bounds of the
+// tridiagonal T.
+void spectral_interval(const Csr& A, int steps, Real* lo, Real* hi) {
+    const Index n = A.rows;
+    const int K = std::min<Index>(n, steps);
+    std::vector<Real> q(n, 1.0 / std::sqrt((Real)n)), qp(n, 0.0), Aq(n);
+    std::vector<Real> al, be;
+    Real bj = 0.0;
+    for (int j = 0; j < K; ++j) {
+        spmv(A, q.data(), Aq.data());
+        const Real aj = dot(n, q.data(), Aq.data());
+        al.push_back(aj);
+        for (Index i = 0; i < n; ++i) Aq[i] -= aj * q[i] + bj * qp[i];
+        bj = norm2(n, Aq.data());
+        if (bj < EPS * 1e6) break;
+        be.push_back(bj);
+        qp = q;
+        q = Aq;
+        scale(n, 1.0 / bj, q.data());
+    }
+    const int sz = (int)al.size();
+    Real a = sz ? al[0] : 0.0, b = a;
+    for (int i = 0; i < sz; ++i) {
+        Real r = 0.0;
+        if (i > 0) r += std::abs(be[i - 1]);
+        if (i < sz - 1) r += std::abs(be[i]);
+        a = std::min(a, al[i] - r);
+        b = std::max(b, al[i] + r);
+    }
+    *lo = a;
+    *hi = b;
+}
+
+// ---- s-step CG: sstep_cg_solver.cpp ----------------------------------------------------
+// Each batch: Newton basis B_0 = r/|r|, B_j = (A - theta_{j-1}) B_{j-1}
+// normalised (Chebyshev shifts on the Lanczos interval, :24-64); Gram G =
+// B^T B (:69-78); if the diagonal ratio of G exceeds sstep_cond_tol, k is
+// halved and the basis rebuilt (:217-227); G y = B^T r by Cholesky
+// (:120-165), x += B y, r = b - A x exactly (:246-258).  A failed Cholesky
+// takes one steepest-descent step with r instead (:232-243).
+static std::vector<Real> cheb_shifts(Real lo, Real hi, int k) {  // :24-35
+    std::vector<Real> s(k);
+    const Real c = 0.5 * (lo + hi), h = 0.5 * (hi - lo);
+    for (int j = 0; j < k; ++j) s[j] = c + h * std::cos(M_PI * (2.0 * j + 1.0) / (2.0 * k));
+    return s;
+}
+
+static void sstep_basis(const Csr& A, const Real* r, const std::vector<Real>& sh,
+                        std::vector<std::vector<Real>>& B) {  // :42-64
+    const Index n = A.rows;
+    const int k = (int)sh.size();
+    B.assign(k, std::vector<Real>(n, 0.0));
+    std::copy(r, r + n, B[0].begin());
+    const Real n0 = norm2(n, B[0].data());
+    if (n0 > EPS) scale(n, 1.0 / n0, B[0].data());
+    std::vector<Real> tmp(n);
+    for (int j = 1; j < k; ++j) {
+        spmv(A, B[j - 1].data(), tmp.data());
+        B[j] = tmp;
+        axpy(n, -sh[j - 1], B[j - 1].data(), B[j].data());
+        const Real nj = norm2(n, B[j].data());
+        if (nj > EPS) scale(n, 1.0 / nj, B[j].data());
+    }
+}
+
+static void sstep_gram(const std::vector<std::vector<Real>>& B, std::vector<std::vector<Real>>& G) {  // :69-78
+    const int k = (int)B.size();
+    const Index n = k ? (Index)B[0].size() : 0;
+    G.assign(k, std::vector<Real>(k, 0.0));
+    for (int i = 0; i < k; ++i)
+        for (int j = i; j < k; ++j) G[i][j] = G[j][i] = dot(n, B[i].data(), B[j].data());
+}
+
+static double sstep_cond(const std::vector<std::vector<Real>>& G) {  // :103-114
+    Real mx = 0.0, mn = std::numeric_limits<Real>::infinity();
+    for (size_t i = 0; i < G.size(); ++i) {
+        mx = std::max(mx, std::abs(G[i][i]));
+        if (std::abs(G[i][i]) > EPS) mn = std::min(mn, std::abs(G[i][i]));
+    }
+    return mn >= mx ? 1.0 : mx / mn;
+}
+
+static bool sstep_cholesky(const std::vector<std::vector<Real>>& G, const std::vector<Real>& rhs,
+                           std::vector<Real>& y) {  // :120-165
+    const int k = (int)G.size();
+    std::vector<std::vector<Real>> L(k, std::vector<Real>(k, 0.0));
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j <= i; ++j) {
+            Real s = G[i][j];
+            for (int l = 0; l < j; ++l) s -= L[i][l] * L[j][l];
+            if (i == j) {
+                if (s <= 0.0) return false;
+                L[i][j] = std::sqrt(s);
+            } else {
+                L[i][j] = s / L[j][j];
+            }
+        }
+    std::vector<Real> z(k, 0.0);
+    for (int i = 0; i < k; ++i) {
+        Real s = rhs[i];
+        for (int j = 0; j < i; ++j) s -= L[i][j] * z[j];
+        z[i] = s / L[i][i];
+    }
+    y.assign(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+        Real s = z[i];
+        for (int j = i + 1; j < k; ++j) s -= L[j][i] * y[j];
+        y[i] = s / L[i][i];
+    }
+    return true;
+}
+
+bool sstep_cg_solve(const Csr& A, const Real* b, Real* x, const Params& prm, Stats& st) {  // :168-305
+    const double t0 = now_s();
+    const Index n = A.rows;
+    const long long nz = A.nnz();
+    const int k = std::min(prm.sstep_k, std::max(1, n / 10));
+    Real lo_raw, hi_raw;
+    spectral_interval(A, 20, &lo_raw, &hi_raw);
+    const Real lo = std::max(lo_raw, EPS);
+    const Real hi = std::max(hi_raw, lo * 2.0);
+    std::vector<Real> r(n);
+    spmv(A, x, r.data());
+    for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    const Real bn = norm2(n, b);
+    Real rn = norm2(n, r.data());
+    st.initial_residual = rn;
+    if (bn < EPS) { st.converged = true; st.final_residual = 0.0; return true; }
+    Real rel = rn / bn;
+    if (rel < prm.tol) { st.converged = true; st.final_residual = rel; return true; }
+    long long F = 0, Bb = 0;
+    int total = 0;
+    std::vector<std::vector<Real>> B, G;
+    while (total < prm.max_iter && rel > prm.tol) {
+        int kb = k;
+        sstep_basis(A, r.data(), cheb_shifts(lo, hi, kb), B);
+        F += (long long)kb * spmv_flops(nz);
+        Bb += (long long)kb * spmv_bytes(n, nz);
+        sstep_gram(B, G);
+        if (sstep_cond(G) > prm.sstep_cond_tol && kb > 1) {
+            kb = std::max(1, kb / 2);
+            sstep_basis(A, r.data(), cheb_shifts(lo, hi, kb), B);
+            sstep_gram(B, G);
+        }
+        std::vector<Real> rhs(kb), y;
+        for (int i = 0; i < kb; ++i) rhs[i] = dot(n, r.data(), B[i].data());
+        F += (long long)kb * 2LL * n;
+        if (!sstep_cholesky(G, rhs, y)) {
+            std::vector<Real> Ap(n);
+            spmv(A, r.data(), Ap.data());
+            const Real rr = dot(n, r.data(), r.data()), rAp = dot(n, r.data(), Ap.data());
+            if (std::abs(rAp) > EPS) {
+                const Real alpha = rr / rAp;
+                axpy(n, alpha, r.data(), x);
+                axpy(n, -alpha, Ap.data(), r.data());
+            }
+            ++total;
+            rel = norm2(n, r.data()) / bn;
+            continue;
+        }
+        for (int i = 0; i < kb; ++i) axpy(n, y[i], B[i].data(), x);
+        spmv(A, x, r.data());
+        F += spmv_flops(nz);
+        Bb += spmv_bytes(n, nz);
+        for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
+        rn = norm2(n, r.data());
+        rel = rn / bn;
+        total += kb;
+    }
+    st.iterations = total;
+    st.final_residual = rel;
+    st.converged = rel < prm.tol;
+    st.solve_time_s = now_s() - t0;
+    st.flop_count = F;
+    st.mem_bytes = Bb;
+    st.energy_joules = 2e-10 * F + 5e-9 * Bb;
+    return st.converged;
+}
+
+// ---- pipelined FGMRES: pipelined_fgmres_solver.cpp:56-243 ---------------------------
+// The FGMRES cycle with the next direction's preconditioner apply and SpMV
+// issued right after it is normalised (the same values), a lucky breakdown
+// ending the cycle before column j is rotated (:165-168), convergence on the
+// Givens residual, and (its + 1) SpMVs charged per cycle (:224-227).
+static int pipelined_cycle(const Csr& A, const Real* b, Real* x, const Precond* M, Real tol_abs, int m, Real& res) {
+    const Index n = A.rows;
+    std::vector<Real> r(n);
+    spmv(A, x, r.data());
+    for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    const Real beta = norm2(n, r.data());
+    res = beta;
+    if (beta < tol_abs) return 0;
+    std::vector<std::vector<Real>> V(m + 1, std::vector<Real>(n, 0.0)), Z(m, std::vector<Real>(n, 0.0)),
+        S(m, std::vector<Real>(n, 0.0));
+    V[0] = r;
+    scale(n, 1.0 / beta, V[0].data());
+    if (M) M->apply(V[0].data(), Z[0].data()); else Z[0] = V[0];
+    spmv(A, Z[0].data(), S[0].data());
+    std::vector<std::vector<Real>> H(m + 1, std::vector<Real>(m, 0.0));
+    std::vector<Real> cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0);
+    g[0] = beta;
+    int j = 0;
+    for (; j < m; ++j) {
+        std::vector<Real>& w = S[j];
+        for (int i = 0; i <= j; ++i) {
+            H[i][j] = dot(n, w.data(), V[i].data());
+            axpy(n, -H[i][j], V[i].data(), w.data());
+        }
+        H[j + 1][j] = norm2(n, w.data());
+        if (H[j + 1][j] > EPS) {
+            V[j + 1] = w;
+            scale(n, 1.0 / H[j + 1][j], V[j + 1].data());
+        } else {
+            ++j;
+            break;
+        }
+        if (j + 1 < m) {
+            if (M) M->apply(V[j + 1].data(), Z[j + 1].data()); else Z[j + 1] = V[j + 1];
+            spmv(A, Z[j + 1].data(), S[j + 1].data());
+        }
+        for (int i = 0; i < j; ++i) rot(H[i][j], H[i + 1][j], cs[i], sn[i]);
+        Real rv;
+        givens(H[j][j], H[j + 1][j], cs[j], sn[j], rv);
+        H[j][j] = rv;
+        H[j + 1][j] = 0.0;
+        rot(g[j], g[j + 1], cs[j], sn[j]);
+        res = std::abs(g[j + 1]);
+        if (res < tol_abs) { ++j; break; }
+    }
+    std::vector<Real> y(j, 0.0);
+    for (int i = j - 1; i >= 0; --i) {
+        Real s = g[i];
+        for (int q = i + 1; q < j; ++q) s -= H[i][q] * y[q];
+        y[i] = std::abs(H[i][i]) > EPS ? s / H[i][i] : 0.0;
+    }
+    for (int i = 0; i < j; ++i) axpy(n, y[i], Z[i].data(), x);
+    return j;
+}
+
+bool pipelined_fgmres_solve(const Csr& A, const Real* b, Real* x, const Precond* M, const Params& prm, Stats& st) {
+    const double t0 = now_s();
+    const Index n = A.rows;
+    const long long nz = A.nnz();
+    const int m = std::min(prm.restart_size, n);
+    const Real bn = norm2(n, b);
+    if (bn < EPS) { st.converged = true; st.final_residual = 0.0; return true; }
+    const Real tol_abs = prm.tol * bn;
+    long long F = 0, B = 0;
+    Real res = bn;
+    int total = 0, restarts = 0;
+    while (total < prm.max_iter) {
+        const int cm = std::min(m, prm.max_iter - total);
+        const int its = pipelined_cycle(A, b, x, M, tol_abs, cm, res);
+        F += (long long)(its + 1) * spmv_flops(nz) + (long long)its * 4LL * n;
+        B += (long long)(its + 1) * spmv_bytes(n, nz) + (long long)its * 4LL * n * 8;
+        total += its;
+        ++restarts;
+        const Real rel = res / bn;
+        if (rel < prm.tol || its == 0) { st.converged = rel < prm.tol; st.final_residual = rel; break; }
+    }
+    st.iterations = total;
+    st.restarts = restarts - 1;
+    st.solve_time_s = now_s() - t0;
+    st.flop_count = F;
+    st.mem_bytes = B;
+    st.energy_joules = 2.0e-10 * F + 5.0e-9 * B;
+    return st.converged;
+}
+
 // ---- factory: solver_factory.cpp:55-133 ----------------------------------------------
 std::unique_ptr<Precond> make_precond(int kind, const Csr& A, const Params& p) {
     switch (kind) {

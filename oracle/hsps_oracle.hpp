@@ -77,6 +77,8 @@ struct Params {
     int amg_smooth_pre = 2;
     int amg_smooth_post = 2;
     int fill_per_row = 0;
+    int sstep_k = 1;             // types.hpp:134
+    Real sstep_cond_tol = 1e12;  // types.hpp:135
 };
 struct Stats {
     int iterations = 0;
@@ -145,6 +147,13 @@ void dense_gepp_solve(const Csr& A, const Real* rhs, Real* x);
 // ---- Krylov solvers ----------------------------------------------------------
 bool cg_solve(const Csr& A, const Real* b, Real* x, const Precond* M,
               const Params& p, Stats& st);
+// s-step CG (sstep_cg_solver.cpp:168-305; the preconditioner is not used) and
+// pipelined FGMRES (pipelined_fgmres_solver.cpp:56-243).
+bool sstep_cg_solve(const Csr& A, const Real* b, Real* x, const Params& prm, Stats& st);
+bool pipelined_fgmres_solve(const Csr& A, const Real* b, Real* x, const Precond* M, const Params& prm, Stats& st);
+// Lanczos estimate of [lambda_min, lambda_max] (sparse_matrix.cpp:451-479).
+void spectral_interval(const Csr& A, int steps, Real* lo, Real* hi);
+
 bool fgmres_solve(const Csr& A, const Real* b, Real* x, const Precond* M,
                   const Params& p, Stats& st);
 

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("restart_size", C.c_int),
                 ("drop_tol", C.c_double), ("amg_levels", C.c_int), ("amg_strong", C.c_double),
-                ("amg_smooth_pre", C.c_int), ("amg_smooth_post", C.c_int)]
+                ("amg_smooth_pre", C.c_int), ("amg_smooth_post", C.c_int), ("sstep_k", C.c_int),
+                ("sstep_cond_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -28,8 +29,8 @@ class Stats(C.Structure):
 
 
 def params(tol=1e-8, max_iter=2000, restart_size=50, drop_tol=1e-4, amg_levels=10,
-           amg_strong=0.25, pre=2, post=2):
-    return Params(tol, max_iter, restart_size, drop_tol, amg_levels, amg_strong, pre, post)
+           amg_strong=0.25, pre=2, post=2, sstep_k=1, sstep_cond_tol=1e12):
+    return Params(tol, max_iter, restart_size, drop_tol, amg_levels, amg_strong, pre, post, sstep_k, sstep_cond_tol)
 
 
 _lib = None
@@ -53,6 +54,7 @@ def lib():
             "orc_csr_export": (None, [vp, ip, ip, dp]),
             "orc_spmv": (None, [vp, dp, dp]),
             "orc_uniform": (None, [C.c_ulonglong, C.c_int, dp]),
+            "orc_spectral_interval": (None, [vp, C.c_int, dp, dp]),
             "orc_solve": (C.c_int, [vp, dp, dp, C.c_int, C.c_int, C.POINTER(Params), C.POINTER(Stats)]),
             "orc_precond_apply": (C.c_int, [vp, C.c_int, C.POINTER(Params), dp, dp]),
             "orc_adaptive": (C.c_int, [vp, dp, dp, C.POINTER(Params), C.c_int, C.POINTER(Stats)]),
@@ -161,7 +163,13 @@ def uniform(seed, n):
     return out
 
 
-SOLVERS = {"cg": 0, "fgmres": 1}
+SOLVERS = {"cg": 0, "fgmres": 1, "sstep_cg": 2, "pipelined_fgmres": 3}
+
+
+def spectral_interval(A, steps=20):
+    lo, hi = C.c_double(), C.c_double()
+    lib().orc_spectral_interval(A.h, steps, C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
 PRECONDS = {"none": 0, "jacobi": 1, "ilu": 2, "amg": 3}
 
 

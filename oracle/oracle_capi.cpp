@@ -5,6 +5,7 @@
 // =============================================================================
 #include <algorithm>
 #include <cstring>
+#include <stdexcept>
 #include <exception>
 #include <string>
 
@@ -27,6 +28,8 @@ struct orc_params {
     double amg_strong;
     int amg_smooth_pre;
     int amg_smooth_post;
+    int sstep_k;
+    double sstep_cond_tol;
 };
 
 struct orc_stats {
@@ -55,6 +58,8 @@ static Params to_params(const orc_params* p) {
     q.amg_strong = p->amg_strong;
     q.amg_smooth_pre = p->amg_smooth_pre;
     q.amg_smooth_post = p->amg_smooth_post;
+    q.sstep_k = p->sstep_k;
+    q.sstep_cond_tol = p->sstep_cond_tol;
     return q;
 }
 
@@ -117,9 +122,13 @@ void orc_csr_export(void* h, int* rp, int* ci, double* v) {
 }
 void orc_spmv(void* h, const double* x, double* y) { spmv(*static_cast<Csr*>(h), x, y); }
 void orc_uniform(unsigned long long seed, int n, double* out) { uniform_pm1(seed, n, out); }
+void orc_spectral_interval(void* h, int steps, double* lo, double* hi) {
+    spectral_interval(*static_cast<Csr*>(h), steps, lo, hi);
+}
 
-// solver: 0 CG, 1 FGMRES.  precond: 0 none, 1 Jacobi, 2 ILU (params.drop_tol;
-// 0 gives ILU(0)), 3 AMG.
+// solver: 0 CG, 1 FGMRES, 2 s-step CG (no preconditioner), 3 pipelined
+// FGMRES.  precond: 0 none, 1 Jacobi, 2 ILU (params.drop_tol; 0 gives
+// ILU(0)), 3 AMG.
 int orc_solve(void* h, const double* b, double* x, int solver, int precond, const orc_params* p,
               orc_stats* out) {
     ORC_TRY({
@@ -127,7 +136,14 @@ int orc_solve(void* h, const double* b, double* x, int solver, int precond, cons
         Params q = to_params(p);
         Stats st;
         auto M = make_precond(precond, A, q);
-        const bool ok = solver == 0 ? cg_solve(A, b, x, M.get(), q, st) : fgmres_solve(A, b, x, M.get(), q, st);
+        bool ok;
+        switch (solver) {
+            case 0: ok = cg_solve(A, b, x, M.get(), q, st); break;
+            case 1: ok = fgmres_solve(A, b, x, M.get(), q, st); break;
+            case 2: ok = sstep_cg_solve(A, b, x, q, st); break;
+            case 3: ok = pipelined_fgmres_solve(A, b, x, M.get(), q, st); break;
+            default: throw std::invalid_argument("orc_solve: unknown solver");
+        }
         from_stats(st, out);
         return ok ? 1 : 0;
     });

@@ -1,0 +1,152 @@
+"""GPU parity of the pipelined FGMRES, s-step CG and Lanczos spectral interval
+(through the C ABI) against the oracle.
+
+Tolerances
+  spectral interval: bounds within 1e-9 of the oracle's (scale: max |bound|)
+  solves: iteration counts within 5% (FP64; 10% or 2 for mixed), solution
+          within 1e-6 relative of the oracle's, FP64 true rel-res <= tol.
+  s-step CG with k >= 4: the reference's iteration count there depends on
+          the summation order (the oracle itself gives 5757..9838 iterations
+          for k = 4 on 1..8 OpenMP threads: whether a near-singular Gram
+          matrix passes Cholesky flips with rounding), so the bar is the
+          oracle's own range over thread counts, widened by 10%.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_gpu_parity import dev_csr, _solve_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _diag(d):
+    n = len(d)
+    return O.Csr.from_coo(n, n, list(range(n)), list(range(n)), list(d))
+
+
+def _scaled_poisson(side, f=0.25):
+    rp, ci, v = O.Csr.poisson(side).arrays()
+    n = side * side
+    return O.Csr.from_csr(n, n, rp, ci, v * f)
+
+
+@pytest.mark.parametrize("case", ["poisson64", "aniso48", "convdiff40", "diag6", "one"])
+def test_spectral_interval_matches_oracle(H, ctx, case):
+    A = {"poisson64": lambda: O.Csr.poisson(64), "aniso48": lambda: O.Csr.aniso(48, 0.05),
+         "convdiff40": lambda: O.Csr.convdiff(40, 0.05, 0.6, 0.8),
+         "diag6": lambda: _diag([0.5, 1.0, 2.0, 3.0, 7.5, 9.0]), "one": lambda: _diag([3.5])}[case]()
+    lo, hi = H.spectral_interval(dev_csr(H, ctx, A), 20)
+    olo, ohi = O.spectral_interval(A, 20)
+    scale = max(abs(olo), abs(ohi))
+    assert abs(lo - olo) <= 1e-9 * scale and abs(hi - ohi) <= 1e-9 * scale, (lo, hi, olo, ohi)
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 8])
+def test_sstep_cg_matches_oracle(H, ctx, k):
+    A = _scaled_poisson(24)
+    n, _, nz = A.shape_nnz()
+    b = O.uniform(5, n)
+    kw = dict(sstep_k=k, tol=1e-8, max_iter=20000)
+    ok, st, x = _solve_dev(H, ctx, A, b, "sstep_cg", "none", **kw)
+    xo, so = O.solve(A, b, "sstep_cg", "none", **kw)
+    assert ok and so.converged and st.sstep_k == k
+    if k <= 2:
+        assert abs(st.iterations - so.iterations) <= max(k, 0.05 * so.iterations), (st.iterations, so.iterations)
+    else:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        its = []
+        try:
+            for t in (1, 2, 3, 4, 8):
+                gomp.omp_set_num_threads(t)
+                its.append(O.solve(A, b, "sstep_cg", "none", **kw)[1].iterations)
+        finally:
+            gomp.omp_set_num_threads(8)
+        assert 0.9 * min(its) <= st.iterations <= 1.1 * max(its), (st.iterations, its)
+    assert st.true_residual <= 1e-8
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    # batches (comm_events) = normal batches (k iterations, basis + residual
+    # SpMVs) + Cholesky-failure fallbacks (1 iteration, no charged SpMV)
+    B = st.comm_events
+    f = (k * B - st.iterations) // (k - 1) if k > 1 else 0
+    assert k * (B - f) + f == st.iterations
+    assert st.flop_count == B * (k * 2 * nz + k * 2 * n) + (B - f) * 2 * nz
+
+
+def test_sstep_cg_exact_cases(H, ctx):
+    n = 40
+    b = O.uniform(3, n)
+    ok, st, x = _solve_dev(H, ctx, _diag([1.0] * n), b, "sstep_cg", "none", sstep_k=1)
+    assert ok and st.iterations == 1 and np.abs(x - b).max() < 1e-14
+    # singular Gram matrix: the steepest-descent fallback (exact for c*I)
+    ok, st, x = _solve_dev(H, ctx, _diag([2.5] * n), b, "sstep_cg", "none", sstep_k=4)
+    assert ok and st.iterations == 1 and np.abs(x - b / 2.5).max() < 1e-14
+    # zero right-hand side: converged, no iterations
+    ok, st, x = _solve_dev(H, ctx, O.Csr.poisson(8), np.zeros(64), "sstep_cg", "none", sstep_k=2)
+    assert ok and st.iterations == 0 and not x.any()
+
+
+def test_sstep_cg_diverges_where_the_reference_does(H, ctx):
+    """Unscaled Poisson (spectrum up to 8): the reference's update x += B y
+    with B y the projection of r is a Richardson-type step and grows."""
+    A = O.Csr.poisson(12)
+    b = O.uniform(5, A.rows)
+    ok, st, x = _solve_dev(H, ctx, A, b, "sstep_cg", "none", sstep_k=4, max_iter=40)
+    xo, so = O.solve(A, b, "sstep_cg", "none", sstep_k=4, max_iter=40)
+    assert not ok and not so.converged and st.iterations == so.iterations == 40
+    assert st.final_residual > 1.0 and so.final_residual > 1.0
+
+
+def test_sstep_cg_is_fp64_only(H, ctx):
+    A = dev_csr(H, ctx, O.Csr.poisson(8))
+    b, x = ctx.array(np.ones(64)), ctx.array(n=64)
+    with pytest.raises(H.HspsError):
+        H.solve(A, b, x, "sstep_cg", None, H.default_params(precision="mixed"))
+
+
+@pytest.mark.parametrize("case", ["poisson32_ilu", "convdiff48_ilu", "poisson40_amg", "aniso32_jacobi",
+                                  "poisson32_ilu_mixed", "convdiff48_ilu_mixed"])
+def test_pipelined_fgmres_matches_oracle(H, ctx, case):
+    mixed = case.endswith("_mixed")
+    name = case.replace("_mixed", "")
+    A = {"poisson32_ilu": lambda: O.Csr.poisson(32), "convdiff48_ilu": lambda: O.Csr.convdiff(48, 0.02, 0.8, 0.6),
+         "poisson40_amg": lambda: O.Csr.poisson(40), "aniso32_jacobi": lambda: O.Csr.aniso(32, 0.1)}[name]()
+    pre = name.split("_")[1]
+    n, _, nz = A.shape_nnz()
+    b = O.uniform(7, n)
+    kw = dict(tol=1e-8, max_iter=3000, restart_size=30, drop_tol=0.0)
+    ok, st, x = _solve_dev(H, ctx, A, b, "pipelined_fgmres", pre, precision="mixed" if mixed else "fp64", **kw)
+    xo, so = O.solve(A, b, "pipelined_fgmres", pre, **kw)
+    assert ok and so.converged and st.true_residual <= 1e-8
+    it = so.iterations
+    bound = max(2, 0.10 * it) if mixed else max(1, 0.05 * it)
+    assert abs(st.iterations - it) <= bound, (st.iterations, it)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    if not mixed:  # one extra SpMV per cycle on top of FGMRES's count
+        cycles = st.restarts + 1
+        assert st.flop_count == (st.iterations + cycles) * 2 * nz + st.iterations * 4 * n
+
+
+def test_pipelined_fgmres_equals_fgmres_on_gpu(H, ctx):
+    A = O.Csr.convdiff(40, 0.05, 0.6, 0.8)
+    b = O.uniform(8, A.rows)
+    kw = dict(tol=1e-9, max_iter=2000, restart_size=25, drop_tol=0.0)
+    ok1, s1, x1 = _solve_dev(H, ctx, A, b, "fgmres", "ilu", **kw)
+    ok2, s2, x2 = _solve_dev(H, ctx, A, b, "pipelined_fgmres", "ilu", **kw)
+    assert ok1 and ok2 and abs(s1.iterations - s2.iterations) <= 1
+    assert np.linalg.norm(x1 - x2) <= 1e-7 * np.linalg.norm(x1)
+
+
+def test_pipelined_fgmres_reference_cases(H, ctx):
+    """test_improvements.cpp:226-283 on the device."""
+    ok, st, x = _solve_dev(H, ctx, _diag([2.0, 4.0, 8.0]), np.array([4.0, 8.0, 16.0]), "pipelined_fgmres", "jacobi",
+                           tol=1e-10)
+    np.testing.assert_allclose(x, [2.0, 2.0, 2.0], atol=1e-8)
+    P = O.Csr.poisson(16)
+    ok, st, x = _solve_dev(H, ctx, P, np.ones(256), "pipelined_fgmres", "ilu", tol=1e-10, drop_tol=0.0)
+    assert ok and np.linalg.norm(np.ones(256) - P.spmv(x)) / 16.0 < 1e-7
+    Cd = O.Csr.convdiff(10, 0.1, 5.0, 0.0)
+    ok, st, x = _solve_dev(H, ctx, Cd, np.ones(100), "pipelined_fgmres", "ilu", tol=1e-10, drop_tol=0.0)
+    assert ok
