@@ -767,7 +767,12 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     int total = 0, cycles = 0, conv = 0;
     double rel = 1.0;
     bool first = true;
-    const int steps_per_poll = 4;
+    // The host reads the cycle state with one poll of lag, so up to two polls
+    // of iterations are enqueued past the end of a cycle (their kernels exit
+    // at once).  Expensive iterations (an AMG V-cycle, large operators) poll
+    // every iteration; cheap ones every 4 to keep the launch queue ahead.
+    const bool costly = (has_m && M->kind == HSPS_PRECOND_AMG) || A.nnz > (1 << 21);
+    const int steps_per_poll = costly ? 1 : 4;
     Event ev[2];
     while (total < p.max_iter) {
         const int cm = std::min(m, p.max_iter - total);
