@@ -152,6 +152,28 @@ def bench_batched(H, ctx, ev, args):
         mats.append(A)
     bd = ctx.array(b)
     xd = ctx.array(n=N)
+    # GPU energy counter (ROCm SMI), read around the same timed solves
+    try:
+        energy = H.Energy(0)
+    except H.HspsError as e:
+        energy = None
+        log(f"[bench] no GPU energy counter: {e}")
+
+    def avg_power(fn, min_s=3.0):
+        """Device power while fn runs back to back: the energy counter ticks
+        about once a second, so the average is taken between two ticks both
+        observed while fn was running, at least min_s apart."""
+        _, _, t_prev = energy.read()
+        while True:
+            fn()
+            j0, _, t0 = energy.read()
+            if t0 != t_prev:
+                break
+        while True:
+            fn()
+            j1, _, t1 = energy.read()
+            if t1 - t0 >= min_s:
+                return (j1 - j0) / (t1 - t0)
 
     def batched(precision, steps, warmup):
         p = H.default_params(tol=1e-8, max_iter=20000, precision=precision, check_every=32,
@@ -294,6 +316,45 @@ def bench_batched(H, ctx, ev, args):
         "cpu_baseline": cpu,
         "batched_iteration": batched_roof,
     }
+    if energy is not None and not args.no_energy:
+        log("[bench] energy: device power under each workload")
+        S = len(systems)
+        pm = H.default_params(tol=1e-8, max_iter=20000, precision="mixed", check_every=32,
+                              batch_tile_rows=args.tile_rows)
+        w_mx = avg_power(lambda: B.solve(bd, xd, pm))
+        p64 = H.default_params(tol=1e-8, max_iter=20000, precision="fp64", check_every=32, cg_engine=0)
+        precs = [H.Precond(A, "jacobi", p64) for A in mats]
+        bufs = [(ctx.array(b[row_off[s]:row_off[s + 1]]), ctx.array(n=A.rows)) for s, A in enumerate(mats)]
+        nxt = [0]
+
+        def one():
+            s = nxt[0] % S
+            nxt[0] += 1
+            H.solve(mats[s], bufs[s][0], bufs[s][1], "cg", precs[s], p64)
+            ctx.sync()
+        w_64 = avg_power(one)
+        for M in precs:
+            M.close()
+        _, _, t_prev = energy.read()
+        while True:  # idle: one whole counter interval with nothing running
+            j0, _, t0 = energy.read()
+            if t0 != t_prev:
+                break
+            time.sleep(0.01)
+        time.sleep(1.05)
+        j1, _, t1 = energy.read()
+        w_idle = (j1 - j0) / (t1 - t0) if t1 > t0 else None
+        jm = w_mx * float(np.mean(t_mx)) / S
+        j6 = w_64 * fp64_time / S
+        res["energy"] = {"source": "GPU energy counter (ROCm SMI rsmi_dev_energy_count_get, whole device, updated "
+                                   "about once a second): average power over >= 3 s of each workload run back to "
+                                   "back, times its measured time per solve",
+                         "batched_mixed_J_per_solve": jm, "batched_mixed_avg_W": w_mx,
+                         "fp64_one_at_a_time_J_per_solve": j6, "fp64_one_at_a_time_avg_W": w_64,
+                         "idle_W": w_idle,
+                         "energy_ratio_fp64_over_batched": j6 / jm if jm > 0 else None,
+                         "device": energy.info()}
+        energy.close()
     for A in mats:
         A.close()
     B.close()
@@ -556,6 +617,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-energy", action="store_true", help="skip the device energy measurement")
     ap.add_argument("--workload", choices=["batched", "single", "amg", "convdiff", "serve"], default="batched")
     ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")

@@ -36,7 +36,9 @@ protected:
     std::shared_ptr<EnergyMonitor> energy_mon_ = nullptr;
 
     Real relative_residual(const SparseMatrix& A, const Vector& b, const Vector& x) const;
-    /// Shared device call for CG / FGMRES.
+    /// Shared device call for the solvers below (C ABI entry per kind).
+    enum DeviceSolver { kCG = 0, kFGMRES = 1, kPipelinedFGMRES = 2, kSStepCG = 3 };
+    int sstep_k_ = 1;
     bool device_solve(int solver, const SparseMatrix& A, const Vector& b, Vector& x, SolverStats& stats);
 };
 

@@ -9,11 +9,14 @@
 #include <iomanip>
 #include <iostream>
 
+#include "energy/energy_monitor.hpp"
 #include "preconditioners/amg.hpp"
 #include "preconditioners/ilu.hpp"
 #include "preconditioners/jacobi.hpp"
 #include "solvers/cg_solver.hpp"
 #include "solvers/fgmres_solver.hpp"
+#include "solvers/pipelined_fgmres_solver.hpp"
+#include "solvers/sstep_cg_solver.hpp"
 #include "solvers/solver_factory.hpp"
 
 namespace hsps {
@@ -35,6 +38,8 @@ hsps_params to_device_params(const SolverParams& p) {
     d.check_every = p.check_every;
     d.batch_graphs = p.device_graphs ? 1 : 0;
     d.fgmres_inner_rel = p.fgmres_inner_rel;
+    d.sstep_k = p.sstep_k;
+    d.sstep_cond_tol = p.sstep_cond_tol;
     return d;
 }
 
@@ -119,13 +124,24 @@ bool SolverBase::device_solve(int solver, const SparseMatrix& A, const Vector& b
         if (!M) throw std::invalid_argument("solve: preconditioner has no device object (call setup first)");
     }
     hsps_params p = to_device_params(params_);
+    if (solver == kSStepCG) p.sstep_k = sstep_k_;
     hsps_stats s{};
     const double* bd = b.device_data();
     double* xd = x.device_data_mut();
-    const int rc = solver == 0 ? hsps_cg_solve(A.device(), M, bd, xd, &p, &s)
-                               : hsps_fgmres_solve(A.device(), M, bd, xd, &p, &s);
+    int rc = 0;
+    switch (solver) {
+        case kCG: rc = hsps_cg_solve(A.device(), M, bd, xd, &p, &s); break;
+        case kFGMRES: rc = hsps_fgmres_solve(A.device(), M, bd, xd, &p, &s); break;
+        case kPipelinedFGMRES: rc = hsps_pipelined_fgmres_solve(A.device(), M, bd, xd, &p, &s); break;
+        case kSStepCG: rc = hsps_sstep_cg_solve(A.device(), bd, xd, &p, &s); break;
+        default: throw std::invalid_argument("solve: unknown solver");
+    }
     dev_check(rc);
-    stats.solver_used = solver == 0 ? SolverType::CG : SolverType::FGMRES;
+    stats.solver_used = solver == kCG ? SolverType::CG : solver == kSStepCG ? SolverType::SSTEP_CG : SolverType::FGMRES;
+    if (solver == kSStepCG) {
+        stats.sstep_k = s.sstep_k;
+        stats.comm_bytes = s.comm_events;  // one global reduction per batch (sstep_cg_solver.cpp:229)
+    }
     stats.precond_used = precond_ ? precond_->type() : PrecondType::NONE;
     stats.iterations = s.iterations;
     stats.initial_residual = s.initial_residual;
@@ -136,13 +152,15 @@ bool SolverBase::device_solve(int solver, const SparseMatrix& A, const Vector& b
     stats.energy_joules = s.energy_joules;
     stats.true_residual = s.true_residual;
     stats.reliable_updates = s.reliable_updates;
-    // FGMRES leaves converged/final_residual untouched on max_iter exhaustion
-    // in FP64 mode (fgmres_solver.cpp:197-226); CG always writes them.
-    const bool write = solver == 0 || s.converged || params_.precision == Precision::MIXED;
+    // (Pipelined) FGMRES leaves converged/final_residual untouched on max_iter
+    // exhaustion in FP64 mode (fgmres_solver.cpp:197-226); CG and s-step CG
+    // always write them.
+    const bool write = solver == kCG || solver == kSStepCG || s.converged || params_.precision == Precision::MIXED;
     if (write) {
         stats.converged = s.converged != 0;
         stats.final_residual = s.final_residual;
     }
+    if (energy_mon_) energy_mon_->record(s.iterations, s.final_residual, s.flop_count, s.mem_bytes);
     if (params_.verbose)
         std::cout << "  " << to_string(stats.solver_used) << " iters=" << s.iterations << " rel_res=" << std::scientific
                   << s.final_residual << " true=" << s.true_residual << "\n";
@@ -150,10 +168,17 @@ bool SolverBase::device_solve(int solver, const SparseMatrix& A, const Vector& b
 }
 
 bool CGSolver::solve(const SparseMatrix& A, const Vector& b, Vector& x, SolverStats& stats) {
-    return device_solve(0, A, b, x, stats);
+    return device_solve(kCG, A, b, x, stats);
 }
 bool FGMRESSolver::solve(const SparseMatrix& A, const Vector& b, Vector& x, SolverStats& stats) {
-    return device_solve(1, A, b, x, stats);
+    return device_solve(kFGMRES, A, b, x, stats);
+}
+bool PipelinedFGMRESSolver::solve(const SparseMatrix& A, const Vector& b, Vector& x, SolverStats& stats) {
+    return device_solve(kPipelinedFGMRES, A, b, x, stats);
+}
+bool SStepCGSolver::solve(const SparseMatrix& A, const Vector& b, Vector& x, SolverStats& stats) {
+    sstep_k_ = k_;
+    return device_solve(kSStepCG, A, b, x, stats);
 }
 
 // ---- factory (solver_factory.cpp:25-175) --------------------------------------------------
@@ -161,8 +186,10 @@ std::unique_ptr<SolverBase> SolverFactory::make_solver(SolverType type, const So
     std::unique_ptr<SolverBase> s;
     switch (type) {
         case SolverType::CG:
-        case SolverType::SSTEP_CG:  // s-step CG: same Krylov space as PCG on one device
             s = std::make_unique<CGSolver>();
+            break;
+        case SolverType::SSTEP_CG:
+            s = std::make_unique<SStepCGSolver>(params.sstep_k);
             break;
         case SolverType::FGMRES:
         case SolverType::BICGSTAB:  // the reference falls back to FGMRES (solver_factory.cpp:34-37)

@@ -94,6 +94,10 @@ SIGNATURES = {
     "hsps_pipelined_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_sstep_cg_solve": (_i, [_vp, _vp, _vp, _PP, _SP]),
     "hsps_spectral_interval": (_i, [_vp, _i, _dp, _dp]),
+    "hsps_energy_open": (_i, [_i, C.POINTER(_vp)]),
+    "hsps_energy_read": (_i, [_vp, _dp, _dp, _dp]),
+    "hsps_energy_info": (_i, [_vp, _dp, _dp, _dp, _dp, C.c_char_p, _i]),
+    "hsps_energy_close": (_i, [_vp]),
     "hsps_features": (_i, [_vp, _dp]),
     "hsps_batch_create": (_i, [_vp, _i, _ip, _ip, _ip, _dp, C.POINTER(_vp)]),
     "hsps_batch_destroy": (_i, [_vp]),
@@ -388,6 +392,40 @@ def solve(A, b, x, solver="cg", M=None, params=None):
          "pipelined_fgmres": lib().hsps_pipelined_fgmres_solve}[solver]
     rc = check(f(A.h, M.h if M is not None else None, b.ptr, x.ptr, C.byref(p), C.byref(st)))
     return bool(rc), st
+
+
+class Energy:
+    """GPU energy counter (ROCm SMI).  Raises HspsError (HSPS_ERR_UNSUPPORTED)
+    where telemetry is unavailable."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib().hsps_energy_open(device, C.byref(h)))
+        self.h = h
+
+    def read(self):
+        """(cumulative joules, current watts, counter timestamp s)."""
+        j, w, t = C.c_double(), C.c_double(), C.c_double()
+        check(lib().hsps_energy_read(self.h, C.byref(j), C.byref(w), C.byref(t)))
+        return j.value, w.value, t.value
+
+    def info(self):
+        v = [C.c_double() for _ in range(4)]
+        name = C.create_string_buffer(128)
+        check(lib().hsps_energy_info(self.h, *[C.byref(x) for x in v], name, 128))
+        return {"power_cap_w": v[0].value, "temperature_c": v[1].value, "busy_pct": v[2].value,
+                "vram_used_mb": v[3].value, "name": name.value.decode(errors="replace")}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hsps_energy_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def spectral_interval(A, steps=20):

@@ -201,6 +201,22 @@ int hsps_sstep_cg_solve(hsps_csr A, const double* d_b, double* d_x, const hsps_p
  * q = 1/sqrt(n) for min(n, steps, 64) steps, Gershgorin bounds of T. */
 int hsps_spectral_interval(hsps_csr A, int steps, double* lo, double* hi);
 
+/* ---- GPU energy telemetry through ROCm SMI: the MI355X counterpart of
+ *      NVMLMonitor (nvml_monitor.cpp:35-193) and of EnergyMonitor's RAPL
+ *      reader (energy_monitor.cpp:27-164).  librocm_smi64 is loaded at run
+ *      time; open returns HSPS_ERR_UNSUPPORTED without it or without a
+ *      device energy counter. ---------------------------------------------- */
+typedef struct hsps_energy_s* hsps_energy;
+int hsps_energy_open(int device, hsps_energy* out);
+/* Cumulative device energy in J (monotonic: difference two reads), current
+ * socket power in W (-1 when unknown), counter timestamp in s; any may be NULL. */
+int hsps_energy_read(hsps_energy E, double* joules, double* watts, double* timestamp_s);
+/* Power cap (W), junction temperature (C), busy percent, VRAM used (MiB),
+ * device name; -1 where unknown; any may be NULL. */
+int hsps_energy_info(hsps_energy E, double* power_cap_w, double* temperature_c, double* busy_pct,
+                     double* vram_used_mb, char* name, int name_len);
+int hsps_energy_close(hsps_energy E);
+
 /* ---- matrix features for the adaptive ladder
  *      (AdaptiveSelector::extract_features, adaptive_selector.cpp:38-220).
  *      out[0..7] = n, nnz, diag_dominance, symmetry_residual, frobenius,

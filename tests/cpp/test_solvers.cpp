@@ -2,12 +2,17 @@
 // the reference's tests/test_preconditioners.cpp:26-159 and
 // tests/test_solvers.cpp:42-264, plus mixed-precision variants.
 #include "hsps_test.hpp"
+#include <chrono>
+#include "energy/energy_monitor.hpp"
+#include "parallel/nvml_monitor.hpp"
 #include "preconditioners/amg.hpp"
 #include "preconditioners/ilu.hpp"
 #include "preconditioners/jacobi.hpp"
 #include "solvers/cg_solver.hpp"
 #include "solvers/fgmres_solver.hpp"
+#include "solvers/pipelined_fgmres_solver.hpp"
 #include "solvers/solver_factory.hpp"
+#include "solvers/sstep_cg_solver.hpp"
 
 using namespace hsps;
 
@@ -172,5 +177,110 @@ TEST(factory_parse) {
     expect_true(SolverFactory::parse_precond("ILU0") == PrecondType::ILU, "parse");
     expect_throw<std::invalid_argument>([] { SolverFactory::parse_solver("gmres2"); }, "unknown solver");
     expect_throw<std::invalid_argument>([] { SolverFactory::parse_precond("ssor"); }, "unknown precond");
+}
+// test_improvements.cpp:226-283 (PipelinedFGMRESSolver)
+TEST(pipelined_fgmres_reference_cases) {
+    auto D = SparseMatrix::from_coo(3, 3, {0, 1, 2}, {0, 1, 2}, {2.0, 4.0, 8.0});
+    Vector b = {4.0, 8.0, 16.0}, x(3, 0.0);
+    auto jac = std::make_shared<JacobiPreconditioner>();
+    jac->setup(D);
+    PipelinedFGMRESSolver pf;
+    pf.set_params(params());
+    pf.set_preconditioner(jac);
+    SolverStats s;
+    pf.solve(D, b, x, s);
+    for (int i = 0; i < 3; ++i) expect_near(x[i], 2.0, 1e-8, "pfgm x");
+    auto A = SparseMatrix::poisson_2d(12);
+    Vector bb(144, 1.0), xs(144, 0.0), xp(144, 0.0);
+    auto ilu = std::make_shared<ILUPreconditioner>();
+    ilu->setup(A);
+    FGMRESSolver f;
+    f.set_params(params());
+    f.set_preconditioner(ilu);
+    PipelinedFGMRESSolver p2;
+    p2.set_params(params());
+    p2.set_preconditioner(ilu);
+    SolverStats ss, sp;
+    f.solve(A, bb, xs, ss);
+    expect_true(p2.solve(A, bb, xp, sp), "pipelined converges");
+    expect_less((xs - xp).norm2() / xs.norm2(), 1e-5, "pipelined and standard produce the same solution");
+    expect_true(sp.solver_used == SolverType::FGMRES, "solver_used");
+    auto C = SparseMatrix::convection_diffusion_2d(10, 0.1, 5.0, 0.0);
+    Vector bc(100, 1.0), xc(100, 0.0);
+    auto iluc = std::make_shared<ILUPreconditioner>();
+    iluc->setup(C);
+    PipelinedFGMRESSolver p3;
+    p3.set_params(params());
+    p3.set_preconditioner(iluc);
+    SolverStats sc;
+    expect_true(p3.solve(C, bc, xc, sc), "pipelined handles a nonsymmetric system");
+}
+TEST(sstep_cg_factory_and_stats) {
+    // Poisson scaled by 1/4 (spectrum in (0, 2)), where the reference's s-step
+    // update converges
+    auto P = SparseMatrix::poisson_2d(16);
+    std::vector<Real> v(P.values().begin(), P.values().end());
+    for (auto& a : v) a *= 0.25;
+    SparseMatrix A(256, 256, P.row_ptr(), P.col_idx(), v);
+    Vector b(256, 1.0), x(256, 0.0);
+    SolverParams p = params();
+    p.tol = 1e-8;
+    p.max_iter = 20000;
+    p.sstep_k = 2;
+    auto s = SolverFactory::make_solver(SolverType::SSTEP_CG, p);
+    expect_true(s->type() == SolverType::SSTEP_CG, "factory builds s-step CG");
+    SolverStats st;
+    expect_true(s->solve(A, b, x, st), "s-step converges on the scaled Poisson");
+    // a batch is 2 iterations, or 1 when Cholesky of a near-singular Gram
+    // matrix fails and the steepest-descent step is taken; one reduction each
+    expect_true(st.sstep_k == 2, "k = 2");
+    expect_true(st.comm_bytes <= st.iterations && st.iterations <= 2 * st.comm_bytes, "one reduction per batch");
+    expect_less(rel_res(A, b, x), 1e-8, "rel res");
+    SStepCGSolver s4(4);
+    s4.set_params(p);
+    expect_true(s4.k() == 4, "k");
+    Vector x4(256, 0.0);
+    SolverStats st4;
+    expect_true(s4.solve(A, b, x4, st4) && st4.sstep_k == 4, "k = 4");
+}
+// NVMLMonitor / EnergyMonitor API over the GPU energy counter
+TEST(energy_monitors) {
+    NVMLMonitor mon(0);
+    EnergyMonitor em;
+    expect_true(mon.available() == em.hw_energy_available(), "both see the same counter");
+    auto A = SparseMatrix::poisson_2d(256);
+    Vector b(A.rows(), 1.0), x(A.rows(), 0.0);
+    auto jac = std::make_shared<JacobiPreconditioner>();
+    jac->setup(A);
+    CGSolver cg;
+    SolverParams p = params();
+    p.max_iter = 5000;
+    cg.set_params(p);
+    cg.set_preconditioner(jac);
+    auto shared = std::make_shared<EnergyMonitor>();
+    cg.set_energy_monitor(shared);
+    shared->start();
+    const bool started = mon.start();
+    SolverStats st;
+    // the device counter updates about once a second: solve for > 2.5 s
+    int solves = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < 2.5) {
+        Vector xr(A.rows(), 0.0);
+        cg.solve(A, b, xr, st);
+        ++solves;
+    }
+    mon.stop();
+    shared->stop();
+    expect_true(shared->total_flops() == solves * st.flop_count, "solver records into its monitor");
+    expect_true((int)shared->history().size() == solves, "one record per solve");
+    if (started) {
+        expect_true(mon.elapsed_energy_j() > 0.0, "GPU energy advances");
+        expect_true(mon.history().size() == 1, "stop() samples");
+        expect_true(shared->total_energy_j() > 0.0 && shared->sample().hw_available, "hardware energy");
+        expect_true(!mon.device_name().empty(), "device name");
+    } else {
+        expect_near(shared->total_energy_j(), shared->total_energy_proxy_j(), 1e-12, "proxy without a counter");
+    }
 }
 int main() { return run_all(); }
