@@ -91,6 +91,10 @@ int hsps_energy_open(int device, hsps_energy* out) {
     return guarded([&] {
         if (!out) throw Error(HSPS_ERR_INVALID, "hsps_energy_open: out is NULL");
         *out = nullptr;
+        int count = 0;
+        HSPS_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw Error(HSPS_ERR_INVALID, "hsps_energy_open: device " + std::to_string(device) + " not present");
         Smi& s = smi();
         if (!s.ok) throw Error(HSPS_ERR_UNSUPPORTED, "GPU energy telemetry unavailable: " + s.why);
         char bus[64] = {0};

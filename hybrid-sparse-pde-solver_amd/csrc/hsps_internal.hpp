@@ -25,11 +25,15 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string& msg);
 
+// A failed call also clears HIP's last-error slot, so that a later launch
+// check does not report this (already thrown) error again.
 #define HSPS_HIP(call)                                                                            \
     do {                                                                                          \
         hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
+        if (e_ != hipSuccess) {                                                                   \
+            (void)hipGetLastError();                                                              \
             throw ::hsps_dev::Error(HSPS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                                         \
     } while (0)
 
 #define HSPS_CHECK_LAUNCH() HSPS_HIP(hipGetLastError())
