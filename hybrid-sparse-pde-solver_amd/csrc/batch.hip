@@ -714,6 +714,12 @@ struct OpF1 {
             S.flag = BS_NORMAL;
             return;
         }
+        if (S.iter >= B.max_iter) {  // a check or reliable update on the last allowed iteration
+            S.halt = 1;
+            S.flag = BS_NORMAL;
+            S.reason = BR_NONE;
+            return;
+        }
         if (S.reason == BR_RESTART) {
             S.rho = tot[1];
             S.rho_old = tot[1];
@@ -1058,7 +1064,10 @@ struct Batch {
     // per-solve work (reused)
     DevBuf st, partials, r64, r, p, p1, q, xlo;
     int work_prec = -1;
-    struct G { hipGraphExec_t exec = nullptr; int wave = -1, prec = -1, chunk = 0; };
+    // a captured chunk replays the kernel arguments it was captured with
+    // (tolerance, iteration cap, tiles, ...), so the whole argument block is
+    // part of its key
+    struct G { hipGraphExec_t exec = nullptr; int wave = -1, prec = -1, chunk = 0; BArgs args{}; };
     std::vector<G> graphs;
     const double* graph_b = nullptr;  // the b and x the captured kernels point at
     const double* graph_x = nullptr;
@@ -1264,7 +1273,7 @@ static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci
 }
 
 static BArgs args_of(const Batch& B, const hsps_params& p) {
-    BArgs a;
+    BArgs a{};  // value-initialised: padding is zero, so argument blocks compare bytewise
     a.rp = B.rp.as<int>();
     a.ci = B.ci.as<int>();
     a.v64 = B.v64.as<double>();
@@ -1339,6 +1348,8 @@ static void ensure_work(Batch& B, int prec) {
     }
 }
 
+constexpr size_t kMaxGraphs = 16;  // captured chunks kept per batch
+
 static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const double* b, double* x, int prec,
                          hipStream_t s, bool use_graph) {
     const int k0 = B.waves[w].first, k1 = B.waves[w].second;
@@ -1351,8 +1362,12 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const doubl
     if (!use_graph) { direct(); return; }
     Batch::G* g = nullptr;
     for (auto& c : B.graphs)
-        if (c.wave == w && c.prec == prec && c.chunk == chunk) g = &c;
+        if (c.wave == w && c.prec == prec && c.chunk == chunk && std::memcmp(&c.args, &a, sizeof(BArgs)) == 0) g = &c;
     if (!g) {
+        if (B.graphs.size() >= kMaxGraphs) {  // oldest first
+            (void)hipGraphExecDestroy(B.graphs.front().exec);
+            B.graphs.erase(B.graphs.begin());
+        }
         hipGraph_t graph = nullptr;
         HSPS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
@@ -1367,6 +1382,7 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const doubl
         ng.wave = w;
         ng.prec = prec;
         ng.chunk = chunk;
+        ng.args = a;
         const hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HSPS_HIP(e);

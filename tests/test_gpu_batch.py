@@ -232,3 +232,28 @@ def test_batch_solves_with_new_vectors_each_call(H, ctx):
             xs = x[B.row_off[s]:B.row_off[s + 1]]
             assert np.linalg.norm(bs[s] - A.spmv(xs)) <= 1e-8 * np.linalg.norm(bs[s])
         outs.append((bd, xd))  # keep the buffers alive: each call gets new addresses
+
+
+def test_batch_params_change_between_calls(H, ctx):
+    """Replayed iteration chunks carry the kernel arguments they were captured
+    with: a later call with another iteration cap, tolerance or precision on
+    the same vectors must run with its own (regression: the graph cache was
+    keyed on wave / precision / chunk only)."""
+    arrs, _, oracles = tiled_systems(H)
+    arrs, oracles = [arrs[0], arrs[3], arrs[4]], [oracles[0], oracles[3], oracles[4]]
+    B = H.Batch.from_systems(ctx, arrs)
+    bs = [A.spmv(O.uniform(40 + s, A.rows)) for s, A in enumerate(oracles)]
+    bd, xd = ctx.array(np.concatenate(bs)), ctx.array(n=B.n)
+
+    def run(**kw):
+        H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
+        return B.solve(bd, xd, H.default_params(batch_tile_rows=1024, **kw))
+
+    ok, it8, cv, tr = run(tol=1e-8, max_iter=5000, precision="mixed")
+    assert ok and (tr <= 1e-8).all()
+    ok, it10, cv, tr = run(tol=1e-8, max_iter=10, precision="mixed")
+    assert not cv.any() and (it10 <= 10).all(), it10
+    ok, it11, cv, tr = run(tol=1e-11, max_iter=5000, precision="mixed")
+    assert ok and (tr <= 1e-11).all() and (it11 > it8).all(), (it8, it11)
+    ok, it8b, cv, tr = run(tol=1e-8, max_iter=5000, precision="mixed")
+    assert ok and np.array_equal(it8b, it8)
