@@ -291,6 +291,21 @@ template <class T>
 __device__ __forceinline__ Pack<T> ld16(const T* p) { return *reinterpret_cast<const Pack<T>*>(p); }
 template <class T>
 __device__ __forceinline__ void st16(T* p, const Pack<T>& v) { *reinterpret_cast<Pack<T>*>(p) = v; }
+// Streaming 16-byte store (nontemporal hint) for K1's new direction: it is
+// read back by the next two kernels only, and keeping it out of the caches'
+// way leaves more of the 256 MB Infinity Cache to r and x_lo.
+template <class T>
+struct Vec16 {
+    typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+template <class T>
+__device__ __forceinline__ void st16_stream(T* p, const Pack<T>& v) {
+    typename Vec16<T>::type w;
+#pragma unroll
+    for (int e = 0; e < Pack<T>::G; ++e) w[e] = v.e[e];
+    __builtin_nontemporal_store(w, reinterpret_cast<typename Vec16<T>::type*>(p));
+}
+constexpr int kStageU = 4;  // K1 staging: groups per lane whose loads are issued together
 template <class T>
 __device__ __forceinline__ unsigned mask_word(const unsigned char* m, int g) {
     if constexpr (Pack<T>::G == 4) return *reinterpret_cast<const unsigned*>(m + g);
@@ -346,19 +361,35 @@ __device__ __forceinline__ void k1_stencil_vec(const Stencil<T>& A, int r0, int 
         const int base = a0 - W.H;
         const int g0 = max(base, lo & ~(G - 1)), g1 = min(c1 + W.H, hi);
         __syncthreads();  // the previous chunk's readers are done with lds
-        for (int g = g0 + tid * G; g < g1; g += kBlock * G) {
-            const Pack<T> rv = ld16(r + g), pv = ld16(po + g);
-            Pack<T> v;
+        // kStageU groups per lane: every load is issued before the first
+        // store (vmcnt counts loads and stores in one in-order queue, so a
+        // load wait placed after a store also waits for that store)
+        for (int gb = g0 + tid * G; gb < g1; gb += kStageU * kBlock * G) {
+            Pack<T> rv[kStageU], pv[kStageU];
 #pragma unroll
-            for (int e = 0; e < G; ++e) v.e[e] = (T)((double)(T)(dv * rv.e[e]) + beta * (double)pv.e[e]);
+            for (int u = 0; u < kStageU; ++u) {
+                const int g = gb + u * kBlock * G;
+                if (g < g1) {
+                    rv[u] = ld16(r + g);
+                    pv[u] = ld16(po + g);
+                }
+            }
 #pragma unroll
-            for (int e = 0; e < G; ++e) lds[g - base + e] = v.e[e];
-            if (g >= c0 && g + G <= c1) {
-                st16(pn + g, v);
-            } else {
+            for (int u = 0; u < kStageU; ++u) {
+                const int g = gb + u * kBlock * G;
+                if (g >= g1) break;
+                Pack<T> v;
 #pragma unroll
-                for (int e = 0; e < G; ++e)
-                    if (g + e >= c0 && g + e < c1) pn[g + e] = v.e[e];
+                for (int e = 0; e < G; ++e) v.e[e] = (T)((double)(T)(dv * rv[u].e[e]) + beta * (double)pv[u].e[e]);
+#pragma unroll
+                for (int e = 0; e < G; ++e) lds[g - base + e] = v.e[e];
+                if (g >= c0 && g + G <= c1) {
+                    st16_stream(pn + g, v);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < G; ++e)
+                        if (g + e >= c0 && g + e < c1) pn[g + e] = v.e[e];
+                }
             }
         }
         __syncthreads();
