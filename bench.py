@@ -47,7 +47,7 @@ SPMV_FP64_HBM_BYTES = (154831 * 2.0 + 32768 * 1.0) * 1024
 # `bench.py --only-batched --steps 1 --warmup 0 --tile-rows 2048`, summed over
 # every K1/K2 dispatch of the run and divided by its row-iterations (sum of
 # rows x iterations; profiles/batched512_pcg_jacobi_kernels.md).
-BATCHED_HBM_BYTES_PER_ROW_ITER = 35.7
+BATCHED_HBM_BYTES_PER_ROW_ITER = 35.6
 
 
 def log(*a):
@@ -608,7 +608,14 @@ def bench_serve(H, ctx, args):
                   "states": {H.STATE_NAMES[s]: states.count(s) for s in sorted(set(states))},
                   "batched": int(sum(r[2].batched for r in results)),
                   "escalated": int(sum(r[2].escalations > 0 for r in results)),
-                  "makespan_s": makespan},
+                  "makespan_s": makespan,
+                  "by_state": {H.STATE_NAMES[s]: {
+                      "n": states.count(s),
+                      "p50_ms": 1e3 * float(np.percentile([l for l, t in zip(lat, states) if t == s], 50)),
+                      "p99_ms": 1e3 * float(np.percentile([l for l, t in zip(lat, states) if t == s], 99)),
+                      "mean_queue_ms": 1e3 * float(np.mean([r[2].queue_s for r, t in zip(results, states) if t == s])),
+                      "max_rows": int(max(len(reqs[i][0]) - 1 for i, t in enumerate(states) if t == s))}
+                      for s in sorted(set(states))}},
     }
 
 
