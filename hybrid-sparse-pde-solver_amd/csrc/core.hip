@@ -85,6 +85,12 @@ void Csr::build_schedule() {
     n_blocks = (int)b.size() - 1;
     blocks = DevBuf(ctx, b.size() * sizeof(int));
     HSPS_HIP(hipMemcpy(blocks.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int> b32 = make_row_blocks(h_rp.data(), rows, kMaxBlockRows32, kLdsNnz32);
+    n_blocks32 = (int)b32.size() - 1;
+    // partial-sum buffers are sized by n_blocks for both precisions
+    if (n_blocks32 > n_blocks) throw Error(HSPS_ERR_HIP, "csr: FP32 schedule has more blocks than FP64");
+    blocks32 = DevBuf(ctx, b32.size() * sizeof(int));
+    HSPS_HIP(hipMemcpy(blocks32.p, b32.data(), b32.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
 __global__ void k_f64_to_f32(int n, const double* __restrict__ x, float* __restrict__ y) {

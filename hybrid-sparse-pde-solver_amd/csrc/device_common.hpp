@@ -114,10 +114,11 @@ template <> struct AccOf<double> { using type = double; };
 template <> struct AccOf<float> { using type = float; };
 
 // CSR row-block SpMV.  Workgroup b owns rows [blocks[b], blocks[b+1]).
-//  * staged path (block nnz <= kLdsNnz): the workgroup streams val/col
+//  * staged path (block nnz <= kLdsNnz for FP64 sums, kLdsNnz32 for FP32,
+//    on the matching block table): the workgroup streams val/col
 //    contiguously, writes val*x[col] products to LDS, then sums each row from
 //    LDS with tpr lanes per row (tpr = pow2 <= 256/rows, <= 64).
-//  * long-row path (one row with nnz > kLdsNnz): the whole workgroup strides
+//  * long-row path (one row with more nonzeros than that): the whole workgroup strides
 //    the row and reduces.
 // Epi supplies: skip(), row(row, sum, red[]), NV, and the partials buffer
 // (NV * n_blocks doubles) when NV > 0; a finish kernel consumes the partials.
@@ -126,15 +127,16 @@ __global__ void __launch_bounds__(kBlock)
 k_spmv(const int* __restrict__ blocks, const int* __restrict__ rp, const int* __restrict__ ci,
        const V* __restrict__ val, const T* __restrict__ x, Epi epi) {
     using Acc = typename AccOf<T>::type;
+    constexpr int kCap = sizeof(Acc) == 8 ? kLdsNnz : kLdsNnz32;  // 16 KB of products either way
     if (epi.skip()) return;
-    __shared__ Acc prod[kLdsNnz];
+    __shared__ Acc prod[kCap];
     const int tid = threadIdx.x;
     const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
     const int k0 = rp[r0], k1 = rp[r1];
     double red[Epi::NV > 0 ? Epi::NV : 1];
 #pragma unroll
     for (int k = 0; k < (Epi::NV > 0 ? Epi::NV : 1); ++k) red[k] = 0.0;
-    if (k1 - k0 > kLdsNnz) {
+    if (k1 - k0 > kCap) {
         Acc s = 0;
         for (int k = k0 + tid; k < k1; k += kBlock) s += (Acc)val[k] * (Acc)x[ci[k]];
         double v1[1] = {(double)s};
@@ -210,11 +212,19 @@ struct EpiSmooth : SkipFlag {
     }
 };
 
+// Row blocks the SpMV over vectors of type T runs on (the count of partial
+// sums an epilogue with NV > 0 leaves for its finish kernel).
+template <typename T> inline int spmv_blocks(const Csr& A) {
+    return sizeof(typename AccOf<T>::type) == 8 ? A.n_blocks : A.n_blocks32;
+}
+
 template <typename V, typename T, class Epi>
 inline void launch_spmv(const Csr& A, const V* val, const T* x, const Epi& epi, hipStream_t s) {
-    if (A.n_blocks == 0) return;
-    hipLaunchKernelGGL((k_spmv<V, T, Epi>), dim3(A.n_blocks), dim3(kBlock), 0, s, A.blocks.as<int>(),
-                       A.rp.as<int>(), A.ci.as<int>(), val, x, epi);
+    const int nb = spmv_blocks<T>(A);
+    if (nb == 0) return;
+    const int* blk = sizeof(typename AccOf<T>::type) == 8 ? A.blocks.as<int>() : A.blocks32.as<int>();
+    hipLaunchKernelGGL((k_spmv<V, T, Epi>), dim3(nb), dim3(kBlock), 0, s, blk, A.rp.as<int>(), A.ci.as<int>(), val,
+                       x, epi);
     HSPS_CHECK_LAUNCH();
 }
 

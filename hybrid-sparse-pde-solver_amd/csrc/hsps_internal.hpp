@@ -91,8 +91,10 @@ void ctx_release(Ctx* c);
 
 // SpMV schedule constants (gfx950: 64-wide waves, 160 KB LDS per CU).
 constexpr int kBlock = 256;       // threads per workgroup (4 waves)
-constexpr int kLdsNnz = 2048;     // products staged in LDS per row block
+constexpr int kLdsNnz = 2048;     // FP64 products staged in LDS per row block (16 KB)
 constexpr int kMaxBlockRows = 256;
+constexpr int kLdsNnz32 = 4096;   // FP32 products in the same 16 KB
+constexpr int kMaxBlockRows32 = 512;
 constexpr int kMaxRedBlocks = 2048;  // cap for grid-stride reduction kernels
 
 // Device CSR: int32 pattern, FP64 values + FP32 copy, diagonal, row blocks.
@@ -103,8 +105,9 @@ struct Csr {
     int rows = 0, cols = 0, nnz = 0;
     std::vector<int> h_rp, h_ci;
     std::vector<double> h_v;
-    DevBuf rp, ci, v64, v32, diag64, blocks;
-    int n_blocks = 0;
+    DevBuf rp, ci, v64, v32, diag64, blocks, blocks32;
+    int n_blocks = 0;    // FP64 row blocks (kMaxBlockRows / kLdsNnz)
+    int n_blocks32 = 0;  // FP32 row blocks (kMaxBlockRows32 / kLdsNnz32), <= n_blocks
     // this matrix as a one-system batch (built on the first PCG + Jacobi
     // solve that uses the batched engine; dropped when the values change)
     std::shared_ptr<Batch> solo;
@@ -123,8 +126,9 @@ struct SoloResult {
 // kernel sequence serves those better.
 bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResult* out);
 
-// Row-block partition: consecutive rows, <= kMaxBlockRows rows and <=
-// kLdsNnz nonzeros per block; a longer row gets a block of its own.
+// Row-block partition: consecutive rows, <= max_rows rows and <= max_nnz
+// nonzeros per block; a longer row gets a block of its own.  Greedy, so a
+// looser (max_rows, max_nnz) never yields more blocks.
 std::vector<int> make_row_blocks(const int* rp, int rows, int max_rows, int max_nnz);
 
 // Wrap a device CSR from host arrays (used by ILU factors).

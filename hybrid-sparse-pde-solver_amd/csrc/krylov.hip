@@ -574,7 +574,7 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
         e.p = p_; e.q = q_; e.st = d; e.partials = part;
         if (MIXED) launch_spmv(A, A.v32.as<float>(), (const float*)p_, e, s);
         else launch_spmv(A, A.v64.as<double>(), (const double*)p_, e, s);
-        launch_finish<1>(part, A.n_blocks, OpAlpha{d}, s);
+        launch_finish<1>(part, spmv_blocks<T>(A), OpAlpha{d}, s);
     };
     auto update = [&]() {
         if (MIXED) {
