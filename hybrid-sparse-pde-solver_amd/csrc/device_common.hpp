@@ -144,7 +144,36 @@ k_spmv(const int* __restrict__ blocks, const int* __restrict__ rp, const int* __
         block_sum<1>(v1, lred);
         if (tid == 0) epi.row(r0, (Acc)v1[0], red);
     } else {
-        for (int k = k0 + tid; k < k1; k += kBlock) prod[k - k0] = (Acc)val[k] * (Acc)x[ci[k]];
+        // kCap / kBlock products per thread: every column and value load is
+        // issued, then every x gather, then the LDS stores, so a wave keeps
+        // all its loads in flight at once instead of two per product.
+        // The trip count nu is uniform over the workgroup (a scalar branch,
+        // no exec-mask split); lanes past the block's last nonzero re-read
+        // that nonzero and park a product in an LDS slot no row reads.
+        constexpr int U = kCap / kBlock;
+        const int nu = (k1 - k0 + kBlock - 1) / kBlock;
+        unsigned col[U];
+        V vv[U];
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            col[u] = 0;
+            vv[u] = V(0);
+            xv[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (u < nu) {
+                const int k = min(k0 + tid + u * kBlock, k1 - 1);
+                col[u] = (unsigned)ci[k];
+                vv[u] = val[k];
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (u < nu) xv[u] = x[col[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (u < nu) prod[tid + u * kBlock] = (Acc)vv[u] * (Acc)xv[u];
         __syncthreads();
         const int nrows = r1 - r0;
         int tpr = 1;
