@@ -219,6 +219,82 @@ k_tri_ell_lds(int n_levels, const int* __restrict__ lptr, const int* __restrict_
     }
 }
 
+// ALL_LDS sweep whose levels are all at most kTriThreads rows wide (a 2-D grid
+// up to 1024 wide in natural order): lane t owns row t of every level, and the
+// level loop holds no branch that depends on the lane or on a loop of its own,
+// so the ring loads kTriAhead levels ahead stay in flight across the barriers
+// instead of being waited for at the top of each level.  The result goes out
+// in level order (out_pos[position], one contiguous run per level instead of
+// one cache line per row); k_tri_scatter puts it in row order afterwards.  A
+// lane past the end of its level loads and solves the level's last row again
+// (the same bits as the owner lane: its duplicate store writes the value
+// already there) and parks its LDS copy in a slot the next level does not
+// read.  A missing dependency (kTriNone) reads slot 0 and discards it with a
+// select; the subtraction order is the one of k_tri_ell_lds.
+template <int K, class V, class T, bool UPPER>
+__global__ void __launch_bounds__(kTriThreads)
+k_tri_ell_narrow(int n_levels, const int* __restrict__ lptr, const int* __restrict__ dep,
+                 const V* __restrict__ val, const T* __restrict__ invd, const T* __restrict__ in_pos,
+                 T* __restrict__ out_pos, const int* skip) {
+    __shared__ T buf[2 * kTriLds];
+    if ((skip && *skip) || n_levels <= 0) return;
+    const int tid = threadIdx.x;
+    TriRow<K, V, T> ring[kTriAhead];
+    // levels are never empty, so b - 1 is a row of level lev
+    auto load = [&](TriRow<K, V, T>& R, int lev) {
+        const int a = lptr[lev], b = lptr[lev + 1];
+        const int idx = min(a + tid, b - 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            R.d[k] = dep[(size_t)idx * K + k];
+            R.v[k] = val[(size_t)idx * K + k];
+        }
+        R.i = idx;  // position in level order
+        R.x = in_pos[idx];
+        if (UPPER) R.dv = invd[idx];
+    };
+    const int last = n_levels - 1;
+    auto step = [&](TriRow<K, V, T>& R, int lev) {
+        T* cur = buf + (lev & 1) * kTriLds;
+        const T* prev = buf + ((lev & 1) ^ 1) * kTriLds;
+        T s = R.x;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const bool has = R.d[k] != kTriNone;
+            const T yj = prev[has ? -R.d[k] - 1 : 0];
+            const T t = s - (T)R.v[k] * yj;
+            s = has ? t : s;
+        }
+        if (UPPER) s = s * R.dv;
+        out_pos[R.i] = s;
+        cur[tid] = s;
+        load(R, min(lev + kTriAhead, last));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+#pragma unroll
+    for (int r = 0; r < kTriAhead; ++r) load(ring[r], min(r, last));
+    // whole rounds of kTriAhead levels with no exit inside (every path into
+    // the loop header then has the same loads in flight), the rest after
+    const int whole = n_levels - n_levels % kTriAhead;
+    for (int lev0 = 0; lev0 < whole; lev0 += kTriAhead) {
+#pragma unroll
+        for (int r = 0; r < kTriAhead; ++r) step(ring[r], lev0 + r);  // constant ring slots
+    }
+#pragma unroll
+    for (int r = 0; r < kTriAhead - 1; ++r)
+        if (whole + r < n_levels) step(ring[r], whole + r);
+}
+
+// out[order[p]] = out_pos[p]: a narrow sweep's result back in row order.
+template <class T>
+__global__ void k_tri_scatter(int n, const int* __restrict__ order, const T* __restrict__ out_pos, T* __restrict__ out,
+                              const int* skip) {
+    if (skip && *skip) return;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) out[order[p]] = out_pos[p];
+}
+
 // in_pos[p] = in[order[p]]: a sweep's input in level order.
 template <class T>
 __global__ void k_tri_gather(int n, const int* __restrict__ order, const T* __restrict__ in, T* __restrict__ in_pos,
@@ -363,20 +439,34 @@ static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {
     f->tmp32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
     f->pos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
     f->pos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
+    f->opos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
+    f->opos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
     f->lu = to_dev(M.ctx, std::move(F));
     M.ilu = std::move(f);
 }
 
 template <int K, class T, bool UPPER>
-static void tri_ell_launch(const TriSchedule& S, const TriEll& E, const T* in, T* out, T* in_pos, hipStream_t s,
-                           const int* skip) {
+static void tri_ell_launch(const TriSchedule& S, const TriEll& E, const T* in, T* out, T* in_pos, T* out_pos,
+                           hipStream_t s, const int* skip) {
     const bool dbl = std::is_same<T, double>::value;
     const T* invd = UPPER ? (dbl ? (const T*)E.invd64.p : (const T*)E.invd32.p) : nullptr;
     const int n = S.n_levels ? S.h_level_ptr.back() : 0;
     if (E.all_lds) {
         hipLaunchKernelGGL(k_tri_gather<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, S.order.as<int>(), in, in_pos,
                            skip);
-        if (dbl)
+        if (S.max_level_width <= kTriThreads) {
+            if (dbl)
+                hipLaunchKernelGGL((k_tri_ell_narrow<K, double, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s,
+                                   S.n_levels, S.level_ptr.as<int>(), E.dep.as<int>(), E.val64.as<double>(), invd,
+                                   (const T*)in_pos, out_pos, skip);
+            else
+                hipLaunchKernelGGL((k_tri_ell_narrow<K, float, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s,
+                                   S.n_levels, S.level_ptr.as<int>(), E.dep.as<int>(), E.val32.as<float>(), invd,
+                                   (const T*)in_pos, out_pos, skip);
+            HSPS_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_tri_scatter<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, S.order.as<int>(),
+                               (const T*)out_pos, out, skip);
+        } else if (dbl)
             hipLaunchKernelGGL((k_tri_ell_lds<K, double, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s, S.n_levels,
                                S.level_ptr.as<int>(), S.order.as<int>(), E.dep.as<int>(), E.val64.as<double>(), invd,
                                (const T*)in_pos, out, skip);
@@ -396,11 +486,11 @@ static void tri_ell_launch(const TriSchedule& S, const TriEll& E, const T* in, T
     HSPS_CHECK_LAUNCH();
 }
 template <class T, bool UPPER>
-static void tri_ell_sweep(const TriSchedule& S, const TriEll& E, const T* in, T* out, T* in_pos, hipStream_t s,
-                          const int* skip) {
-    if (E.K == 2) tri_ell_launch<2, T, UPPER>(S, E, in, out, in_pos, s, skip);
-    else if (E.K == 4) tri_ell_launch<4, T, UPPER>(S, E, in, out, in_pos, s, skip);
-    else tri_ell_launch<8, T, UPPER>(S, E, in, out, in_pos, s, skip);
+static void tri_ell_sweep(const TriSchedule& S, const TriEll& E, const T* in, T* out, T* in_pos, T* out_pos,
+                          hipStream_t s, const int* skip) {
+    if (E.K == 2) tri_ell_launch<2, T, UPPER>(S, E, in, out, in_pos, out_pos, s, skip);
+    else if (E.K == 4) tri_ell_launch<4, T, UPPER>(S, E, in, out, in_pos, out_pos, s, skip);
+    else tri_ell_launch<8, T, UPPER>(S, E, in, out, in_pos, out_pos, s, skip);
 }
 
 static void alloc_level_work(Ctx* c, AmgLevelDev& L, int n) {
@@ -562,8 +652,9 @@ static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int
             const T* invd = dbl ? (const T*)f.inv_diag64.p : (const T*)f.inv_diag32.p;
             if (f.fwd_ell.K && f.bwd_ell.K) {
                 T* ip = dbl ? (T*)f.pos64.p : (T*)f.pos32.p;
-                tri_ell_sweep<T, false>(f.fwd, f.fwd_ell, r, y, ip, s, skip);
-                tri_ell_sweep<T, true>(f.bwd, f.bwd_ell, (const T*)y, z, ip, s, skip);
+                T* op = dbl ? (T*)f.opos64.p : (T*)f.opos32.p;
+                tri_ell_sweep<T, false>(f.fwd, f.fwd_ell, r, y, ip, op, s, skip);
+                tri_ell_sweep<T, true>(f.bwd, f.bwd_ell, (const T*)y, z, ip, op, s, skip);
             } else if (dbl) {
                 hipLaunchKernelGGL((k_tri_lower<double, T>), dim3(1), dim3(kTriThreads), 0, s, f.fwd.n_levels,
                                    f.fwd.level_ptr.as<int>(), f.fwd.order.as<int>(), f.lu->rp.as<int>(),

@@ -42,6 +42,7 @@ struct IluFactor {
     TriEll fwd_ell, bwd_ell;
     DevBuf tmp64, tmp32;      // y of L y = r
     DevBuf pos64, pos32;      // a sweep's input in level order
+    DevBuf opos64, opos32;    // a narrow sweep's output in level order
 };
 
 struct AmgLevelDev {

@@ -281,6 +281,26 @@ def test_ilu_apply_matches_oracle(H, ctx):
         assert np.linalg.norm(zd.download() - zo) <= 1e-12 * np.linalg.norm(zo)
 
 
+@pytest.mark.parametrize("n", [23, 25, 1024, 1030])
+def test_ilu_apply_level_shapes(H, ctx, n):
+    """ILU(0) of an n x n convection-diffusion grid in natural order: the
+    sweeps run 2n - 1 anti-diagonal levels up to n rows wide.  n = 23 / 25
+    leave 0 / 1 levels after the last whole round of three prefetched levels
+    (n = 24 above leaves 2); n = 1024 fills the 1024-lane sweep at its widest
+    level; n = 1030 exceeds it (the sweep that loops over a level's rows)."""
+    A = O.Csr.convdiff(n, 0.05, 1.0, 0.5)
+    Ad = dev_csr(H, ctx, A)
+    M = H.Precond(Ad, "ilu", H.default_params(drop_tol=0.0))
+    r = O.uniform(11, A.rows)
+    rd, zd = ctx.array(r), ctx.array(n=A.rows)
+    M.apply(rd, zd)
+    z = zd.download()
+    zo = O.precond_apply(A, r, "ilu", drop_tol=0.0)
+    assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo)
+    M.apply(rd, zd)  # a second apply reuses the level-ordered buffers
+    assert np.array_equal(zd.download(), z)
+
+
 def test_reference_unit_known_answers(H, ctx):
     """tests/test_solvers.cpp:42-76, 151-167 of the reference."""
     A = O.Csr.from_coo(1, 1, [0], [0], [2.0])
