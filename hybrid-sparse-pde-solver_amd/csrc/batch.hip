@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -82,13 +83,48 @@ enum : int { BR_NONE = 0, BR_RELIABLE = 1, BR_CHECK = 2, BR_RESTART = 3 };
 enum : int { SK_CSR = 0, SK_DIA = 1, SK_STENCIL = 2 };
 
 struct BState {
-    double rho, rho_old, alpha, beta, bnorm, rel, rmax, true_rel;
-    int iter, halt, done, flag, reason, reliable, restarts, conv;
+    // the fields every K1 / K2 workgroup reads come first: one 32-byte load (BHot)
+    double alpha, beta;
+    int halt, flag;
     int pcur;  // which of the two p buffers holds the current search direction
+    int iter;
+    double rho, rho_old, bnorm, rel, rmax, true_rel;
+    int done, reason, reliable, restarts, conv;
     double r0;  // initial ||r|| (absolute)
+};
+struct BHot {
+    double alpha, beta;
+    int halt, flag, pcur, iter;
+};
+static_assert(offsetof(BState, alpha) == offsetof(BHot, alpha) && offsetof(BState, iter) == offsetof(BHot, iter),
+              "BHot is the head of BState");
+// The head of a system's state in one load (the compiler would otherwise
+// issue each field's load after the branch on the one before).
+__device__ __forceinline__ BHot load_hot(const BState* S) {
+    BHot h;
+    __builtin_memcpy(&h, S, sizeof(BHot));
+    return h;
+}
+
+// Per-tile record of the 16-byte five-point path (a stencil-stored system with
+// at most five diagonals, a scalar Jacobi diagonal and a staging window that
+// fits the launch's LDS): everything K1 / K2 read before their first vector
+// load, in one 128-byte record, so that a workgroup waits for two dependent
+// scalar loads (this record, then its system's state) instead of one per
+// table.  vec = 0 sends the tile down the general paths, which read the
+// tables themselves.
+template <class T>
+struct alignas(128) TileVec {
+    int r0, r1, sys, vec;  // rows [r0, r1) of internal system sys
+    int row0, last, H, C;  // the system's rows [row0, last]; staging halo and chunk (stage_halo / stage_chunk)
+    int off[5];            // diagonal offsets, ascending (padding: offset 0 with value 0, never present)
+    T uv[5];               // the diagonals' values
+    T cd;                  // D^-1
 };
 
 struct BArgs {
+    const TileVec<float>* tv32;
+    const TileVec<double>* tv64;
     const int* rp;
     const int* ci;
     const double* v64;
@@ -131,6 +167,7 @@ struct BArgs {
 
 template <class T> struct Prec;
 template <> struct Prec<double> {
+    static __device__ const TileVec<double>* tv(const BArgs& B) { return B.tv64; }
     static __device__ const double* val(const BArgs& B) { return B.v64; }
     static __device__ const double* dinv(const BArgs& B) { return B.dinv64; }
     static __device__ const double* dia(const BArgs& B) { return B.dia64; }
@@ -138,6 +175,7 @@ template <> struct Prec<double> {
     static __device__ const double* cd(const BArgs& B) { return B.cd64; }
 };
 template <> struct Prec<float> {
+    static __device__ const TileVec<float>* tv(const BArgs& B) { return B.tv32; }
     static __device__ const float* val(const BArgs& B) { return B.v32; }
     static __device__ const float* dinv(const BArgs& B) { return B.dinv32; }
     static __device__ const float* dia(const BArgs& B) { return B.dia32; }
@@ -314,15 +352,18 @@ __device__ __forceinline__ unsigned mask_word(const unsigned char* m, int g) {
 
 // Row i of a five-point stencil system from x staged in LDS (lds[j - base]
 // holds x_j), presence bits mb: the same sum, in the same order and rounding,
-// as Stencil::row<5>.
+// as Stencil::row<5>.  Every index stays inside the staging window (|off| <=
+// halo <= H and the rows lie in [a0, c1 + G - 1)); a neighbour outside the
+// system (whose LDS slot may hold anything) has its presence bit clear and is
+// selected away, so no index needs clamping and a group's rows read
+// consecutive LDS words.
 template <class T>
-__device__ __forceinline__ T row_lds(const Stencil<T>& A, int i, unsigned mb, lds_ptr<T> lds, int base) {
+__device__ __forceinline__ T row_lds(const TileVec<T>& D, int i, unsigned mb, lds_ptr<T> lds, int base) {
 #pragma clang fp contract(off)
-    const int last = A.row0 + A.n_s - 1;
     T sum = 0;
 #pragma unroll
     for (int d = 0; d < 5; ++d) {
-        const T t = A.uv[d] * lds[min(max(i + A.off[d], A.row0), last) - base];
+        const T t = D.uv[d] * lds[i + D.off[d] - base];
         sum = ((mb >> d) & 1u) ? sum + t : sum;
     }
     return sum;
@@ -332,44 +373,50 @@ __device__ __forceinline__ T row_lds(const Stencil<T>& A, int i, unsigned mb, ld
 // multiple of G) whose staged window [g0, g1) (groups aligned to G, rows
 // inside the system) holds every column the chunk's rows touch.
 template <class T>
-struct VecWin {
+struct VecChunk {
     static constexpr int G = Pack<T>::G;
-    int H, C;
-    bool fits;
-    __device__ __forceinline__ VecWin(const Stencil<T>& A, int cap) {
-        H = stage_halo(A.halo, (int)sizeof(T));
-        C = stage_chunk(A.halo, (int)sizeof(T));
-        fits = stage_need(A.halo, (int)sizeof(T)) <= cap;
+    int c0, c1, a0, base, g0, g1;
+    __device__ __forceinline__ VecChunk(const TileVec<T>& D, int c) {
+        c0 = c;
+        c1 = min(D.r1, c + D.C);
+        a0 = c0 & ~(G - 1);
+        base = a0 - D.H;
+        g0 = max(base, D.row0 & ~(G - 1));
+        g1 = min(c1 + D.H, D.last + 1);
     }
-    __device__ __forceinline__ bool ok() const { return fits; }
 };
+constexpr int kPreM = 4;  // K1: presence words per lane loaded with the staging loads
 
 // K1 rows [r0, r1) of a five-point stencil system: p_new = dv r + beta p_old
 // by groups into LDS (and into pn for the chunk's own rows), then
 // q = A p_new from LDS and pq += p_new . q.  q is not stored: K2 recomputes it.
+// The presence words of a lane's first kPreM groups are loaded together with
+// the staging loads, so the stencil phase after the barrier waits on LDS only.
 template <class T>
-__device__ __forceinline__ void k1_stencil_vec(const Stencil<T>& A, int r0, int r1, lds_ptr<T> lds, int cap,
-                                               const T* __restrict__ r, const T* __restrict__ po,
-                                               T* __restrict__ pn, T dv, double beta, double& pq) {
+__device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsigned char* __restrict__ mask,
+                                               lds_ptr<T> lds, const T* __restrict__ r, const T* __restrict__ po,
+                                               T* __restrict__ pn, double beta, double& pq) {
     constexpr int G = Pack<T>::G;
-    const VecWin<T> W(A, cap);
     const int tid = threadIdx.x;
-    const int lo = A.row0, hi = A.row0 + A.n_s;
-    for (int c0 = r0; c0 < r1; c0 += W.C) {
-        const int c1 = min(r1, c0 + W.C);
-        const int a0 = c0 & ~(G - 1);
-        const int base = a0 - W.H;
-        const int g0 = max(base, lo & ~(G - 1)), g1 = min(c1 + W.H, hi);
+    const T dv = D.cd;
+    for (int c = D.r0; c < D.r1; c += D.C) {
+        const VecChunk<T> W(D, c);
         __syncthreads();  // the previous chunk's readers are done with lds
+        unsigned mw[kPreM];
+#pragma unroll
+        for (int k = 0; k < kPreM; ++k) {
+            const int g = W.a0 + (tid + k * kBlock) * G;
+            mw[k] = g < W.c1 ? mask_word<T>(mask, g) : 0u;
+        }
         // kStageU groups per lane: every load is issued before the first
         // store (vmcnt counts loads and stores in one in-order queue, so a
         // load wait placed after a store also waits for that store)
-        for (int gb = g0 + tid * G; gb < g1; gb += kStageU * kBlock * G) {
+        for (int gb = W.g0 + tid * G; gb < W.g1; gb += kStageU * kBlock * G) {
             Pack<T> rv[kStageU], pv[kStageU];
 #pragma unroll
             for (int u = 0; u < kStageU; ++u) {
                 const int g = gb + u * kBlock * G;
-                if (g < g1) {
+                if (g < W.g1) {
                     rv[u] = ld16(r + g);
                     pv[u] = ld16(po + g);
                 }
@@ -377,32 +424,39 @@ __device__ __forceinline__ void k1_stencil_vec(const Stencil<T>& A, int r0, int 
 #pragma unroll
             for (int u = 0; u < kStageU; ++u) {
                 const int g = gb + u * kBlock * G;
-                if (g >= g1) break;
+                if (g >= W.g1) break;
                 Pack<T> v;
 #pragma unroll
                 for (int e = 0; e < G; ++e) v.e[e] = (T)((double)(T)(dv * rv[u].e[e]) + beta * (double)pv[u].e[e]);
 #pragma unroll
-                for (int e = 0; e < G; ++e) lds[g - base + e] = v.e[e];
-                if (g >= c0 && g + G <= c1) {
+                for (int e = 0; e < G; ++e) lds[g - W.base + e] = v.e[e];
+                if (g >= W.c0 && g + G <= W.c1) {
                     st16_stream(pn + g, v);
                 } else {
 #pragma unroll
                     for (int e = 0; e < G; ++e)
-                        if (g + e >= c0 && g + e < c1) pn[g + e] = v.e[e];
+                        if (g + e >= W.c0 && g + e < W.c1) pn[g + e] = v.e[e];
                 }
             }
         }
         __syncthreads();
-        for (int g = a0 + tid * G; g < c1; g += kBlock * G) {
-            const unsigned mw = mask_word<T>(A.mask, g);
+        auto rows = [&](int g, unsigned m) {
 #pragma unroll
             for (int e = 0; e < G; ++e) {
                 const int i = g + e;
-                if (i >= c0 && i < c1) {
-                    const T sum = row_lds(A, i, (mw >> (8 * e)) & 0xffu, lds, base);
-                    pq += (double)lds[i - base] * (double)sum;
+                if (i >= W.c0 && i < W.c1) {
+                    const T sum = row_lds(D, i, (m >> (8 * e)) & 0xffu, lds, W.base);
+                    pq += (double)lds[i - W.base] * (double)sum;
                 }
             }
+        };
+        int k = 0;
+#pragma clang loop unroll(disable)
+        for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G, ++k) {
+            unsigned m;
+            if (k < kPreM) m = k == 0 ? mw[0] : k == 1 ? mw[1] : k == 2 ? mw[2] : mw[3];
+            else m = mask_word<T>(mask, g);
+            rows(g, m);
         }
     }
 }
@@ -411,50 +465,47 @@ __device__ __forceinline__ void k1_stencil_vec(const Stencil<T>& A, int r0, int 
 // staged in LDS, then x += alpha p (x_lo in mixed mode), r -= alpha q, with
 // r.r and r.z (z = dv r) partials; own-row vectors by 16-byte groups.
 template <class T, bool MIXED>
-__device__ __forceinline__ void k2_stencil_vec(const Stencil<T>& A, int r0, int r1, lds_ptr<T> lds, int cap,
-                                               const T* __restrict__ p, T* __restrict__ r, double* __restrict__ x64,
-                                               float* __restrict__ xlo, T dv, double alpha, double (&red)[2]) {
+__device__ __forceinline__ void k2_stencil_vec(const TileVec<T>& D, const unsigned char* __restrict__ mask,
+                                               lds_ptr<T> lds, const T* __restrict__ p, T* __restrict__ r,
+                                               double* __restrict__ x64, float* __restrict__ xlo, double alpha,
+                                               double (&red)[2]) {
     constexpr int G = Pack<T>::G;
     using X = std::conditional_t<MIXED, float, double>;  // the x part K2 updates
     static_assert(Pack<X>::G == G, "x and the Krylov vectors group alike");
-    const VecWin<T> W(A, cap);
     const int tid = threadIdx.x;
-    const int lo = A.row0, hi = A.row0 + A.n_s;
+    const T dv = D.cd;
     X* __restrict__ xs = MIXED ? (X*)xlo : (X*)x64;
-    for (int c0 = r0; c0 < r1; c0 += W.C) {
-        const int c1 = min(r1, c0 + W.C);
-        const int a0 = c0 & ~(G - 1);
-        const int base = a0 - W.H;
-        const int g0 = max(base, lo & ~(G - 1)), g1 = min(c1 + W.H, hi);
+    for (int c = D.r0; c < D.r1; c += D.C) {
+        const VecChunk<T> W(D, c);
         __syncthreads();
-        for (int g = g0 + tid * G; g < g1; g += kBlock * G) {
+        for (int g = W.g0 + tid * G; g < W.g1; g += kBlock * G) {
             const Pack<T> v = ld16(p + g);
 #pragma unroll
-            for (int e = 0; e < G; ++e) lds[g - base + e] = v.e[e];
+            for (int e = 0; e < G; ++e) lds[g - W.base + e] = v.e[e];
         }
         __syncthreads();
-        for (int g = a0 + tid * G; g < c1; g += kBlock * G) {
-            const bool full = g >= c0 && g + G <= c1;
-            const unsigned mw = mask_word<T>(A.mask, g);
+        for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G) {
+            const bool full = g >= W.c0 && g + G <= W.c1;
+            const unsigned mw = mask_word<T>(mask, g);
             const Pack<T> rv = ld16(r + g);
             Pack<X> xv;
             if (MIXED || full) {  // x_lo is padded; the caller's x64 only read in full groups
                 xv = ld16(xs + g);
             } else {
 #pragma unroll
-                for (int e = 0; e < G; ++e) xv.e[e] = (g + e >= c0 && g + e < c1) ? xs[g + e] : X(0);
+                for (int e = 0; e < G; ++e) xv.e[e] = (g + e >= W.c0 && g + e < W.c1) ? xs[g + e] : X(0);
             }
             Pack<T> ro;
             Pack<X> xo;
 #pragma unroll
             for (int e = 0; e < G; ++e) {
                 const int i = g + e;
-                const T pi = lds[i - base];
-                const T qi = row_lds(A, i, (mw >> (8 * e)) & 0xffu, lds, base);
+                const T pi = lds[i - W.base];
+                const T qi = row_lds(D, i, (mw >> (8 * e)) & 0xffu, lds, W.base);
                 xo.e[e] = (X)((double)xv.e[e] + alpha * (double)pi);
                 const T ri = (T)((double)rv.e[e] - alpha * (double)qi);
                 ro.e[e] = ri;
-                if (i >= c0 && i < c1) {
+                if (i >= W.c0 && i < W.c1) {
                     const double rd = (double)ri;
                     red[0] += rd * rd;
                     red[1] += rd * (double)(T)(dv * ri);
@@ -466,7 +517,7 @@ __device__ __forceinline__ void k2_stencil_vec(const Stencil<T>& A, int r0, int 
             } else {
 #pragma unroll
                 for (int e = 0; e < G; ++e)
-                    if (g + e >= c0 && g + e < c1) {
+                    if (g + e >= W.c0 && g + e < W.c1) {
                         r[g + e] = ro.e[e];
                         xs[g + e] = xo.e[e];
                     }
@@ -656,6 +707,7 @@ k_b_k1(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ 
     // every access to the staging buffer goes through address-space-3 pointers
     const lds_ptr<double> prod64 = (lds_ptr<double>)stage_lds;
     const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
+    if (Prec<T>::tv(B)[tile].vec) return;  // k_b_k1v runs it
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
@@ -693,9 +745,7 @@ k_b_k1(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ 
             if (kind == SK_DIA) q[i] = sum;
             red[0] += (double)pi * (double)sum;
         };
-        if (kind == SK_STENCIL && A.nd <= 5 && cdok && VecWin<T>(A, cap).ok()) {  // 16-byte groups
-            k1_stencil_vec(A, r0, r1, lds, cap, r, po, pn, cdv, beta, red[0]);
-        } else if (stage_need(A.halo, (int)sizeof(T)) <= cap) {  // p_new staged in LDS, evaluated once per row
+        if (stage_need(A.halo, (int)sizeof(T)) <= cap) {  // p_new staged in LDS, evaluated once per row
             A.dispatch([&](auto nd) {
                 stencil_staged<nd()>(A, r0, r1, lds, cap, pnew, [&](int j, T v) { pn[j] = v; }, out);
             });
@@ -774,6 +824,7 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, cons
        T* __restrict__ r, double* __restrict__ x64, float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
     const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
+    if (Prec<T>::tv(B)[tile].vec) return;  // k_b_k2v runs it
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
@@ -808,9 +859,7 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, cons
         const lds_ptr<T> lds = (lds_ptr<T>)(lds_ptr<double>)stage_lds;
         const int cap = B.stage_doubles * (int)(sizeof(double) / sizeof(T));
         auto pv = [&](int j) { return p[j]; };
-        if (B.dia_kind[s] == SK_STENCIL && A.nd <= 5 && cdok && VecWin<T>(A, cap).ok()) {  // 16-byte groups
-            k2_stencil_vec<T, MIXED>(A, r0, r1, lds, cap, p, r, x64, xlo, cdv, alpha, red);
-        } else if (stage_need(A.halo, (int)sizeof(T)) <= cap) {
+        if (stage_need(A.halo, (int)sizeof(T)) <= cap) {
             A.dispatch([&](auto nd) { stencil_staged<nd()>(A, r0, r1, lds, cap, pv, [](int, T) {}, body); });
         } else {
             for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], A.template row<0>(i, pv));
@@ -818,6 +867,70 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, cons
     } else {
         for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], q[i]);
     }
+    tile_partial<2>(red, B, tile);
+}
+
+// ---- K1 / K2 of the 16-byte five-point path ------------------------------------------------
+// Tiles whose record has vec set run here in every state (the general
+// kernels skip them).  Kernels of their own keep the general paths' register
+// budget out of these, and every value they need before the first vector load
+// comes from the tile record and the system's state.
+// K1: NORMAL / FLUSHED as k_b_k1's stencil path; FLUSH: the FP64 true
+// residual r = b - A (x64 + x_lo) with the rows read straight from memory
+// (Stencil::row<5>, the same sum as k_b_k1's tile_rows64).
+template <class T, bool MIXED>
+__global__ void __launch_bounds__(kBlock)
+k_b_k1v(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, const double* __restrict__ b,
+        const double* __restrict__ x64, const float* __restrict__ xlo) {
+    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
+    const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
+    const TileVec<T> D = Prec<T>::tv(B)[tile];
+    if (!D.vec) return;
+    const BHot h = load_hot(&B.st[D.sys]);
+    const int mode = h.halt ? -1 : h.flag;  // one branch on both fields: their loads issue together
+    if (mode < 0) return;
+    double red[2] = {0.0, 0.0};
+    if (mode == BS_FLUSH) {
+        const Stencil<double> A(B, D.sys);
+        const T dv = D.cd;
+        auto xv = [&](int j) { return MIXED ? x64[j] + (double)xlo[j] : x64[j]; };
+        for (int i = D.r0 + threadIdx.x; i < D.r1; i += kBlock) {
+            const double rh = b[i] - A.template row<5>(i, xv);
+            const T ri = (T)rh;
+            r[i] = ri;
+            red[0] += rh * rh;
+            red[1] += (double)ri * (double)(T)(dv * ri);
+        }
+    } else {
+        k1_stencil_vec(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, r, h.pcur ? p1 : p0, h.pcur ? p0 : p1,
+                       h.beta, red[0]);
+    }
+    tile_partial<2>(red, B, tile);
+}
+
+// K2: NORMAL as k_b_k2's stencil path; FLUSHED: x64 += x_lo, x_lo = 0.
+template <class T, bool MIXED>
+__global__ void __launch_bounds__(kBlock, 8)
+k_b_k2v(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, T* __restrict__ r,
+        double* __restrict__ x64, float* __restrict__ xlo) {
+    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
+    const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
+    const TileVec<T> D = Prec<T>::tv(B)[tile];
+    if (!D.vec) return;
+    const BHot h = load_hot(&B.st[D.sys]);
+    const int mode = h.halt ? -1 : h.flag;  // one branch on both fields: their loads issue together
+    if (mode != BS_NORMAL && mode != BS_FLUSHED) return;
+    if (mode == BS_FLUSHED) {
+        if (MIXED)
+            for (int i = D.r0 + threadIdx.x; i < D.r1; i += kBlock) {
+                x64[i] += (double)xlo[i];
+                xlo[i] = 0.0f;
+            }
+        return;
+    }
+    double red[2] = {0.0, 0.0};
+    k2_stencil_vec<T, MIXED>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, h.pcur ? p1 : p0, r, x64, xlo,
+                             h.alpha, red);
     tile_partial<2>(red, B, tile);
 }
 
@@ -1084,12 +1197,16 @@ struct Batch {
     DevBuf dia_kind, dia_nd, dia_off, dia_base, dia64, dia32, uv64, uv32, umask, cd_ok, cd64, cd32;
     int n_kind[3] = {0, 0, 0};  // systems per SK_* storage
     std::vector<int> h_kind;    // SK_* per internal system
+    // host copies of the storage tables (per internal system) for the tile records
+    std::vector<int> h_nd, h_off, h_cdok;
+    std::vector<double> h_uv, h_cd;
+    std::vector<char> h_vec[2];  // per internal system: the 16-byte five-point path applies ([0] FP64, [1] FP32)
     int stage_doubles[2] = {kStageMinDoubles, kStageMinDoubles};  // K1 / K2 LDS: [0] FP64, [1] FP32
     // tiled schedule: rows per tile for CSR systems (0 = not tuned yet; also
     // capped at kLdsNnz nonzeros) and for diagonal-stored systems
     int tile_rows = 0, tile_rows_dia = 0;
     int T = 0;
-    DevBuf tile_r0, tile_r1, tile_sys, sys_t0, sys_t1;
+    DevBuf tile_r0, tile_r1, tile_sys, sys_t0, sys_t1, tv32, tv64;
     std::vector<int> h_sys_t0, h_sys_t1;
     std::vector<std::pair<int, int>> waves;  // internal system ranges within [0, n_large)
     // per-solve work (reused)
@@ -1114,6 +1231,42 @@ struct Batch {
         graphs.clear();
     }
 };
+
+// One TileVec<T> per tile.  vec is set where the 16-byte five-point path
+// applies: a stencil-stored system with at most five diagonals, a scalar
+// D^-1, and a chunk and halo that fit the precision's LDS window.
+template <class T>
+static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::vector<int>& r1,
+                               const std::vector<int>& sys, DevBuf& out) {
+    const int es = (int)sizeof(T), cap = B.stage_doubles[es == 4 ? 1 : 0] * 8 / es;
+    std::vector<TileVec<T>> tv(std::max<size_t>(r0.size(), 1));
+    std::memset(tv.data(), 0, tv.size() * sizeof(TileVec<T>));
+    for (size_t t = 0; t < r0.size(); ++t) {
+        const int k = sys[t], s = B.order[k];
+        TileVec<T>& d = tv[t];
+        d.r0 = r0[t];
+        d.r1 = r1[t];
+        d.sys = k;
+        d.row0 = B.h_row_off[s];
+        d.last = B.h_row_off[s + 1] - 1;
+        const int nd = B.h_nd[k];
+        int halo = 0;
+        for (int j = 0; j < nd; ++j) halo = std::max(halo, std::abs(B.h_off[(size_t)k * kMaxDiags + j]));
+        d.H = stage_halo(halo, es);
+        d.C = stage_chunk(halo, es);
+        for (int j = 0; j < 5; ++j) {
+            d.off[j] = j < nd ? B.h_off[(size_t)k * kMaxDiags + j] : 0;
+            d.uv[j] = j < nd ? (T)B.h_uv[(size_t)k * kMaxDiags + j] : T(0);
+        }
+        d.cd = (T)B.h_cd[k];
+        d.vec = B.h_kind[k] == SK_STENCIL && nd <= 5 && B.h_cdok[k] && stage_need(halo, es) <= cap;
+    }
+    std::vector<char>& sv = B.h_vec[es == 4 ? 1 : 0];
+    sv.assign(B.S, 0);
+    for (size_t t = 0; t < r0.size(); ++t) sv[sys[t]] = (char)tv[t].vec;
+    out = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
+    HSPS_HIP(hipMemcpy(out.p, tv.data(), tv.size() * sizeof(TileVec<T>), hipMemcpyHostToDevice));
+}
 
 static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     std::vector<int> r0, r1, sys;
@@ -1149,6 +1302,8 @@ static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     up(B.tile_r0, r0);
     up(B.tile_r1, r1);
     up(B.tile_sys, sys);
+    build_tile_records<float>(B, r0, r1, sys, B.tv32);
+    build_tile_records<double>(B, r0, r1, sys, B.tv64);
     up(B.sys_t0, B.h_sys_t0);
     up(B.sys_t1, B.h_sys_t1);
     B.partials = DevBuf(B.ctx, sizeof(double) * 2 * (size_t)std::max(B.T, 1));
@@ -1286,6 +1441,11 @@ static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci
         if (bytes) HSPS_HIP(hipMemcpy(dst.p, src, bytes, hipMemcpyHostToDevice));
     };
     B.h_kind = kind;
+    B.h_nd = nd;
+    B.h_off = offs;
+    B.h_cdok = cdok;
+    B.h_uv = uv;
+    B.h_cd = cd;
     up(B.dia_kind, kind.data(), kind.size() * sizeof(int));
     up(B.dia_nd, nd.data(), nd.size() * sizeof(int));
     up(B.dia_off, offs.data(), offs.size() * sizeof(int));
@@ -1305,6 +1465,8 @@ static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci
 
 static BArgs args_of(const Batch& B, const hsps_params& p) {
     BArgs a{};  // value-initialised: padding is zero, so argument blocks compare bytewise
+    a.tv32 = B.tv32.as<TileVec<float>>();
+    a.tv64 = B.tv64.as<TileVec<double>>();
     a.rp = B.rp.as<int>();
     a.ci = B.ci.as<int>();
     a.v64 = B.v64.as<double>();
@@ -1349,13 +1511,25 @@ static void enqueue_iters(const Batch& B, const BArgs& a, int k0, int k1, int t0
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
     const int ns = k1 - k0;
+    // which kernels the wave needs: the five-point path's, the general ones', or both
+    const std::vector<char>& vec = B.h_vec[sizeof(T) == 4 ? 1 : 0];
+    bool any_vec = false, any_gen = false;
+    for (int k = k0; k < k1; ++k) (vec[k] ? any_vec : any_gen) = true;
     for (int it = 0; it < iters; ++it) {
         const size_t lds = sizeof(double) * (size_t)a.stage_doubles;
-        hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, q, r, b, (const double*)x,
-                           (const float*)xl);
+        if (any_vec)
+            hipLaunchKernelGGL((k_b_k1v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, r, b,
+                               (const double*)x, (const float*)xl);
+        if (any_gen)
+            hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, q, r, b,
+                               (const double*)x, (const float*)xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF1{});
-        hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0, (const T*)p1,
-                           (const T*)q, r, x, xl);
+        if (any_vec)
+            hipLaunchKernelGGL((k_b_k2v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0,
+                               (const T*)p1, r, x, xl);
+        if (any_gen)
+            hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0,
+                               (const T*)p1, (const T*)q, r, x, xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF2<MIXED>{});
     }
     HSPS_CHECK_LAUNCH();
