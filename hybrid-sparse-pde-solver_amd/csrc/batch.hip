@@ -343,7 +343,7 @@ __device__ __forceinline__ void st16_stream(T* p, const Pack<T>& v) {
     for (int e = 0; e < Pack<T>::G; ++e) w[e] = v.e[e];
     __builtin_nontemporal_store(w, reinterpret_cast<typename Vec16<T>::type*>(p));
 }
-constexpr int kStageU = 4;  // K1 staging: groups per lane whose loads are issued together
+constexpr int kStageU = 3;  // staging: groups per lane whose loads are issued together (a 2048-row tile with 512-row halos in one round)
 template <class T>
 __device__ __forceinline__ unsigned mask_word(const unsigned char* m, int g) {
     if constexpr (Pack<T>::G == 4) return *reinterpret_cast<const unsigned*>(m + g);
@@ -430,24 +430,22 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
                 for (int e = 0; e < G; ++e) v.e[e] = (T)((double)(T)(dv * rv[u].e[e]) + beta * (double)pv[u].e[e]);
 #pragma unroll
                 for (int e = 0; e < G; ++e) lds[g - W.base + e] = v.e[e];
-                if (g >= W.c0 && g + G <= W.c1) {
-                    st16_stream(pn + g, v);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < G; ++e)
-                        if (g + e >= W.c0 && g + e < W.c1) pn[g + e] = v.e[e];
-                }
             }
         }
         __syncthreads();
+        // all G rows of a group at once (a row outside [c0, c1) reads inside
+        // the window and is discarded), so their LDS reads overlap
         auto rows = [&](int g, unsigned m) {
+            T sum[G], pv[G];
 #pragma unroll
             for (int e = 0; e < G; ++e) {
-                const int i = g + e;
-                if (i >= W.c0 && i < W.c1) {
-                    const T sum = row_lds(D, i, (m >> (8 * e)) & 0xffu, lds, W.base);
-                    pq += (double)lds[i - W.base] * (double)sum;
-                }
+                sum[e] = row_lds(D, g + e, (m >> (8 * e)) & 0xffu, lds, W.base);
+                pv[e] = lds[g + e - W.base];
+            }
+#pragma unroll
+            for (int e = 0; e < G; ++e) {
+                const double t = fma((double)pv[e], (double)sum[e], pq);
+                pq = g + e >= W.c0 && g + e < W.c1 ? t : pq;
             }
         };
         int k = 0;
@@ -457,6 +455,21 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
             if (k < kPreM) m = k == 0 ? mw[0] : k == 1 ? mw[1] : k == 2 ? mw[2] : mw[3];
             else m = mask_word<T>(mask, g);
             rows(g, m);
+        }
+        // the chunk's own rows of p_new, stored only now: a store issued
+        // before the stencil phase would hold up its first presence-word use
+        // (vmcnt counts loads and stores in one in-order queue)
+        for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G) {
+            Pack<T> v;
+#pragma unroll
+            for (int e = 0; e < G; ++e) v.e[e] = lds[g - W.base + e];
+            if (g >= W.c0 && g + G <= W.c1) {
+                st16_stream(pn + g, v);
+            } else {
+#pragma unroll
+                for (int e = 0; e < G; ++e)
+                    if (g + e >= W.c0 && g + e < W.c1) pn[g + e] = v.e[e];
+            }
         }
     }
 }
@@ -478,10 +491,19 @@ __device__ __forceinline__ void k2_stencil_vec(const TileVec<T>& D, const unsign
     for (int c = D.r0; c < D.r1; c += D.C) {
         const VecChunk<T> W(D, c);
         __syncthreads();
-        for (int g = W.g0 + tid * G; g < W.g1; g += kBlock * G) {
-            const Pack<T> v = ld16(p + g);
+        // staging loads kStageU groups at a time, all in flight before the first LDS store
+        for (int g0 = W.g0 + tid * G; g0 < W.g1; g0 += kStageU * kBlock * G) {
+            Pack<T> v[kStageU];
 #pragma unroll
-            for (int e = 0; e < G; ++e) lds[g - W.base + e] = v.e[e];
+            for (int u = 0; u < kStageU; ++u)
+                if (g0 + u * kBlock * G < W.g1) v[u] = ld16(p + g0 + u * kBlock * G);
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u) {
+                const int g = g0 + u * kBlock * G;
+                if (g >= W.g1) break;
+#pragma unroll
+                for (int e = 0; e < G; ++e) lds[g - W.base + e] = v[u].e[e];
+            }
         }
         __syncthreads();
         for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G) {
@@ -573,24 +595,44 @@ __device__ __forceinline__ void tile_rows64(const BArgs& B, int s, int r0, int r
     }
 }
 
-// Per-system finish: one workgroup per system of [k0, k0+gridDim.x); sums the
-// system's tile partials in tile order and runs op(S, tot) in thread 0.
+// Per-system finish: one wave per system of [k0, k0+gridDim.x).  The
+// system's state and tile range are loaded first (one round trip), then its
+// tile partials; op runs on a register copy of the state in lane 0, which
+// writes it back.  The partials are summed in the order of a 256-thread
+// block_sum: "thread" 64 w + l (w = 0..3) adds tiles t0 + 64 w + l + 256 j,
+// each group of 64 is summed by the xor butterfly of wave_sum, and the four
+// group sums are added in order, so one wave gives the same bits.
+constexpr int kFinishThreads = 64;
 template <int NV, class Op>
-__global__ void __launch_bounds__(kBlock) k_b_finish(BArgs B, int k0, Op op) {
+__global__ void __launch_bounds__(kFinishThreads) k_b_finish(BArgs B, int k0, Op op) {
     const int s = k0 + blockIdx.x;
-    BState* S = &B.st[s];
-    if (!op.active(*S)) return;
     const int t0 = B.sys_t0[s], t1 = B.sys_t1[s];
-    __shared__ double red[NV * (kBlock / 64)];
-    double acc[NV];
+    BState S = B.st[s];
+    if (!op.active(S)) return;
+    const int lane = threadIdx.x;
+    constexpr int W = kBlock / 64;
+    double acc[W][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    for (int w = 0; w < W; ++w) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += B.partials[(size_t)k * B.T + t];
+        for (int k = 0; k < NV; ++k) acc[w][k] = 0.0;
+        for (int t = t0 + 64 * w + lane; t < t1; t += kBlock) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[w][k] += B.partials[(size_t)k * B.T + t];
+        }
     }
-    block_sum<NV>(acc, red);
-    if (threadIdx.x == 0) op(*S, acc, B);
+    double tot[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) sum += wave_sum(acc[w][k]);
+        tot[k] = sum;
+    }
+    if (lane == 0) {
+        op(S, tot, B);
+        B.st[s] = S;
+    }
 }
 
 template <int NV>
@@ -879,7 +921,7 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, cons
 // residual r = b - A (x64 + x_lo) with the rows read straight from memory
 // (Stencil::row<5>, the same sum as k_b_k1's tile_rows64).
 template <class T, bool MIXED>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 8)
 k_b_k1v(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, const double* __restrict__ b,
         const double* __restrict__ x64, const float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
@@ -910,7 +952,7 @@ k_b_k1v(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__
 
 // K2: NORMAL as k_b_k2's stencil path; FLUSHED: x64 += x_lo, x_lo = 0.
 template <class T, bool MIXED>
-__global__ void __launch_bounds__(kBlock, 8)
+__global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 7)
 k_b_k2v(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, T* __restrict__ r,
         double* __restrict__ x64, float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
@@ -969,8 +1011,10 @@ struct OpF2 {
 };
 
 // ---- final: x64 += x_lo and the FP64 true residual per (large) system ----------------------
-__global__ void k_b_flush_range(int a, int e, double* __restrict__ x64, float* __restrict__ xlo) {
-    for (int i = a + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(kBlock)
+k_b_flush_tiles(BArgs B, double* __restrict__ x64, float* __restrict__ xlo) {
+    const int r0 = B.tile_r0[blockIdx.x], r1 = B.tile_r1[blockIdx.x];
+    for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
         x64[i] += (double)xlo[i];
         xlo[i] = 0.0f;
     }
@@ -1523,14 +1567,14 @@ static void enqueue_iters(const Batch& B, const BArgs& a, int k0, int k1, int t0
         if (any_gen)
             hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, q, r, b,
                                (const double*)x, (const float*)xl);
-        hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF1{});
+        hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kFinishThreads), 0, s, a, k0, OpF1{});
         if (any_vec)
             hipLaunchKernelGGL((k_b_k2v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0,
                                (const T*)p1, r, x, xl);
         if (any_gen)
             hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0,
                                (const T*)p1, (const T*)q, r, x, xl);
-        hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kBlock), 0, s, a, k0, OpF2<MIXED>{});
+        hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kFinishThreads), 0, s, a, k0, OpF2<MIXED>{});
     }
     HSPS_CHECK_LAUNCH();
 }
@@ -1621,14 +1665,14 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
     // large systems: tiled waves on the main stream
     if (B.T > 0) {
         hipLaunchKernelGGL(k_b_init1, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x, B.r64.as<double>());
-        hipLaunchKernelGGL((k_b_finish<2, OpInit1>), dim3(B.n_large), dim3(kBlock), 0, s, a, 0, OpInit1{});
+        hipLaunchKernelGGL((k_b_finish<2, OpInit1>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, OpInit1{});
         if (prec == HSPS_MIXED)
             hipLaunchKernelGGL((k_b_init2<float, true>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
                                (const double*)B.r64.as<double>(), B.r.as<float>(), B.p.as<float>(), B.xlo.as<float>());
         else
             hipLaunchKernelGGL((k_b_init2<double, false>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
                                (const double*)B.r64.as<double>(), B.r.as<double>(), B.p.as<double>(), (float*)nullptr);
-        hipLaunchKernelGGL((k_b_finish<1, OpInit2>), dim3(B.n_large), dim3(kBlock), 0, s, a, 0, OpInit2{});
+        hipLaunchKernelGGL((k_b_finish<1, OpInit2>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, OpInit2{});
         HSPS_CHECK_LAUNCH();
     }
     const int chunk = std::max(1, std::min(p.check_every > 0 ? p.check_every : 32, 64));
@@ -1669,17 +1713,10 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
             }
         }
         if (B.T > 0) {
-            if (prec == HSPS_MIXED) {
-                // flush every large system's rows (systems are not contiguous in row order)
-                for (int k = 0; k < B.n_large; ++k) {
-                    const int sid = B.order[k];
-                    const int a0 = B.h_row_off[sid], a1 = B.h_row_off[sid + 1];
-                    hipLaunchKernelGGL(k_b_flush_range, dim3(grid_for(a1 - a0)), dim3(kBlock), 0, s, a0, a1, x,
-                                       B.xlo.as<float>());
-                }
-            }
+            if (prec == HSPS_MIXED)  // every large system's rows, one workgroup per tile
+                hipLaunchKernelGGL(k_b_flush_tiles, dim3(B.T), dim3(kBlock), 0, s, a, x, B.xlo.as<float>());
             hipLaunchKernelGGL(k_b_final, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x);
-            hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(B.n_large), dim3(kBlock), 0, s, a, 0, OpFinal{});
+            hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, OpFinal{});
             HSPS_CHECK_LAUNCH();
         }
         HSPS_HIP(hipStreamWaitEvent(s, B.join, 0));
