@@ -43,11 +43,11 @@ HBM_PEAK_GBS = 8000.0
 # read and write 256 MiB).
 SPMV_FP64_HBM_BYTES = (154831 * 2.0 + 32768 * 1.0) * 1024
 # HBM bytes per row-iteration of the stencil-stored FP32 batched iteration
-# (k_b_k1 + k_b_k2), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
-# `bench.py --only-batched --steps 1 --warmup 0 --tile-rows 2048`, summed over
-# every K1/K2 dispatch of the run and divided by its row-iterations (sum of
-# rows x iterations; profiles/batched512_pcg_jacobi_kernels.md).
-BATCHED_HBM_BYTES_PER_ROW_ITER = 35.6
+# (k_b_k1v + k_b_k2v), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --full-load-only --tile-rows 2048`: the median full-load dispatch
+# of each kernel (factors 2.000 / 1.000 from the 256 MiB copy of the same run)
+# divided by the 21,108,314 tiled rows (profiles/batched512_pcg_jacobi_kernels.md).
+BATCHED_HBM_BYTES_PER_ROW_ITER = 35.1
 
 
 def log(*a):
@@ -195,19 +195,18 @@ def bench_batched(H, ctx, ev, args):
                             np.linalg.norm(xs[row_off[s]:row_off[s + 1]]) for s in range(len(systems))]))
         return times, it, tr, err
 
-    log("[bench] batched mixed")
-    t_mx, it_mx, tr_mx, err_mx = batched("mixed", args.steps, args.warmup)
-    log(f"[bench] batched mixed {np.mean(t_mx):.3f}s")
     storage = B.storage()
     sizes = np.diff(row_off).astype(np.float64)
-    row_iters = float(np.dot(sizes, it_mx.astype(np.float64)))
     # algorithmic bytes of one row-iteration of the stencil-stored FP32 path:
     # K1 reads r, p_old, the presence byte and writes p_new (13 B); K2 reads
     # p, the presence byte, r, x_lo and writes r, x_lo (21 B)
     per_row = 34 if storage["stencil"] == len(systems) else None
     # Full-load iteration: every tiled system (rows > 20,000; the smaller ones run in the persistent
     # one-workgroup kernel on the second stream) is still iterating during the first 96 iterations,
-    # so (t(96) - t(32)) / 64 is one iteration (K1, F1, K2, F2) over all tiled rows.
+    # so (t(96) - t(32)) / 64 is one iteration (K1, F1, K2, F2) over all tiled rows, timed with HIP
+    # events on the stream the batched kernels run on.
+    stream = H.lib().hsps_ctx_stream(ctx.h)
+
     def capped(iters):
         p = H.default_params(tol=1e-8, max_iter=iters, precision="mixed", check_every=32,
                              batch_tile_rows=args.tile_rows)
@@ -215,12 +214,33 @@ def bench_batched(H, ctx, ev, args):
         for _ in range(3):
             H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
             ctx.sync()
-            t0 = time.perf_counter()
+            e0, e1 = ev.event(), ev.event()
+            ev.record(e0, stream)
             B.solve(bd, xd, p)
-            ctx.sync()
-            ts.append(time.perf_counter() - t0)
+            ev.record(e1, stream)
+            ts.append(ev.elapsed_ms(e0, e1) * 1e-3)
         return float(np.median(ts))
     tiled_rows = float(sizes[sizes > 20000].sum())
+    if args.full_load_only:
+        # for rocprofv3 runs: full-load iterations only, plus a 256 MiB device copy that calibrates
+        # the FETCH_SIZE / WRITE_SIZE counters
+        nbytes = 256 << 20
+        src, dst = ctx.array(n=nbytes // 8), ctx.array(n=nbytes // 8)
+        H.check(H.lib().hsps_memcpy_d2d(ctx.h, dst.ptr, src.ptr, nbytes))
+        ctx.sync()
+        capped(32)  # warm-up (tile autotuning happens here when --tile-rows is 0)
+        t_it = (capped(96) - capped(32)) / 64
+        out = {"full_load_iteration_us": 1e6 * t_it, "tiled_rows": tiled_rows, "bytes_per_row_iteration": per_row,
+               "full_load_achieved_GBs": per_row * tiled_rows / t_it / 1e9 if per_row else None,
+               "calibration_copy_bytes": nbytes}
+        for A in mats:
+            A.close()
+        B.close()
+        return out
+    log("[bench] batched mixed")
+    t_mx, it_mx, tr_mx, err_mx = batched("mixed", args.steps, args.warmup)
+    log(f"[bench] batched mixed {np.mean(t_mx):.3f}s")
+    row_iters = float(np.dot(sizes, it_mx.astype(np.float64)))
     t_it = (capped(96) - capped(32)) / 64
     batched_roof = {"bytes_per_row_iteration": per_row, "storage": storage, "tiled_rows": tiled_rows,
                     "full_load_iteration_us": 1e6 * t_it,
@@ -639,6 +659,9 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, help="batched tile rows (0: autotuned on the first solve)")
     ap.add_argument("--only-batched", action="store_true",
                     help="batched mixed solves only (no FP64 or CPU baselines); for rocprofv3 runs")
+    ap.add_argument("--full-load-only", action="store_true",
+                    help="only the batched full-load iterations (capped solves) and a 256 MiB calibration copy; "
+                         "for rocprofv3 runs")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV roofline kernel (plus a 256 MiB device copy that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters); for rocprofv3 runs")
@@ -650,6 +673,11 @@ def main():
     H = load_pkg()
     ev = HipEvents()
     ctx = H.Context(0)
+    if args.full_load_only:
+        res = bench_batched(H, ctx, ev, args)
+        print(json.dumps({"batched_full_load": res}), flush=True)
+        ctx.close()
+        return
     log("[bench] spmv roofline")
     roof, rn, rnnz = spmv_roofline(H, ctx, ev)
     if args.spmv_only:
@@ -683,13 +711,13 @@ def main():
     if bi and bi.get("full_load_achieved_GBs"):
         alg = bi["bytes_per_row_iteration"] * bi["tiled_rows"]
         roofline = {"bound": "hbm",
-                    "kernel": "batched PCG iteration at full load: k_b_k1<float> + k_b_k2<float> + the two "
+                    "kernel": "batched PCG iteration at full load: k_b_k1v<float> + k_b_k2v<float> + the two "
                               f"per-system finish kernels over {bi['tiled_rows']:.0f} stencil-stored rows",
                     "achieved": bi["full_load_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": bi["full_load_achieved_GBs"] / HBM_PEAK_GBS,
                     "traffic": BATCHED_HBM_BYTES_PER_ROW_ITER * bi["tiled_rows"],
-                    "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 summed over the k_b_k1 / "
-                                      "k_b_k2 dispatches of a batched solve, per row-iteration "
+                    "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 of the k_b_k1v / k_b_k2v "
+                                      "dispatches at full load (median per dispatch / tiled rows) "
                                       f"({BATCHED_HBM_BYTES_PER_ROW_ITER:.1f} B vs {bi['bytes_per_row_iteration']} B "
                                       "algorithmic; profiles/batched512_pcg_jacobi_kernels.md)",
                     "algorithmic_bytes": alg, "avg_launch_us": bi["full_load_iteration_us"],
