@@ -112,12 +112,15 @@ __device__ __forceinline__ BHot load_hot(const BState* S) {
 // load, in one 128-byte record, so that a workgroup waits for two dependent
 // scalar loads (this record, then its system's state) instead of one per
 // table.  vec = 0 sends the tile down the general paths, which read the
-// tables themselves.
+// tables themselves.  The iteration kernels read the records of the tiles
+// whose systems are still iterating from a compacted copy (BArgs::tva*),
+// which the host rebuilds as systems halt.
 template <class T>
 struct alignas(128) TileVec {
     int r0, r1, sys, vec;  // rows [r0, r1) of internal system sys
     int row0, last, H, C;  // the system's rows [row0, last]; staging halo and chunk (stage_halo / stage_chunk)
     int off[5];            // diagonal offsets, ascending (padding: offset 0 with value 0, never present)
+    int tile;              // the tile's index (its partials slot)
     T uv[5];               // the diagonals' values
     T cd;                  // D^-1
 };
@@ -125,6 +128,9 @@ struct alignas(128) TileVec {
 struct BArgs {
     const TileVec<float>* tv32;
     const TileVec<double>* tv64;
+    const TileVec<float>* tva32;   // records of the active tiles (one per workgroup of K1 / K2)
+    const TileVec<double>* tva64;
+    const int* act_sys;            // the active systems (one per workgroup of F1 / F2)
     const int* rp;
     const int* ci;
     const double* v64;
@@ -168,6 +174,7 @@ struct BArgs {
 template <class T> struct Prec;
 template <> struct Prec<double> {
     static __device__ const TileVec<double>* tv(const BArgs& B) { return B.tv64; }
+    static __device__ const TileVec<double>* tva(const BArgs& B) { return B.tva64; }
     static __device__ const double* val(const BArgs& B) { return B.v64; }
     static __device__ const double* dinv(const BArgs& B) { return B.dinv64; }
     static __device__ const double* dia(const BArgs& B) { return B.dia64; }
@@ -176,6 +183,7 @@ template <> struct Prec<double> {
 };
 template <> struct Prec<float> {
     static __device__ const TileVec<float>* tv(const BArgs& B) { return B.tv32; }
+    static __device__ const TileVec<float>* tva(const BArgs& B) { return B.tva32; }
     static __device__ const float* val(const BArgs& B) { return B.v32; }
     static __device__ const float* dinv(const BArgs& B) { return B.dinv32; }
     static __device__ const float* dia(const BArgs& B) { return B.dia32; }
@@ -595,7 +603,8 @@ __device__ __forceinline__ void tile_rows64(const BArgs& B, int s, int r0, int r
     }
 }
 
-// Per-system finish: one wave per system of [k0, k0+gridDim.x).  The
+// Per-system finish: one wave per system of [k0, k0+gridDim.x), or per entry
+// of sys when given (the active systems of an iteration).  The
 // system's state and tile range are loaded first (one round trip), then its
 // tile partials; op runs on a register copy of the state in lane 0, which
 // writes it back.  The partials are summed in the order of a 256-thread
@@ -604,8 +613,8 @@ __device__ __forceinline__ void tile_rows64(const BArgs& B, int s, int r0, int r
 // group sums are added in order, so one wave gives the same bits.
 constexpr int kFinishThreads = 64;
 template <int NV, class Op>
-__global__ void __launch_bounds__(kFinishThreads) k_b_finish(BArgs B, int k0, Op op) {
-    const int s = k0 + blockIdx.x;
+__global__ void __launch_bounds__(kFinishThreads) k_b_finish(BArgs B, int k0, const int* __restrict__ sys, Op op) {
+    const int s = sys ? sys[blockIdx.x] : k0 + blockIdx.x;
     const int t0 = B.sys_t0[s], t1 = B.sys_t1[s];
     BState S = B.st[s];
     if (!op.active(S)) return;
@@ -743,13 +752,14 @@ struct OpInit2 {
 // FLUSH: FP64 true residual r = b - A (x64 + x_lo).
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
-k_b_k1(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
+k_b_k1(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
        const double* __restrict__ b, const double* __restrict__ x64, const float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
     // every access to the staging buffer goes through address-space-3 pointers
     const lds_ptr<double> prod64 = (lds_ptr<double>)stage_lds;
-    const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
-    if (Prec<T>::tv(B)[tile].vec) return;  // k_b_k1v runs it
+    const TileVec<T>& D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    if (D.vec) return;  // k_b_k1v runs it
+    const int tile = D.tile;
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
@@ -862,11 +872,12 @@ struct OpF1 {
 // systems), r.r / r.z partials.  FLUSHED: x64 += x_lo, x_lo = 0.
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock)
-k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, const T* __restrict__ q,
+k_b_k2(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __restrict__ q,
        T* __restrict__ r, double* __restrict__ x64, float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
-    const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
-    if (Prec<T>::tv(B)[tile].vec) return;  // k_b_k2v runs it
+    const TileVec<T>& D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    if (D.vec) return;  // k_b_k2v runs it
+    const int tile = D.tile;
     const int s = B.tile_sys[tile];
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
@@ -922,11 +933,11 @@ k_b_k2(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, cons
 // (Stencil::row<5>, the same sum as k_b_k1's tile_rows64).
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock, 8)
-k_b_k1v(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, const double* __restrict__ b,
+k_b_k1v(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, const double* __restrict__ b,
         const double* __restrict__ x64, const float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
-    const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
-    const TileVec<T> D = Prec<T>::tv(B)[tile];
+    const TileVec<T> D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    const int tile = D.tile;
     if (!D.vec) return;
     const BHot h = load_hot(&B.st[D.sys]);
     const int mode = h.halt ? -1 : h.flag;  // one branch on both fields: their loads issue together
@@ -953,11 +964,11 @@ k_b_k1v(BArgs B, int t0, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__
 // K2: NORMAL as k_b_k2's stencil path; FLUSHED: x64 += x_lo, x_lo = 0.
 template <class T, bool MIXED>
 __global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 7)
-k_b_k2v(BArgs B, int t0, const T* __restrict__ p0, const T* __restrict__ p1, T* __restrict__ r,
+k_b_k2v(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, T* __restrict__ r,
         double* __restrict__ x64, float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
-    const int tile = t0 + xcd_tile(blockIdx.x, gridDim.x);
-    const TileVec<T> D = Prec<T>::tv(B)[tile];
+    const TileVec<T> D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    const int tile = D.tile;
     if (!D.vec) return;
     const BHot h = load_hot(&B.st[D.sys]);
     const int mode = h.halt ? -1 : h.flag;  // one branch on both fields: their loads issue together
@@ -1251,6 +1262,15 @@ struct Batch {
     int tile_rows = 0, tile_rows_dia = 0;
     int T = 0;
     DevBuf tile_r0, tile_r1, tile_sys, sys_t0, sys_t1, tv32, tv64;
+    // active-tile compaction: host copies of the records, the compacted device
+    // copies K1 / K2 read, the active systems F1 / F2 read, and two pinned
+    // staging buffers (an upload never overwrites the one still being copied)
+    std::vector<TileVec<float>> h_tv32;
+    std::vector<TileVec<double>> h_tv64;
+    DevBuf tva32, tva64, act_sys;
+    void* pin_act[2] = {nullptr, nullptr};
+    size_t pin_act_bytes = 0;
+    int pin_next = 0;
     std::vector<int> h_sys_t0, h_sys_t1;
     std::vector<std::pair<int, int>> waves;  // internal system ranges within [0, n_large)
     // per-solve work (reused)
@@ -1259,12 +1279,18 @@ struct Batch {
     // a captured chunk replays the kernel arguments it was captured with
     // (tolerance, iteration cap, tiles, ...), so the whole argument block is
     // part of its key
-    struct G { hipGraphExec_t exec = nullptr; int wave = -1, prec = -1, chunk = 0; BArgs args{}; };
+    struct G {
+        hipGraphExec_t exec = nullptr;
+        int wave = -1, prec = -1, chunk = 0, nt = 0, ns = 0, kinds = 0;
+        BArgs args{};
+    };
     std::vector<G> graphs;
     const double* graph_b = nullptr;  // the b and x the captured kernels point at
     const double* graph_x = nullptr;
     ~Batch() {
         drop_graphs();
+        for (void* q : pin_act)
+            if (q) (void)hipHostFree(q);
         if (join) (void)hipEventDestroy(join);
         if (fork) (void)hipEventDestroy(fork);
         if (aux) (void)hipStreamDestroy(aux);
@@ -1291,6 +1317,7 @@ static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::
         d.r0 = r0[t];
         d.r1 = r1[t];
         d.sys = k;
+        d.tile = (int)t;
         d.row0 = B.h_row_off[s];
         d.last = B.h_row_off[s + 1] - 1;
         const int nd = B.h_nd[k];
@@ -1310,6 +1337,9 @@ static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::
     for (size_t t = 0; t < r0.size(); ++t) sv[sys[t]] = (char)tv[t].vec;
     out = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
     HSPS_HIP(hipMemcpy(out.p, tv.data(), tv.size() * sizeof(TileVec<T>), hipMemcpyHostToDevice));
+    (es == 4 ? B.tva32 : B.tva64) = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
+    if constexpr (sizeof(T) == 4) B.h_tv32 = std::move(tv);
+    else B.h_tv64 = std::move(tv);
 }
 
 static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
@@ -1348,6 +1378,18 @@ static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     up(B.tile_sys, sys);
     build_tile_records<float>(B, r0, r1, sys, B.tv32);
     build_tile_records<double>(B, r0, r1, sys, B.tv64);
+    B.act_sys = DevBuf(B.ctx, sizeof(int) * (size_t)std::max(B.S, 1));
+    static_assert(sizeof(TileVec<float>) == sizeof(TileVec<double>), "one staging size for both precisions");
+    const size_t pb = sizeof(TileVec<double>) * (size_t)std::max(B.T, 1) + sizeof(int) * (size_t)std::max(B.S, 1);
+    if (pb > B.pin_act_bytes) {
+        for (void*& q : B.pin_act) {
+            if (q) HSPS_HIP(hipHostFree(q));
+            q = nullptr;
+        }
+        B.pin_act_bytes = 0;
+        for (void*& q : B.pin_act) HSPS_HIP(hipHostMalloc(&q, pb, hipHostMallocDefault));
+        B.pin_act_bytes = pb;
+    }
     up(B.sys_t0, B.h_sys_t0);
     up(B.sys_t1, B.h_sys_t1);
     B.partials = DevBuf(B.ctx, sizeof(double) * 2 * (size_t)std::max(B.T, 1));
@@ -1511,6 +1553,9 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     BArgs a{};  // value-initialised: padding is zero, so argument blocks compare bytewise
     a.tv32 = B.tv32.as<TileVec<float>>();
     a.tv64 = B.tv64.as<TileVec<double>>();
+    a.tva32 = B.tva32.as<TileVec<float>>();
+    a.tva64 = B.tva64.as<TileVec<double>>();
+    a.act_sys = B.act_sys.as<int>();
     a.rp = B.rp.as<int>();
     a.ci = B.ci.as<int>();
     a.v64 = B.v64.as<double>();
@@ -1546,35 +1591,68 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     return a;
 }
 
+// The active set of a wave: the tiles K1 / K2 run (their records, compacted)
+// and the systems F1 / F2 run.  kinds: bit 0 = some tile takes the
+// five-point kernels, bit 1 = some tile takes the general ones.
+struct Active {
+    int nt = 0, ns = 0, kinds = 0;
+};
+
+// Stages the records of the tiles of the live systems of [k0, k1) (live =
+// nullptr: all of them) and the systems themselves, and copies both to the
+// device on stream s behind the work already queued there.
+template <class T>
+static Active upload_active(Batch& B, int k0, int k1, const std::vector<char>* live, hipStream_t s) {
+    const std::vector<TileVec<T>>& tv = sizeof(T) == 4 ? (const std::vector<TileVec<T>>&)B.h_tv32
+                                                       : (const std::vector<TileVec<T>>&)B.h_tv64;
+    char* base = (char*)B.pin_act[B.pin_next];
+    B.pin_next ^= 1;
+    TileVec<T>* rec = (TileVec<T>*)base;
+    int* sys = (int*)(base + sizeof(TileVec<T>) * (size_t)std::max(B.T, 1));
+    Active A;
+    for (int k = k0; k < k1; ++k) {
+        if (live && !(*live)[k - k0]) continue;
+        sys[A.ns++] = k;
+        for (int t = B.h_sys_t0[k]; t < B.h_sys_t1[k]; ++t) {
+            rec[A.nt++] = tv[t];
+            A.kinds |= tv[t].vec ? 1 : 2;
+        }
+    }
+    if (A.nt > 0) {
+        HSPS_HIP(hipMemcpyAsync((sizeof(T) == 4 ? B.tva32 : B.tva64).p, rec, sizeof(TileVec<T>) * (size_t)A.nt,
+                                hipMemcpyHostToDevice, s));
+        HSPS_HIP(hipMemcpyAsync(B.act_sys.p, sys, sizeof(int) * (size_t)A.ns, hipMemcpyHostToDevice, s));
+    }
+    return A;
+}
+
 template <class T, bool MIXED>
-static void enqueue_iters(const Batch& B, const BArgs& a, int k0, int k1, int t0, int nt, int iters, const double* b,
-                          double* x, hipStream_t s) {
+static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int iters, const double* b, double* x,
+                          hipStream_t s) {
     T* r = B.r.as<T>();
     T* p0 = B.p.as<T>();
     T* p1 = B.p1.as<T>();
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
-    const int ns = k1 - k0;
-    // which kernels the wave needs: the five-point path's, the general ones', or both
-    const std::vector<char>& vec = B.h_vec[sizeof(T) == 4 ? 1 : 0];
-    bool any_vec = false, any_gen = false;
-    for (int k = k0; k < k1; ++k) (vec[k] ? any_vec : any_gen) = true;
+    const int nt = act.nt, ns = act.ns;
+    const bool any_vec = act.kinds & 1, any_gen = act.kinds & 2;
     for (int it = 0; it < iters; ++it) {
         const size_t lds = sizeof(double) * (size_t)a.stage_doubles;
         if (any_vec)
-            hipLaunchKernelGGL((k_b_k1v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, r, b,
-                               (const double*)x, (const float*)xl);
+            hipLaunchKernelGGL((k_b_k1v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, r, b, (const double*)x,
+                               (const float*)xl);
         if (any_gen)
-            hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, p0, p1, q, r, b,
+            hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, q, r, b,
                                (const double*)x, (const float*)xl);
-        hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kFinishThreads), 0, s, a, k0, OpF1{});
+        hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kFinishThreads), 0, s, a, 0, a.act_sys, OpF1{});
         if (any_vec)
-            hipLaunchKernelGGL((k_b_k2v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0,
-                               (const T*)p1, r, x, xl);
+            hipLaunchKernelGGL((k_b_k2v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, (const T*)p0, (const T*)p1, r, x,
+                               xl);
         if (any_gen)
-            hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, t0, (const T*)p0,
-                               (const T*)p1, (const T*)q, r, x, xl);
-        hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kFinishThreads), 0, s, a, k0, OpF2<MIXED>{});
+            hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, (const T*)p0, (const T*)p1,
+                               (const T*)q, r, x, xl);
+        hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kFinishThreads), 0, s, a, 0, a.act_sys,
+                           OpF2<MIXED>{});
     }
     HSPS_CHECK_LAUNCH();
 }
@@ -1599,19 +1677,19 @@ static void ensure_work(Batch& B, int prec) {
 
 constexpr size_t kMaxGraphs = 16;  // captured chunks kept per batch
 
-static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const double* b, double* x, int prec,
-                         hipStream_t s, bool use_graph) {
-    const int k0 = B.waves[w].first, k1 = B.waves[w].second;
-    const int t0 = B.h_sys_t0[k0], nt = B.h_sys_t1[k1 - 1] - t0;
-    if (nt <= 0) return;
+static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const Active& act, const double* b, double* x,
+                         int prec, hipStream_t s, bool use_graph) {
+    if (act.nt <= 0) return;
     auto direct = [&]() {
-        if (prec == HSPS_MIXED) enqueue_iters<float, true>(B, a, k0, k1, t0, nt, chunk, b, x, s);
-        else enqueue_iters<double, false>(B, a, k0, k1, t0, nt, chunk, b, x, s);
+        if (prec == HSPS_MIXED) enqueue_iters<float, true>(B, a, act, chunk, b, x, s);
+        else enqueue_iters<double, false>(B, a, act, chunk, b, x, s);
     };
     if (!use_graph) { direct(); return; }
     Batch::G* g = nullptr;
     for (auto& c : B.graphs)
-        if (c.wave == w && c.prec == prec && c.chunk == chunk && std::memcmp(&c.args, &a, sizeof(BArgs)) == 0) g = &c;
+        if (c.wave == w && c.prec == prec && c.chunk == chunk && c.nt == act.nt && c.ns == act.ns &&
+            c.kinds == act.kinds && std::memcmp(&c.args, &a, sizeof(BArgs)) == 0)
+            g = &c;
     if (!g) {
         if (B.graphs.size() >= kMaxGraphs) {  // oldest first
             (void)hipGraphExecDestroy(B.graphs.front().exec);
@@ -1631,6 +1709,9 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const doubl
         ng.wave = w;
         ng.prec = prec;
         ng.chunk = chunk;
+        ng.nt = act.nt;
+        ng.ns = act.ns;
+        ng.kinds = act.kinds;
         ng.args = a;
         const hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -1665,14 +1746,14 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
     // large systems: tiled waves on the main stream
     if (B.T > 0) {
         hipLaunchKernelGGL(k_b_init1, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x, B.r64.as<double>());
-        hipLaunchKernelGGL((k_b_finish<2, OpInit1>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, OpInit1{});
+        hipLaunchKernelGGL((k_b_finish<2, OpInit1>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, (const int*)nullptr, OpInit1{});
         if (prec == HSPS_MIXED)
             hipLaunchKernelGGL((k_b_init2<float, true>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
                                (const double*)B.r64.as<double>(), B.r.as<float>(), B.p.as<float>(), B.xlo.as<float>());
         else
             hipLaunchKernelGGL((k_b_init2<double, false>), dim3(B.T), dim3(kBlock), 0, s, a, 0,
                                (const double*)B.r64.as<double>(), B.r.as<double>(), B.p.as<double>(), (float*)nullptr);
-        hipLaunchKernelGGL((k_b_finish<1, OpInit2>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, OpInit2{});
+        hipLaunchKernelGGL((k_b_finish<1, OpInit2>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, (const int*)nullptr, OpInit2{});
         HSPS_CHECK_LAUNCH();
     }
     const int chunk = std::max(1, std::min(p.check_every > 0 ? p.check_every : 32, 64));
@@ -1694,8 +1775,17 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
             int k = 0, chunks = 0;
             bool pending = false;
             const int cap = (p.max_iter + chunk - 1) / chunk + 4;
+            // the kernels run the tiles and systems still iterating: every one at
+            // first, and the halt flags the host reads (one chunk behind) shrink
+            // the set whenever its live tiles are at most half of those launched
+            auto upload = [&](const std::vector<char>* live) {
+                return prec == HSPS_MIXED ? upload_active<float>(B, k0, k1, live, s)
+                                          : upload_active<double>(B, k0, k1, live, s);
+            };
+            Active act = upload(nullptr);
+            std::vector<char> live(ns, 1);
             for (;;) {
-                launch_chunk(B, a, w, chunk, b, x, prec, s, use_graph);
+                launch_chunk(B, a, w, chunk, act, b, x, prec, s, use_graph);
                 ++chunks;
                 BState* slot = pin + (size_t)(k & 1) * B.S;
                 HSPS_HIP(hipMemcpyAsync(slot + k0, B.st.as<BState>() + k0, sizeof(BState) * ns, hipMemcpyDeviceToHost, s));
@@ -1705,7 +1795,15 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
                     HSPS_HIP(hipEventSynchronize(ev[(k - 1) & 1]));
                     const BState* prev = pin + (size_t)((k - 1) & 1) * B.S;
                     all = true;
-                    for (int i = k0; i < k1 && all; ++i) all = prev[i].halt != 0;
+                    int live_tiles = 0;
+                    for (int i = k0; i < k1; ++i) {
+                        live[i - k0] = prev[i].halt == 0;
+                        if (live[i - k0]) {
+                            all = false;
+                            live_tiles += B.h_sys_t1[i] - B.h_sys_t0[i];
+                        }
+                    }
+                    if (!all && 2 * live_tiles <= act.nt) act = upload(&live);
                 }
                 pending = true;
                 ++k;
@@ -1716,7 +1814,7 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
             if (prec == HSPS_MIXED)  // every large system's rows, one workgroup per tile
                 hipLaunchKernelGGL(k_b_flush_tiles, dim3(B.T), dim3(kBlock), 0, s, a, x, B.xlo.as<float>());
             hipLaunchKernelGGL(k_b_final, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x);
-            hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, OpFinal{});
+            hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, (const int*)nullptr, OpFinal{});
             HSPS_CHECK_LAUNCH();
         }
         HSPS_HIP(hipStreamWaitEvent(s, B.join, 0));
