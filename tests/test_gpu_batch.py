@@ -257,3 +257,45 @@ def test_batch_params_change_between_calls(H, ctx):
     assert ok and (tr <= 1e-11).all() and (it11 > it8).all(), (it8, it11)
     ok, it8b, cv, tr = run(tol=1e-8, max_iter=5000, precision="mixed")
     assert ok and np.array_equal(it8b, it8)
+
+
+def dominant_stencil(n, w, d=8.0):
+    """Five-point pattern (offsets +-1, +-w) with diagonal d > 4: CG+Jacobi
+    converges in a few dozen iterations."""
+    import scipy.sparse as sp
+    return csr_of(sp.diags([-np.ones(n - w), -np.ones(n - 1), d * np.ones(n), -np.ones(n - 1), -np.ones(n - w)],
+                           [-w, -1, 0, 1, w]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_grid_shrinks_to_live_systems(H, ctx, precision):
+    """Systems that halt leave the launch grids: once the live systems' tiles
+    are at most half of those launched, the host rebuilds the active tile
+    records and system list.  Three fast systems (354 of the 404 tiles) halt
+    within two chunks and the slow one runs on alone for hundreds of
+    iterations; every system gives the same iterations and bits as alone."""
+    arrs = [dominant_stencil(60000, 250), dominant_stencil(60000, 240), dominant_stencil(60000, 260),
+            H.poisson_2d(160)]
+    oracles = [O.Csr.from_csr(len(rp) - 1, len(rp) - 1, rp, ci, v) for rp, ci, v in arrs]
+    b_all = [A.spmv(O.uniform(70 + s, A.rows)) for s, A in enumerate(oracles)]
+    p = H.default_params(tol=1e-8, max_iter=5000, precision=precision, batch_tile_rows=512)
+    B = H.Batch.from_systems(ctx, arrs)
+    assert B.storage() == {"csr": 0, "dia": 0, "stencil": 4}
+    bd, xd = ctx.array(np.concatenate(b_all)), ctx.array(n=B.n)
+    ok, it_mix, conv, tr = B.solve(bd, xd, p)
+    x_mix = xd.download()
+    assert ok and conv.all(), (it_mix, conv, tr)
+    assert max(it_mix[:3]) < 64 and it_mix[3] > 4 * 32, it_mix  # fast ones halt within two chunks of 32
+    for s, A in enumerate(oracles):
+        xs = x_mix[B.row_off[s]:B.row_off[s + 1]]
+        rel = np.linalg.norm(b_all[s] - A.spmv(xs)) / np.linalg.norm(b_all[s])
+        assert rel < 1e-8, (s, rel)
+        xo, st = O.solve(A, b_all[s], "cg", "jacobi", tol=1e-8, max_iter=5000)
+        assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo), s
+        slack = 0.05 if precision == "fp64" else 0.10
+        assert abs(it_mix[s] - st.iterations) <= max(2, slack * st.iterations), (s, it_mix[s], st.iterations)
+        Bs = H.Batch.from_systems(ctx, [arrs[s]])
+        bs, x1 = ctx.array(b_all[s]), ctx.array(n=len(b_all[s]))
+        ok1, it1, _, _ = Bs.solve(bs, x1, p)
+        assert ok1 and it1[0] == it_mix[s], (s, it1[0], it_mix[s])
+        np.testing.assert_array_equal(x1.download(), xs)
