@@ -1614,13 +1614,18 @@ struct Active {
     int nt = 0, ns = 0, kinds = 0;
 };
 
+template <class T>
+static const std::vector<TileVec<T>>& host_records(const Batch& B) {
+    if constexpr (sizeof(T) == 4) return B.h_tv32;
+    else return B.h_tv64;
+}
+
 // Stages the records of the tiles of the live systems of [k0, k1) (live =
 // nullptr: all of them) and the systems themselves, and copies both to the
 // device on stream s behind the work already queued there.
 template <class T>
 static Active upload_active(Batch& B, int k0, int k1, const std::vector<char>* live, hipStream_t s) {
-    const std::vector<TileVec<T>>& tv = sizeof(T) == 4 ? (const std::vector<TileVec<T>>&)B.h_tv32
-                                                       : (const std::vector<TileVec<T>>&)B.h_tv64;
+    const std::vector<TileVec<T>>& tv = host_records<T>(B);
     char* base = (char*)B.pin_act[B.pin_next];
     B.pin_next ^= 1;
     TileVec<T>* rec = (TileVec<T>*)base;
