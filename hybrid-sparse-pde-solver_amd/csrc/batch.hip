@@ -111,7 +111,8 @@ __device__ __forceinline__ BHot load_hot(const BState* S) {
 // fits the launch's LDS): everything K1 / K2 read before their first vector
 // load, in one 128-byte record, so that a workgroup waits for two dependent
 // scalar loads (this record, then its system's state) instead of one per
-// table.  vec = 0 sends the tile down the general paths, which read the
+// table.  vec = 1 / 2: the five-point kernels with kVecNarrow / kVecWide
+// threads; vec = 0 sends the tile down the general paths, which read the
 // tables themselves.  The iteration kernels read the records of the tiles
 // whose systems are still iterating from a compacted copy (BArgs::tva*),
 // which the host rebuilds as systems halt.
@@ -394,13 +395,20 @@ struct VecChunk {
     }
 };
 constexpr int kPreM = 4;  // K1: presence words per lane loaded with the staging loads
+// Workgroup sizes of the five-point kernels: 256 threads, or 512 for a
+// system whose staging window exceeds 16 KB (a wide halo: at 32-64 KB of LDS a
+// CU holds 2-4 workgroups, and 512-thread ones keep twice the waves in flight;
+// Poisson 2048^2, one system, mixed PCG: 82 -> 76 ms; the 16 KB windows of the
+// 512-system batch run faster at 256).  The size is a property of the system
+// (TileVec::vec = 1 or 2), so a system gives the same bits alone and in any batch.
+constexpr int kVecNarrow = kBlock, kVecWide = 512;
 
 // K1 rows [r0, r1) of a five-point stencil system: p_new = dv r + beta p_old
 // by groups into LDS (and into pn for the chunk's own rows), then
 // q = A p_new from LDS and pq += p_new . q.  q is not stored: K2 recomputes it.
 // The presence words of a lane's first kPreM groups are loaded together with
 // the staging loads, so the stencil phase after the barrier waits on LDS only.
-template <class T>
+template <class T, int BS>
 __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsigned char* __restrict__ mask,
                                                lds_ptr<T> lds, const T* __restrict__ r, const T* __restrict__ po,
                                                T* __restrict__ pn, double beta, double& pq) {
@@ -413,17 +421,17 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
         unsigned mw[kPreM];
 #pragma unroll
         for (int k = 0; k < kPreM; ++k) {
-            const int g = W.a0 + (tid + k * kBlock) * G;
+            const int g = W.a0 + (tid + k * BS) * G;
             mw[k] = g < W.c1 ? mask_word<T>(mask, g) : 0u;
         }
         // kStageU groups per lane: every load is issued before the first
         // store (vmcnt counts loads and stores in one in-order queue, so a
         // load wait placed after a store also waits for that store)
-        for (int gb = W.g0 + tid * G; gb < W.g1; gb += kStageU * kBlock * G) {
+        for (int gb = W.g0 + tid * G; gb < W.g1; gb += kStageU * BS * G) {
             Pack<T> rv[kStageU], pv[kStageU];
 #pragma unroll
             for (int u = 0; u < kStageU; ++u) {
-                const int g = gb + u * kBlock * G;
+                const int g = gb + u * BS * G;
                 if (g < W.g1) {
                     rv[u] = ld16(r + g);
                     pv[u] = ld16(po + g);
@@ -431,7 +439,7 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
             }
 #pragma unroll
             for (int u = 0; u < kStageU; ++u) {
-                const int g = gb + u * kBlock * G;
+                const int g = gb + u * BS * G;
                 if (g >= W.g1) break;
                 Pack<T> v;
 #pragma unroll
@@ -458,7 +466,7 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
         };
         int k = 0;
 #pragma clang loop unroll(disable)
-        for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G, ++k) {
+        for (int g = W.a0 + tid * G; g < W.c1; g += BS * G, ++k) {
             unsigned m;
             if (k < kPreM) m = k == 0 ? mw[0] : k == 1 ? mw[1] : k == 2 ? mw[2] : mw[3];
             else m = mask_word<T>(mask, g);
@@ -467,7 +475,7 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
         // the chunk's own rows of p_new, stored only now: a store issued
         // before the stencil phase would hold up its first presence-word use
         // (vmcnt counts loads and stores in one in-order queue)
-        for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G) {
+        for (int g = W.a0 + tid * G; g < W.c1; g += BS * G) {
             Pack<T> v;
 #pragma unroll
             for (int e = 0; e < G; ++e) v.e[e] = lds[g - W.base + e];
@@ -485,7 +493,7 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
 // K2 rows [r0, r1) of a five-point stencil system: q = A p recomputed from p
 // staged in LDS, then x += alpha p (x_lo in mixed mode), r -= alpha q, with
 // r.r and r.z (z = dv r) partials; own-row vectors by 16-byte groups.
-template <class T, bool MIXED>
+template <class T, bool MIXED, int BS>
 __device__ __forceinline__ void k2_stencil_vec(const TileVec<T>& D, const unsigned char* __restrict__ mask,
                                                lds_ptr<T> lds, const T* __restrict__ p, T* __restrict__ r,
                                                double* __restrict__ x64, float* __restrict__ xlo, double alpha,
@@ -500,21 +508,21 @@ __device__ __forceinline__ void k2_stencil_vec(const TileVec<T>& D, const unsign
         const VecChunk<T> W(D, c);
         __syncthreads();
         // staging loads kStageU groups at a time, all in flight before the first LDS store
-        for (int g0 = W.g0 + tid * G; g0 < W.g1; g0 += kStageU * kBlock * G) {
+        for (int g0 = W.g0 + tid * G; g0 < W.g1; g0 += kStageU * BS * G) {
             Pack<T> v[kStageU];
 #pragma unroll
             for (int u = 0; u < kStageU; ++u)
-                if (g0 + u * kBlock * G < W.g1) v[u] = ld16(p + g0 + u * kBlock * G);
+                if (g0 + u * BS * G < W.g1) v[u] = ld16(p + g0 + u * BS * G);
 #pragma unroll
             for (int u = 0; u < kStageU; ++u) {
-                const int g = g0 + u * kBlock * G;
+                const int g = g0 + u * BS * G;
                 if (g >= W.g1) break;
 #pragma unroll
                 for (int e = 0; e < G; ++e) lds[g - W.base + e] = v[u].e[e];
             }
         }
         __syncthreads();
-        for (int g = W.a0 + tid * G; g < W.c1; g += kBlock * G) {
+        for (int g = W.a0 + tid * G; g < W.c1; g += BS * G) {
             const bool full = g >= W.c0 && g + G <= W.c1;
             const unsigned mw = mask_word<T>(mask, g);
             const Pack<T> rv = ld16(r + g);
@@ -660,10 +668,10 @@ __global__ void __launch_bounds__(kFinishThreads) k_b_finish(BArgs B, int k0, co
     }
 }
 
-template <int NV>
+template <int NV, int BS = kBlock>
 __device__ __forceinline__ void tile_partial(double (&v)[NV], const BArgs& B, int tile) {
-    __shared__ double red[NV * (kBlock / 64)];
-    block_sum<NV>(v, red);
+    __shared__ double red[NV * (BS / 64)];
+    block_sum<NV, BS>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) B.partials[(size_t)k * B.T + tile] = v[k];
@@ -940,21 +948,21 @@ k_b_k2(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __r
 }
 
 // ---- K1 / K2 of the 16-byte five-point path ------------------------------------------------
-// Tiles whose record has vec set run here in every state (the general
-// kernels skip them).  Kernels of their own keep the general paths' register
+// Tiles whose record has vec = 1 (BS = kVecNarrow) or vec = 2 (BS = kVecWide)
+// run here in every state (the general kernels skip them).  Kernels of their own keep the general paths' register
 // budget out of these, and every value they need before the first vector load
 // comes from the tile record and the system's state.
 // K1: NORMAL / FLUSHED as k_b_k1's stencil path; FLUSH: the FP64 true
 // residual r = b - A (x64 + x_lo) with the rows read straight from memory
 // (Stencil::row<5>, the same sum as k_b_k1's tile_rows64).
-template <class T, bool MIXED>
-__global__ void __launch_bounds__(kBlock, 8)
+template <class T, bool MIXED, int BS>
+__global__ void __launch_bounds__(BS, 8)
 k_b_k1v(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, const double* __restrict__ b,
         const double* __restrict__ x64, const float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
     const TileVec<T> D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
     const int tile = D.tile;
-    if (!D.vec) return;
+    if (D.vec != (BS == kVecNarrow ? 1 : 2)) return;
     const BHot h = load_hot(&B.st[D.sys]);
     const int mode = h.halt ? -1 : h.flag;  // one branch on both fields: their loads issue together
     if (mode < 0) return;
@@ -963,7 +971,7 @@ k_b_k1v(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, cons
         const Stencil<double> A(B, D.sys);
         const T dv = D.cd;
         auto xv = [&](int j) { return MIXED ? x64[j] + (double)xlo[j] : x64[j]; };
-        for (int i = D.r0 + threadIdx.x; i < D.r1; i += kBlock) {
+        for (int i = D.r0 + threadIdx.x; i < D.r1; i += BS) {
             const double rh = b[i] - A.template row<5>(i, xv);
             const T ri = (T)rh;
             r[i] = ri;
@@ -971,36 +979,36 @@ k_b_k1v(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, cons
             red[1] += (double)ri * (double)(T)(dv * ri);
         }
     } else {
-        k1_stencil_vec(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, r, h.pcur ? p1 : p0, h.pcur ? p0 : p1,
+        k1_stencil_vec<T, BS>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, r, h.pcur ? p1 : p0, h.pcur ? p0 : p1,
                        h.beta, red[0]);
     }
-    tile_partial<2>(red, B, tile);
+    tile_partial<2, BS>(red, B, tile);
 }
 
 // K2: NORMAL as k_b_k2's stencil path; FLUSHED: x64 += x_lo, x_lo = 0.
-template <class T, bool MIXED>
-__global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 8 : 7)
+template <class T, bool MIXED, int BS>
+__global__ void __launch_bounds__(BS, sizeof(T) == 4 ? 8 : 7)
 k_b_k2v(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, T* __restrict__ r,
         double* __restrict__ x64, float* __restrict__ xlo) {
     extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
     const TileVec<T> D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
     const int tile = D.tile;
-    if (!D.vec) return;
+    if (D.vec != (BS == kVecNarrow ? 1 : 2)) return;
     const BHot h = load_hot(&B.st[D.sys]);
     const int mode = h.halt ? -1 : h.flag;  // one branch on both fields: their loads issue together
     if (mode != BS_NORMAL && mode != BS_FLUSHED) return;
     if (mode == BS_FLUSHED) {
         if (MIXED)
-            for (int i = D.r0 + threadIdx.x; i < D.r1; i += kBlock) {
+            for (int i = D.r0 + threadIdx.x; i < D.r1; i += BS) {
                 x64[i] += (double)xlo[i];
                 xlo[i] = 0.0f;
             }
         return;
     }
     double red[2] = {0.0, 0.0};
-    k2_stencil_vec<T, MIXED>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, h.pcur ? p1 : p0, r, x64, xlo,
+    k2_stencil_vec<T, MIXED, BS>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, h.pcur ? p1 : p0, r, x64, xlo,
                              h.alpha, red);
-    tile_partial<2>(red, B, tile);
+    tile_partial<2, BS>(red, B, tile);
 }
 
 template <bool MIXED>
@@ -1271,7 +1279,6 @@ struct Batch {
     // host copies of the storage tables (per internal system) for the tile records
     std::vector<int> h_nd, h_off, h_cdok;
     std::vector<double> h_uv, h_cd;
-    std::vector<char> h_vec[2];  // per internal system: the 16-byte five-point path applies ([0] FP64, [1] FP32)
     int stage_doubles[2] = {kStageMinDoubles, kStageMinDoubles};  // K1 / K2 LDS: [0] FP64, [1] FP32
     // tiled schedule: rows per tile for CSR systems (0 = not tuned yet; also
     // capped at kLdsNnz nonzeros) and for diagonal-stored systems
@@ -1346,11 +1353,11 @@ static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::
             d.uv[j] = j < nd ? (T)B.h_uv[(size_t)k * kMaxDiags + j] : T(0);
         }
         d.cd = (T)B.h_cd[k];
-        d.vec = B.h_kind[k] == SK_STENCIL && nd <= 5 && B.h_cdok[k] && stage_need(halo, es) <= cap;
+        const int need = stage_need(halo, es);
+        d.vec = B.h_kind[k] == SK_STENCIL && nd <= 5 && B.h_cdok[k] && need <= cap
+                    ? ((long long)need * es > kStageMinDoubles * 8LL ? 2 : 1)
+                    : 0;
     }
-    std::vector<char>& sv = B.h_vec[es == 4 ? 1 : 0];
-    sv.assign(B.S, 0);
-    for (size_t t = 0; t < r0.size(); ++t) sv[sys[t]] = (char)tv[t].vec;
     out = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
     HSPS_HIP(hipMemcpy(out.p, tv.data(), tv.size() * sizeof(TileVec<T>), hipMemcpyHostToDevice));
     (es == 4 ? B.tva32 : B.tva64) = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
@@ -1608,8 +1615,8 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
 }
 
 // The active set of a wave: the tiles K1 / K2 run (their records, compacted)
-// and the systems F1 / F2 run.  kinds: bit 0 = some tile takes the
-// five-point kernels, bit 1 = some tile takes the general ones.
+// and the systems F1 / F2 run.  kinds: bit 0 / bit 2 = some tile takes the
+// narrow / wide five-point kernels, bit 1 = some tile takes the general ones.
 struct Active {
     int nt = 0, ns = 0, kinds = 0;
 };
@@ -1636,7 +1643,7 @@ static Active upload_active(Batch& B, int k0, int k1, const std::vector<char>* l
         sys[A.ns++] = k;
         for (int t = B.h_sys_t0[k]; t < B.h_sys_t1[k]; ++t) {
             rec[A.nt++] = tv[t];
-            A.kinds |= tv[t].vec ? 1 : 2;
+            A.kinds |= tv[t].vec == 1 ? 1 : tv[t].vec == 2 ? 4 : 2;
         }
     }
     if (A.nt > 0) {
@@ -1656,19 +1663,25 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
     const int nt = act.nt, ns = act.ns;
-    const bool any_vec = act.kinds & 1, any_gen = act.kinds & 2;
+    const bool narrow = act.kinds & 1, any_gen = act.kinds & 2, wide = act.kinds & 4;
     for (int it = 0; it < iters; ++it) {
         const size_t lds = sizeof(double) * (size_t)a.stage_doubles;
-        if (any_vec)
-            hipLaunchKernelGGL((k_b_k1v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, r, b, (const double*)x,
-                               (const float*)xl);
+        if (narrow)
+            hipLaunchKernelGGL((k_b_k1v<T, MIXED, kVecNarrow>), dim3(nt), dim3(kVecNarrow), lds, s, a, p0, p1, r, b,
+                               (const double*)x, (const float*)xl);
+        if (wide)
+            hipLaunchKernelGGL((k_b_k1v<T, MIXED, kVecWide>), dim3(nt), dim3(kVecWide), lds, s, a, p0, p1, r, b,
+                               (const double*)x, (const float*)xl);
         if (any_gen)
             hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, q, r, b,
                                (const double*)x, (const float*)xl);
         hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kFinishThreads), 0, s, a, 0, a.act_sys, OpF1{});
-        if (any_vec)
-            hipLaunchKernelGGL((k_b_k2v<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, (const T*)p0, (const T*)p1, r, x,
-                               xl);
+        if (narrow)
+            hipLaunchKernelGGL((k_b_k2v<T, MIXED, kVecNarrow>), dim3(nt), dim3(kVecNarrow), lds, s, a, (const T*)p0,
+                               (const T*)p1, r, x, xl);
+        if (wide)
+            hipLaunchKernelGGL((k_b_k2v<T, MIXED, kVecWide>), dim3(nt), dim3(kVecWide), lds, s, a, (const T*)p0,
+                               (const T*)p1, r, x, xl);
         if (any_gen)
             hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, (const T*)p0, (const T*)p1,
                                (const T*)q, r, x, xl);

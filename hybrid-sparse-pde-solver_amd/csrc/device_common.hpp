@@ -22,12 +22,12 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Sum NV values over the workgroup; the result is valid in thread 0.
-// `lds` must hold NV * (kBlock/64) doubles.  Ends with a barrier-free result
+// Sum NV values over a workgroup of BS threads; the result is valid in
+// thread 0.  `lds` must hold NV * (BS/64) doubles.  Ends with a barrier-free result
 // in thread 0; callers that reuse `lds` must barrier first.
-template <int NV>
+template <int NV, int BS = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
-    constexpr int W = kBlock / 64;
+    constexpr int W = BS / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
