@@ -222,7 +222,7 @@ k_tri_ell_lds(int n_levels, const int* __restrict__ lptr, const int* __restrict_
 // ALL_LDS sweep whose levels are all at most kTriThreads rows wide (a 2-D grid
 // up to 1024 wide in natural order): lane t owns row t of every level, and the
 // level loop holds no branch that depends on the lane or on a loop of its own,
-// so the ring loads kTriAhead levels ahead stay in flight across the barriers
+// so the ring loads several levels ahead stay in flight across the barriers
 // instead of being waited for at the top of each level.  The result goes out
 // in level order (out_pos[position], one contiguous run per level instead of
 // one cache line per row); k_tri_scatter puts it in row order afterwards.  A
@@ -236,10 +236,16 @@ __global__ void __launch_bounds__(kTriThreads)
 k_tri_ell_narrow(int n_levels, const int* __restrict__ lptr, const int* __restrict__ dep,
                  const V* __restrict__ val, const T* __restrict__ invd, const T* __restrict__ in_pos,
                  T* __restrict__ out_pos, const int* skip) {
+    // levels each lane loads ahead: for factors with at most 2 entries per
+    // row (five-point ILU(0)) measured on MI355X (conv-diff 1024^2,
+    // FGMRES(30) + ILU(0)): FP32 3 -> 12 levels, mixed solve 43.6 -> 37.9 ms;
+    // FP64 best at 8 (67.3 -> 63.4 ms; 12 levels: 91 ms).  Wider rows keep 3
+    // (a deeper ring would not fit the 128 registers of a 1024-thread workgroup).
+    constexpr int kAhead = K <= 2 ? (sizeof(T) == 4 ? 12 : 8) : kTriAhead;
     __shared__ T buf[2 * kTriLds];
     if ((skip && *skip) || n_levels <= 0) return;
     const int tid = threadIdx.x;
-    TriRow<K, V, T> ring[kTriAhead];
+    TriRow<K, V, T> ring[kAhead];
     // levels are never empty, so b - 1 is a row of level lev
     auto load = [&](TriRow<K, V, T>& R, int lev) {
         const int a = lptr[lev], b = lptr[lev + 1];
@@ -268,22 +274,22 @@ k_tri_ell_narrow(int n_levels, const int* __restrict__ lptr, const int* __restri
         if (UPPER) s = s * R.dv;
         out_pos[R.i] = s;
         cur[tid] = s;
-        load(R, min(lev + kTriAhead, last));
+        load(R, min(lev + kAhead, last));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     };
 #pragma unroll
-    for (int r = 0; r < kTriAhead; ++r) load(ring[r], min(r, last));
-    // whole rounds of kTriAhead levels with no exit inside (every path into
+    for (int r = 0; r < kAhead; ++r) load(ring[r], min(r, last));
+    // whole rounds of kAhead levels with no exit inside (every path into
     // the loop header then has the same loads in flight), the rest after
-    const int whole = n_levels - n_levels % kTriAhead;
-    for (int lev0 = 0; lev0 < whole; lev0 += kTriAhead) {
+    const int whole = n_levels - n_levels % kAhead;
+    for (int lev0 = 0; lev0 < whole; lev0 += kAhead) {
 #pragma unroll
-        for (int r = 0; r < kTriAhead; ++r) step(ring[r], lev0 + r);  // constant ring slots
+        for (int r = 0; r < kAhead; ++r) step(ring[r], lev0 + r);  // constant ring slots
     }
 #pragma unroll
-    for (int r = 0; r < kTriAhead - 1; ++r)
+    for (int r = 0; r < kAhead - 1; ++r)
         if (whole + r < n_levels) step(ring[r], whole + r);
 }
 
