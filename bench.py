@@ -599,6 +599,8 @@ def bench_serve(H, ctx, args):
         n = side * side
         b = sp.csr_matrix((v, ci, rp), shape=(n, n)) @ H.uniform_pm1(9000 + k, n)
         reqs.append((rp, ci, v, b))
+        if (k + 1) % 100 == 0:
+            log(f"[bench] serve: {k + 1} requests built")
     # mixed precision; ILU(0) at MODERATE (ILUT's dense row scan is O(n^2))
     p = H.default_params(tol=1e-8, max_iter=5000, precision="mixed", drop_tol=0.0)
     rng = np.random.default_rng(5)
@@ -615,7 +617,13 @@ def bench_serve(H, ctx, args):
         if dt > 0:
             time.sleep(dt)
         tickets.append(srv.submit(rp, ci, v, b, p))
-    results = [srv.wait(t) for t in tickets]
+        if len(tickets) % 100 == 0:
+            log(f"[bench] serve: {len(tickets)} submitted at {time.perf_counter() - t0:.1f} s")
+    results = []
+    for t in tickets:
+        results.append(srv.wait(t))
+        if len(results) % 100 == 0:
+            log(f"[bench] serve: {len(results)} done at {time.perf_counter() - t0:.1f} s")
     makespan = time.perf_counter() - t0
     srv.close()
     lat = np.array([r[2].latency_s for r in results])
