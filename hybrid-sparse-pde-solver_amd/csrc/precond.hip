@@ -219,8 +219,10 @@ k_tri_ell_lds(int n_levels, const int* __restrict__ lptr, const int* __restrict_
     }
 }
 
-// ALL_LDS sweep whose levels are all at most kTriThreads rows wide (a 2-D grid
-// up to 1024 wide in natural order): lane t owns row t of every level, and the
+// ALL_LDS sweep whose levels are all at most ROWS * kTriThreads rows wide (a
+// 2-D grid up to 1024 / 2048 wide in natural order; two rows per lane only for
+// factor rows of at most two entries): lane t owns rows t and t + 1024 of
+// every level, and the
 // level loop holds no branch that depends on the lane or on a loop of its own,
 // so the ring loads several levels ahead stay in flight across the barriers
 // instead of being waited for at the top of each level.  The result goes out
@@ -231,49 +233,62 @@ k_tri_ell_lds(int n_levels, const int* __restrict__ lptr, const int* __restrict_
 // already there) and parks its LDS copy in a slot the next level does not
 // read.  A missing dependency (kTriNone) reads slot 0 and discards it with a
 // select; the subtraction order is the one of k_tri_ell_lds.
-template <int K, class V, class T, bool UPPER>
+template <int K, class V, class T, bool UPPER, int ROWS>
 __global__ void __launch_bounds__(kTriThreads)
 k_tri_ell_narrow(int n_levels, const int* __restrict__ lptr, const int* __restrict__ dep,
                  const V* __restrict__ val, const T* __restrict__ invd, const T* __restrict__ in_pos,
                  T* __restrict__ out_pos, const int* skip) {
+    static_assert(ROWS == 1 || (ROWS == 2 && K <= 2), "two rows per lane only for two-entry factor rows");
+    static_assert(ROWS * kTriThreads <= kTriLds, "a level's LDS copy holds every position");
     // levels each lane loads ahead: for factors with at most 2 entries per
     // row (five-point ILU(0)) measured on MI355X (conv-diff 1024^2,
     // FGMRES(30) + ILU(0)): FP32 3 -> 12 levels, mixed solve 43.6 -> 37.9 ms;
-    // FP64 best at 8 (67.3 -> 63.4 ms; 12 levels: 91 ms).  Wider rows keep 3
-    // (a deeper ring would not fit the 128 registers of a 1024-thread workgroup).
-    constexpr int kAhead = K <= 2 ? (sizeof(T) == 4 ? 12 : 8) : kTriAhead;
+    // FP64 best at 8 (67.3 -> 63.4 ms; 12 levels: 91 ms); half as many with
+    // two rows per lane (the same rows in flight).  Wider rows keep 3 (a
+    // deeper ring would not fit the 128 registers of a 1024-thread workgroup).
+    constexpr int kAhead = K <= 2 ? (sizeof(T) == 4 ? 12 : 8) / ROWS : kTriAhead;
     __shared__ T buf[2 * kTriLds];
     if ((skip && *skip) || n_levels <= 0) return;
     const int tid = threadIdx.x;
-    TriRow<K, V, T> ring[kAhead];
-    // levels are never empty, so b - 1 is a row of level lev
-    auto load = [&](TriRow<K, V, T>& R, int lev) {
+    struct Rows {
+        TriRow<K, V, T> q[ROWS];
+    };
+    Rows ring[kAhead];
+    // lane tid owns positions tid + q * kTriThreads of every level; levels are
+    // never empty, so b - 1 is a row of level lev
+    auto load = [&](Rows& R, int lev) {
         const int a = lptr[lev], b = lptr[lev + 1];
-        const int idx = min(a + tid, b - 1);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            R.d[k] = dep[(size_t)idx * K + k];
-            R.v[k] = val[(size_t)idx * K + k];
+        for (int q = 0; q < ROWS; ++q) {
+            const int idx = min(a + tid + q * kTriThreads, b - 1);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                R.q[q].d[k] = dep[(size_t)idx * K + k];
+                R.q[q].v[k] = val[(size_t)idx * K + k];
+            }
+            R.q[q].i = idx;  // position in level order
+            R.q[q].x = in_pos[idx];
+            if (UPPER) R.q[q].dv = invd[idx];
         }
-        R.i = idx;  // position in level order
-        R.x = in_pos[idx];
-        if (UPPER) R.dv = invd[idx];
     };
     const int last = n_levels - 1;
-    auto step = [&](TriRow<K, V, T>& R, int lev) {
+    auto step = [&](Rows& R, int lev) {
         T* cur = buf + (lev & 1) * kTriLds;
         const T* prev = buf + ((lev & 1) ^ 1) * kTriLds;
-        T s = R.x;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const bool has = R.d[k] != kTriNone;
-            const T yj = prev[has ? -R.d[k] - 1 : 0];
-            const T t = s - (T)R.v[k] * yj;
-            s = has ? t : s;
+        for (int q = 0; q < ROWS; ++q) {
+            T s = R.q[q].x;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const bool has = R.q[q].d[k] != kTriNone;
+                const T yj = prev[has ? -R.q[q].d[k] - 1 : 0];
+                const T t = s - (T)R.q[q].v[k] * yj;
+                s = has ? t : s;
+            }
+            if (UPPER) s = s * R.q[q].dv;
+            out_pos[R.q[q].i] = s;
+            cur[tid + q * kTriThreads] = s;
         }
-        if (UPPER) s = s * R.dv;
-        out_pos[R.i] = s;
-        cur[tid] = s;
         load(R, min(lev + kAhead, last));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
@@ -460,15 +475,21 @@ static void tri_ell_launch(const TriSchedule& S, const TriEll& E, const T* in, T
     if (E.all_lds) {
         hipLaunchKernelGGL(k_tri_gather<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, S.order.as<int>(), in, in_pos,
                            skip);
-        if (S.max_level_width <= kTriThreads) {
-            if (dbl)
-                hipLaunchKernelGGL((k_tri_ell_narrow<K, double, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s,
-                                   S.n_levels, S.level_ptr.as<int>(), E.dep.as<int>(), E.val64.as<double>(), invd,
-                                   (const T*)in_pos, out_pos, skip);
-            else
-                hipLaunchKernelGGL((k_tri_ell_narrow<K, float, T, UPPER>), dim3(1), dim3(kTriThreads), 0, s,
-                                   S.n_levels, S.level_ptr.as<int>(), E.dep.as<int>(), E.val32.as<float>(), invd,
-                                   (const T*)in_pos, out_pos, skip);
+        const bool one = S.max_level_width <= kTriThreads;
+        if (one || (K <= 2 && S.max_level_width <= 2 * kTriThreads)) {
+            auto narrow = [&](auto rows) {
+                constexpr int R = decltype(rows)::value;
+                if constexpr (std::is_same<T, double>::value)
+                    hipLaunchKernelGGL((k_tri_ell_narrow<K, double, T, UPPER, R>), dim3(1), dim3(kTriThreads), 0, s,
+                                       S.n_levels, S.level_ptr.as<int>(), E.dep.as<int>(), E.val64.as<double>(), invd,
+                                       (const T*)in_pos, out_pos, skip);
+                else
+                    hipLaunchKernelGGL((k_tri_ell_narrow<K, float, T, UPPER, R>), dim3(1), dim3(kTriThreads), 0, s,
+                                       S.n_levels, S.level_ptr.as<int>(), E.dep.as<int>(), E.val32.as<float>(), invd,
+                                       (const T*)in_pos, out_pos, skip);
+            };
+            if (one) narrow(std::integral_constant<int, 1>{});
+            else if constexpr (K <= 2) narrow(std::integral_constant<int, 2>{});
             HSPS_CHECK_LAUNCH();
             hipLaunchKernelGGL(k_tri_scatter<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, S.order.as<int>(),
                                (const T*)out_pos, out, skip);

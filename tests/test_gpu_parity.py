@@ -281,13 +281,14 @@ def test_ilu_apply_matches_oracle(H, ctx):
         assert np.linalg.norm(zd.download() - zo) <= 1e-12 * np.linalg.norm(zo)
 
 
-@pytest.mark.parametrize("n", [23, 25, 1024, 1030])
+@pytest.mark.parametrize("n", [23, 25, 1024, 1030, 2048, 2049])
 def test_ilu_apply_level_shapes(H, ctx, n):
     """ILU(0) of an n x n convection-diffusion grid in natural order: the
-    sweeps run 2n - 1 anti-diagonal levels up to n rows wide.  n = 23 / 25
-    leave 0 / 1 levels after the last whole round of three prefetched levels
-    (n = 24 above leaves 2); n = 1024 fills the 1024-lane sweep at its widest
-    level; n = 1030 exceeds it (the sweep that loops over a level's rows)."""
+    sweeps run 2n - 1 anti-diagonal levels up to n rows wide.  n = 23 / 24
+    (above) / 25 leave 5 / 7 / 1 levels after the last whole round of the
+    FP64 sweep's 8 prefetched levels; n = 1024 fills the 1024-lane sweep at
+    its widest level; n = 1030 and 2048 take two rows per lane (2048 fills
+    them); n = 2049 exceeds that (the sweep that loops over a level's rows)."""
     A = O.Csr.convdiff(n, 0.05, 1.0, 0.5)
     Ad = dev_csr(H, ctx, A)
     M = H.Precond(Ad, "ilu", H.default_params(drop_tol=0.0))
@@ -364,6 +365,25 @@ def test_cg_engine_matches_plain_kernels(H, ctx, precision):
         if k == 3:  # large and CSR-stored: the plain kernels run either way
             assert res[0][0] == res[1][0]
             np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+def test_mixed_ilu_two_rows_per_lane(H, ctx):
+    """Conv-diff 1100^2: ILU levels up to 1100 rows wide, so the FP32 sweeps
+    of the mixed solve take two rows per lane.  The mixed solve reaches the
+    FP64 tolerance in the FP64 device solve's iterations (within 10% or 2),
+    and both match the FP64 oracle's solution."""
+    A = O.Csr.convdiff(1100, 0.05, 1.0, 0.5)
+    b = A.spmv(O.uniform(21, A.rows))
+    xo, so = O.solve(A, b, "fgmres", "ilu", tol=1e-10, max_iter=2000, restart_size=30, drop_tol=0.0)
+    its = {}
+    for precision in ("fp64", "mixed"):
+        ok, st, x = _solve_dev(H, ctx, A, b, "fgmres", "ilu", tol=1e-10, max_iter=2000, restart_size=30,
+                               drop_tol=0.0, precision=precision, fgmres_inner_rel=1e-5 if precision == "mixed" else 0.0)
+        assert ok and _rel(A, b, x) < 1e-10, precision
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), precision
+        its[precision] = st.iterations
+    assert abs(its["fp64"] - so.iterations) <= max(1, 0.05 * so.iterations), (its, so.iterations)
+    assert abs(its["mixed"] - its["fp64"]) <= max(2, 0.10 * its["fp64"]), its
 
 
 def test_mixed_fgmres_inner_tolerance(H, ctx):
