@@ -348,8 +348,11 @@ def test_cg_engine_matches_plain_kernels(H, ctx, precision):
     G.eliminate_zeros()
     L = (G + sp.diags(-np.asarray(G.sum(axis=1)).ravel() + 0.05)).tocsr()
     L.sort_indices()
+    # the last case has a 1100-row halo: the engine cuts one staging chunk per tile (2200 rows) and
+    # runs the five-point kernels with 512-thread workgroups (a 17.6 KB FP32 / 35 KB FP64 window)
     cases = [O.Csr.poisson(200), O.Csr.aniso(150, 0.05), O.Csr.poisson(40),
-             O.Csr.from_csr(n, n, L.indptr.astype(np.int32), L.indices.astype(np.int32), L.data)]
+             O.Csr.from_csr(n, n, L.indptr.astype(np.int32), L.indices.astype(np.int32), L.data),
+             O.Csr.poisson(1100)]
     for k, A in enumerate(cases):
         b = A.spmv(O.uniform(40 + k, A.rows))
         xo, so = O.solve(A, b, "cg", "jacobi", tol=1e-8, max_iter=20000)
