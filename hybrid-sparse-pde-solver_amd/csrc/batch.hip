@@ -352,6 +352,24 @@ __device__ __forceinline__ void st16_stream(T* p, const Pack<T>& v) {
     for (int e = 0; e < Pack<T>::G; ++e) w[e] = v.e[e];
     __builtin_nontemporal_store(w, reinterpret_cast<typename Vec16<T>::type*>(p));
 }
+// 16-byte LDS store / load of a group (its offset is a multiple of the
+// group): one ds_write_b128 / ds_read_b128 instead of word pairs that lie
+// four words apart across lanes, on the same banks
+template <class T>
+__device__ __forceinline__ void st_lds16(__attribute__((address_space(3))) T* p, const Pack<T>& v) {
+    typename Vec16<T>::type w;
+#pragma unroll
+    for (int e = 0; e < Pack<T>::G; ++e) w[e] = v.e[e];
+    *reinterpret_cast<__attribute__((address_space(3))) typename Vec16<T>::type*>(p) = w;
+}
+template <class T>
+__device__ __forceinline__ Pack<T> ld_lds16(const __attribute__((address_space(3))) T* p) {
+    const typename Vec16<T>::type w = *reinterpret_cast<const __attribute__((address_space(3))) typename Vec16<T>::type*>(p);
+    Pack<T> v;
+#pragma unroll
+    for (int e = 0; e < Pack<T>::G; ++e) v.e[e] = w[e];
+    return v;
+}
 constexpr int kStageU = 3;  // staging: groups per lane whose loads are issued together (a 2048-row tile with 512-row halos in one round)
 template <class T>
 __device__ __forceinline__ unsigned mask_word(const unsigned char* m, int g) {
@@ -437,8 +455,7 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
                 Pack<T> v;
 #pragma unroll
                 for (int e = 0; e < G; ++e) v.e[e] = (T)((double)(T)(dv * rv[u].e[e]) + beta * (double)pv[u].e[e]);
-#pragma unroll
-                for (int e = 0; e < G; ++e) lds[g - W.base + e] = v.e[e];
+                st_lds16(lds + (g - W.base), v);
             }
         }
         __syncthreads();
@@ -464,9 +481,7 @@ __device__ __forceinline__ void k1_stencil_vec(const TileVec<T>& D, const unsign
         // before the stencil phase would hold up its first presence-word use
         // (vmcnt counts loads and stores in one in-order queue)
         for (int g = W.a0 + tid * G; g < W.c1; g += BS * G) {
-            Pack<T> v;
-#pragma unroll
-            for (int e = 0; e < G; ++e) v.e[e] = lds[g - W.base + e];
+            const Pack<T> v = ld_lds16(lds + (g - W.base));
             if (g >= W.c0 && g + G <= W.c1) {
                 st16_stream(pn + g, v);
             } else {
@@ -505,8 +520,7 @@ __device__ __forceinline__ void k2_stencil_vec(const TileVec<T>& D, const unsign
             for (int u = 0; u < kStageU; ++u) {
                 const int g = g0 + u * BS * G;
                 if (g >= W.g1) break;
-#pragma unroll
-                for (int e = 0; e < G; ++e) lds[g - W.base + e] = v[u].e[e];
+                st_lds16(lds + (g - W.base), v[u]);
             }
         }
         __syncthreads();
