@@ -90,8 +90,10 @@ struct BState {
     int iter;
     double rho, rho_old, bnorm, rel, rmax, true_rel;
     int done, reason, reliable, restarts, conv;
+    int restarts_pq;  // restarts after a p.q breakdown (the rest follow a rho breakdown)
     double r0;  // initial ||r|| (absolute)
 };
+static_assert(sizeof(BState) == 112, "the counter fills the padding before r0");
 struct BHot {
     double alpha, beta;
     int halt, flag, pcur, iter;
@@ -852,6 +854,7 @@ struct OpF1 {
             if (fabs(pq) < kEps * 1e6 * fabs(S.rho)) {  // cg_solver.cpp:109-120: restart
                 S.iter += 1;
                 S.restarts += 1;
+                S.restarts_pq += 1;
                 S.flag = BS_FLUSH;
                 S.reason = BR_RESTART;
                 if (S.iter >= B.max_iter) S.halt = 1;
@@ -1176,6 +1179,7 @@ k_b_small(BArgs B, int k0, const double* __restrict__ b, double* __restrict__ x6
             if (fabs(pq) < kEps * 1e6 * fabs(S.rho)) {
                 S.iter += 1;
                 S.restarts += 1;
+                S.restarts_pq += 1;
                 reason = BR_RESTART;
             } else {
                 const double alpha = S.rho / pq;
@@ -2029,6 +2033,7 @@ bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResul
     out->iterations = S.iter;
     out->converged = S.conv;
     out->restarts = S.restarts;
+    out->restarts_pq = S.restarts_pq;
     out->reliable_updates = S.reliable;
     out->initial_residual = S.r0;
     out->final_residual = S.rel;

@@ -30,6 +30,16 @@ void ctx_release(Ctx* c) {
     delete static_cast<hsps_ctx_s*>(c);
 }
 
+void csr_retain(hsps_csr_s* A) { A->refs.fetch_add(1); }
+void csr_release(hsps_csr_s* A) {
+    if (A->refs.fetch_sub(1) != 1) return;
+    Ctx* c = A->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete A;
+    ctx_release(c);
+}
+
 // ---- DevBuf ------------------------------------------------------------------
 DevBuf::DevBuf(Ctx* c, size_t b) : ctx(c), bytes(b) {
     if (b == 0) return;
@@ -486,13 +496,7 @@ int hsps_csr_create(hsps_ctx ctx, int rows, int cols, int nnz, const int* rp, co
 
 int hsps_csr_destroy(hsps_csr A) {
     return guarded([&] {
-        if (A) {
-            Ctx* c = A->ctx;
-            (void)hipSetDevice(c->device);
-            (void)hipStreamSynchronize(c->stream);
-            delete A;
-            ctx_release(c);
-        }
+        if (A) csr_release(A);
         return HSPS_OK;
     });
 }
@@ -513,6 +517,7 @@ int hsps_csr_update_values(hsps_csr A, const double* v) {
         HSPS_HIP(hipStreamSynchronize(A->ctx->stream));
         std::copy(v, v + A->nnz, A->h_v.begin());
         A->upload_values();
+        A->values_version += 1;
         return HSPS_OK;
     });
 }

@@ -112,6 +112,9 @@ struct Csr {
     // solve that uses the batched engine; dropped when the values change)
     std::shared_ptr<Batch> solo;
     int solo_use = -1;  // -1 not decided, 0 plain kernels, 1 batched engine
+    // bumped by every hsps_csr_update_values: a preconditioner records the
+    // version it was set up from
+    unsigned long long values_version = 0;
     void upload_values();
     void build_schedule();
 };
@@ -120,11 +123,39 @@ struct Csr {
 // (batch.hip): diagonal / stencil storage and the fused iteration kernels.
 struct SoloResult {
     int iterations = 0, converged = 0, restarts = 0, reliable_updates = 0;
+    int restarts_pq = 0;  // restarts after a p.q breakdown (the others follow a rho breakdown)
     double initial_residual = 0, final_residual = 0, true_residual = 0;
 };
 // Returns false (nothing run) for a large CSR-stored system: the plain
 // kernel sequence serves those better.
 bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResult* out);
+
+// flop_model (energy_monitor.hpp:163-177) with vb-byte values and vectors
+// (8: the reference's Real; 4: the FP32 inner loop of mixed precision).
+inline long long model_spmv_bytes(long long rows, long long nnz, long long vb) {
+    return (rows + 1 + nnz) * 4 + nnz * vb + rows * vb * 2;
+}
+// CGSolver::solve's counters (cg_solver.cpp:98-175): a full iteration is an
+// SpMV, p.q, two axpys, r.z and the p update (2 nnz + 10 n FLOPs, SpMV bytes
+// + 13 n vector accesses); the converging iteration stops after the axpys
+// (2 nnz + 6 n, + 8 n); a p.q breakdown after p.q (2 nnz + 2 n, + 2 n); a
+// rho breakdown after r.z (2 nnz + 8 n, + 10 n).  Breakdown iterations count
+// as iterations (the loop's `continue`).  The residual norm is not counted.
+struct CgCost {
+    long long flops = 0, bytes = 0;
+};
+inline CgCost cg_cost(long long n, long long nnz, long long iters, bool converged_last, long long pq_breaks,
+                      long long rho_breaks, long long vb) {
+    converged_last = converged_last && iters > 0;  // 0 iterations: converged at the start
+    const long long full = iters - (converged_last ? 1 : 0) - pq_breaks - rho_breaks;
+    const long long S = model_spmv_bytes(n, nnz, vb);
+    CgCost c;
+    c.flops = full * (2 * nnz + 10 * n) + (converged_last ? 2 * nnz + 6 * n : 0) + pq_breaks * (2 * nnz + 2 * n) +
+              rho_breaks * (2 * nnz + 8 * n);
+    c.bytes = full * (S + 13 * n * vb) + (converged_last ? S + 8 * n * vb : 0) + pq_breaks * (S + 2 * n * vb) +
+              rho_breaks * (S + 10 * n * vb);
+    return c;
+}
 
 // Row-block partition: consecutive rows, <= max_rows rows and <= max_nnz
 // nonzeros per block; a longer row gets a block of its own.  Greedy, so a
@@ -184,4 +215,12 @@ const float* precond_jacobi_dinv32(const Precond& M);
 }  // namespace hsps_dev
 
 struct hsps_ctx_s : hsps_dev::Ctx {};
-struct hsps_csr_s : hsps_dev::Csr {};
+// A matrix handle lives until hsps_csr_destroy has been called AND every
+// preconditioner built on it has been destroyed (they read it at apply time).
+struct hsps_csr_s : hsps_dev::Csr {
+    std::atomic<int> refs{1};
+};
+namespace hsps_dev {
+void csr_retain(hsps_csr_s* A);
+void csr_release(hsps_csr_s* A);
+}  // namespace hsps_dev

@@ -469,9 +469,6 @@ static void finish_stats(hsps_stats* s, long long F, long long B) {
     s->energy_joules = 2.0e-10 * (double)F + 5.0e-9 * (double)B;
 }
 
-// flop_model (energy_monitor.hpp:163-177)
-static long long spmv_bytes(long long rows, long long nnz) { return (rows + 1 + nnz) * 4 + nnz * 8 + rows * 16; }
-
 // True relative residual ||b - A x|| / ||b|| in FP64 (device), for stats.
 static double true_rel(const Csr& A, const double* b, const double* x, double* r64, double bnorm, double* d_out,
                        double* partials, hipStream_t s) {
@@ -618,6 +615,7 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     Event ev[2];
     int k = 0;
     bool pending = false;
+    long long pq_breaks = 0, rho_breaks = 0;
     auto read_state = [&]() {
         HSPS_HIP(hipMemcpyAsync(hst, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
         HSPS_HIP(hipStreamSynchronize(s));
@@ -647,6 +645,7 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
         if (cur.done || (cur.iter >= cur.max_iter && !refine)) break;
         if (cur.flag == FLAG_BREAK_PQ || cur.flag == FLAG_BREAK_RHO) {
             // restart from the current x (cg_solver.cpp:109-120, 157-167)
+            if (cur.flag == FLAG_BREAK_PQ) ++pq_breaks; else ++rho_breaks;
             if (MIXED) hipLaunchKernelGGL(k_flush, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, xlo.as<float>());
             residual_f64(A, b, x, r64.as<double>(), s, nullptr, nullptr, nullptr);
             start_direction();
@@ -695,9 +694,8 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     st.final_residual = fin.rel;
     st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, s);
     const int conv = MIXED ? (st.true_residual < p.tol && fin.done) : (fin.rel < p.tol);
-    const long long it = fin.iter;
-    const size_t vb = MIXED ? 4 : 8;
-    finish_stats(&st, it * (2LL * A.nnz + 10LL * n), it * (spmv_bytes(n, A.nnz) + 7LL * n * (long long)vb));
+    const CgCost cc = cg_cost(n, A.nnz, fin.iter, fin.done != 0, pq_breaks, rho_breaks, MIXED ? 4 : 8);
+    finish_stats(&st, cc.flops, cc.bytes);
     return finish(conv);
 }
 
@@ -862,10 +860,11 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     st.final_residual = rel;
     st.true_residual = true_rel(A, b, x, r64.as<double>(), bnorm, scal.as<double>(), part, s);
     if (MIXED) conv = conv && st.true_residual < p.tol;
-    const size_t vb = MIXED ? 4 : 8;
+    // fgmres_solver.cpp:196-198: per inner iteration one SpMV and 4 n (vector accesses)
+    const long long vb = MIXED ? 4 : 8;
     const long long spmvs = (long long)total + (pipe ? cycles : 0);  // pipelined_fgmres_solver.cpp:224-227
     finish_stats(&st, spmvs * 2LL * A.nnz + (long long)total * 4LL * n,
-                 spmvs * spmv_bytes(n, A.nnz) + (long long)total * 4LL * n * (long long)vb);
+                 spmvs * model_spmv_bytes(n, A.nnz, vb) + (long long)total * 4LL * n * vb);
     return finish(conv);
 }
 
@@ -887,7 +886,12 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const 
         hsps_params p;
         if (pp) p = *pp; else hsps_params_default(&p);
         A->ctx->set_current();
-        if (p.cg_engine && M && M->kind == HSPS_PRECOND_JACOBI && M->jacobi_omega == 1.0) {
+        // The engine rebuilds D^-1 from A, so it runs only for a Jacobi set
+        // up from this very matrix and its current values; any other Jacobi
+        // applies its own stored D^-1 through the plain kernels, as the
+        // reference's CG applies M (cg_solver.cpp:79, 147).
+        if (p.cg_engine && M && M->kind == HSPS_PRECOND_JACOBI && M->jacobi_omega == 1.0 && M->A0 == A &&
+            M->A0_version == A->values_version) {
             const double t0 = now_s();
             SoloResult r;
             if (solo_cg(*A, b, x, p, &r)) {
@@ -900,9 +904,9 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const 
                 s.final_residual = r.final_residual;
                 s.true_residual = r.true_residual;
                 s.solve_time_s = now_s() - t0;
-                const long long it = r.iterations, n = A->rows;
-                const long long vb = p.precision == HSPS_MIXED ? 4 : 8;
-                finish_stats(&s, it * (2LL * A->nnz + 10LL * n), it * (spmv_bytes(n, A->nnz) + 7LL * n * vb));
+                const CgCost cc = cg_cost(A->rows, A->nnz, r.iterations, r.converged != 0, r.restarts_pq,
+                                          r.restarts - r.restarts_pq, p.precision == HSPS_MIXED ? 4 : 8);
+                finish_stats(&s, cc.flops, cc.bytes);
                 if (st) *st = s;
                 return r.converged;
             }

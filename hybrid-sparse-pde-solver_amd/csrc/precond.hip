@@ -704,6 +704,11 @@ static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int
             break;
         }
         case HSPS_PRECOND_AMG:
+            // level 0 smooths with the caller's matrix; the coarse levels and
+            // D^-1 were built from the values it had at setup
+            if (M.A0->values_version != M.A0_version)
+                throw Error(HSPS_ERR_INVALID, "AMG preconditioner: the matrix values changed since setup "
+                                              "(hsps_csr_update_values); create the preconditioner again");
             vcycle<T>(M, *M.A0, 0, r, z, s, skip);
             break;
         default:
@@ -740,6 +745,7 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
         M->ctx = A->ctx;
         M->n = A->rows;
         M->A0 = A;
+        M->A0_version = A->values_version;
         try {
             switch (kind) {
                 case HSPS_PRECOND_NONE: break;
@@ -756,6 +762,7 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
         if (setup_seconds)
             *setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         ctx_retain(M->ctx);
+        csr_retain(A);
         *out = M;
         return HSPS_OK;
     });
@@ -765,9 +772,11 @@ int hsps_precond_destroy(hsps_precond M) {
     return guarded([&] {
         if (M) {
             Ctx* c = M->ctx;
+            hsps_csr_s* A = M->A0;
             (void)hipSetDevice(c->device);
             (void)hipStreamSynchronize(c->stream);
             delete M;
+            if (A) csr_release(A);
             ctx_release(c);
         }
         return HSPS_OK;

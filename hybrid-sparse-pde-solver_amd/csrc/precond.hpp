@@ -63,7 +63,8 @@ struct Precond {
     int kind = HSPS_PRECOND_NONE;
     Ctx* ctx = nullptr;
     int n = 0;
-    const Csr* A0 = nullptr;  // the operator it was built for (owned by the caller)
+    hsps_csr_s* A0 = nullptr;  // the operator it was built for (retained until destroy)
+    unsigned long long A0_version = 0;  // A0->values_version at setup
     // Jacobi
     DevBuf dinv64, dinv32;
     double jacobi_omega = 1.0;

@@ -233,6 +233,13 @@ class Csr:
         except Exception:
             pass
 
+    def update_values(self, v):
+        """New values, same pattern (hsps_csr_update_values)."""
+        v = np.ascontiguousarray(v, np.float64)
+        if len(v) != self.nnz:
+            raise ValueError("update_values: wrong number of values")
+        check(lib().hsps_csr_update_values(self.h, v.ctypes.data_as(_dp)))
+
     def spmv(self, x, y, precision=HSPS_FP64):
         f = lib().hsps_spmv_f32 if precision == HSPS_MIXED else lib().hsps_spmv_f64
         check(f(self.h, x.ptr, y.ptr))

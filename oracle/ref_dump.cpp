@@ -75,7 +75,15 @@ static double relres(const SparseMatrix& A, const Vector& b, const Vector& x) {
     return r.norm2() / b.norm2();
 }
 
-struct Run { int iters; bool conv; double rel_true; double final_res; int restarts; Vector x; };
+struct Run {
+    int iters;
+    bool conv;
+    double rel_true;
+    double final_res;
+    int restarts;
+    Vector x;
+    long long flops, bytes;  // SolverStats::flop_count / mem_bytes (cg_solver.cpp:178-179)
+};
 
 static Run run(const std::string& solver, const std::string& pre, const SparseMatrix& A, const Vector& b,
                SolverParams p, int amg_levels = 10) {
@@ -91,7 +99,7 @@ static Run run(const std::string& solver, const std::string& pre, const SparseMa
     Vector x(A.rows(), 0.0);
     SolverStats st;
     bool ok = s->solve(A, b, x, st);
-    return {st.iterations, ok, relres(A, b, x), st.final_residual, st.restarts, x};
+    return {st.iterations, ok, relres(A, b, x), st.final_residual, st.restarts, x, st.flop_count, st.mem_bytes};
 }
 
 static std::string run_json(const Run& r) {
@@ -99,7 +107,8 @@ static std::string run_json(const Run& r) {
     std::ostringstream o;
     o << "{\"iters\":" << r.iters << ",\"converged\":" << (r.conv ? 1 : 0) << ",\"restarts\":" << r.restarts
       << ",\"rel_true\":" << num(r.rel_true) << ",\"final_residual\":" << num(r.final_res)
-      << ",\"x_norm\":" << num(xn) << ",\"x_head\":" << dvec(r.x.data(), std::min(8, r.x.size())) << "}";
+      << ",\"x_norm\":" << num(xn) << ",\"x_head\":" << dvec(r.x.data(), std::min(8, r.x.size()))
+      << ",\"flop_count\":" << r.flops << ",\"mem_bytes\":" << r.bytes << "}";
     return o.str();
 }
 

@@ -129,6 +129,21 @@ def test_readme_table_on_gpu(H, ctx, golden):
         assert abs(st.iterations - it) <= max(1, 0.05 * it), (key, st.iterations, it)
         assert st.true_residual < 1e-8
         assert abs(np.linalg.norm(x) - ref[key]["x_norm"]) <= 1e-6 * ref[key]["x_norm"], key
+        # SolverStats::flop_count / mem_bytes of the reference build (cg_solver.cpp:98-179,
+        # fgmres_solver.cpp:196-198) whenever the iteration counts agree
+        if st.iterations == it:
+            assert (st.flop_count, st.mem_bytes) == (ref[key]["flop_count"], ref[key]["mem_bytes"]), key
+
+
+# Mixed FGMRES cases that need one FP64 refinement cycle more than the
+# reference's iterations (measured on MI355X).  An FP32 cycle leaves a
+# residual gap of about u32 * || |A| |x - x0| || (FP32 Krylov vectors and
+# SpMV): when its Givens estimate meets tol the FP64 true residual is still
+# above tol, so a restarted cycle of two iterations follows.  With small
+# counts those two exceed 5%; the test proves the gap on every listed case
+# (the mixed solve stopped at the reference's count misses tol).
+MIXED_REFINEMENT_CYCLE = {("poisson16_s1", "fgmres30_ilu0"): 20, ("poisson32_s7", "fgmres30_ilu0"): 31,
+                          ("convdiff32_s5", "fgmres30_jacobi"): 18, ("convdiff64_s9", "fgmres30_jacobi"): 22}
 
 
 @pytest.mark.parametrize("idx", range(6))
@@ -138,23 +153,33 @@ def test_seeded_solves(H, ctx, golden, idx, precision):
     A = O.Csr.poisson(c["n"]) if c["kind"] == "poisson" else O.Csr.convdiff(c["n"], c["eps"], c["bx"], c["by"])
     xs = O.uniform(c["seed"], A.rows)
     b = A.spmv(xs)
+    kw = dict(tol=1e-8, max_iter=20000, restart_size=30, drop_tol=0.0)
+    bad = []
     for key, solver, pre in [("cg_jacobi", "cg", "jacobi"), ("fgmres30_jacobi", "fgmres", "jacobi"),
                              ("fgmres30_ilu0", "fgmres", "ilu"), ("fgmres30_amg", "fgmres", "amg")]:
         if key not in c:
             continue
         ref = c[key]
-        ok, st, x = _solve_dev(H, ctx, A, b, solver, pre, tol=1e-8, max_iter=20000, restart_size=30, drop_tol=0.0,
-                               precision=precision)
+        ok, st, x = _solve_dev(H, ctx, A, b, solver, pre, precision=precision, **kw)
         assert ok, (key, precision)
         assert _rel(A, b, x) < 1e-8
-        xo, sto = O.solve(A, b, solver, pre, tol=1e-8, max_iter=20000, restart_size=30, drop_tol=0.0)
+        xo, sto = O.solve(A, b, solver, pre, **kw)
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), (key, precision)
         it = ref["iters"]
-        # FP64: within 5% of the reference.  Mixed: total inner iterations
-        # within 10% (or 2): an FGMRES cycle that the FP32 Givens estimate
-        # ends early is followed by a short FP64-verified cycle.
-        bound = max(1, 0.05 * it) if precision == "fp64" else max(2, 0.10 * it)
-        assert abs(st.iterations - it) <= bound, (key, precision, st.iterations, it)
+        print(f"{c['case']} {key} {precision}: {st.iterations} iterations (reference {it})", flush=True)
+        # within 5% of the reference (at least 1), FP64 and mixed alike
+        # (mixed: total inner iterations)
+        if abs(st.iterations - it) > max(1, 0.05 * it):
+            exc = MIXED_REFINEMENT_CYCLE.get((c["case"], key)) if precision == "mixed" else None
+            if exc is None or st.iterations != exc:
+                bad.append((key, precision, st.iterations, it))
+            else:  # the gap: stopped at the reference's count, the mixed solve misses tol
+                _, st2, x2 = _solve_dev(H, ctx, A, b, solver, pre, precision=precision,
+                                        **dict(kw, max_iter=it))
+                assert st2.true_residual > 1e-8 and _rel(A, b, x2) > 1e-8, (key, st2.true_residual)
+        if precision == "fp64" and st.iterations == it:  # the reference's flop model, counted alike
+            assert (st.flop_count, st.mem_bytes) == (ref["flop_count"], ref["mem_bytes"]), key
+    assert not bad, bad
 
 
 def test_amg_hierarchy_bit_exact(H, ctx, golden):
@@ -362,8 +387,8 @@ def test_cg_engine_matches_plain_kernels(H, ctx, precision):
                                    cg_engine=engine)
             assert ok and _rel(A, b, x) < 1e-8 and abs(st.true_residual - _rel(A, b, x)) <= 1e-3 * st.true_residual
             assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), (k, engine)
-            bound = max(1, 0.05 * so.iterations) if precision == "fp64" else max(2, 0.10 * so.iterations)
-            assert abs(st.iterations - so.iterations) <= bound, (k, engine, st.iterations, so.iterations)
+            assert abs(st.iterations - so.iterations) <= max(1, 0.05 * so.iterations), (k, engine, precision,
+                                                                                        st.iterations, so.iterations)
             res[engine] = (st.iterations, x)
         if k == 3:  # large and CSR-stored: the plain kernels run either way
             assert res[0][0] == res[1][0]
@@ -386,7 +411,7 @@ def test_mixed_ilu_two_rows_per_lane(H, ctx):
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo), precision
         its[precision] = st.iterations
     assert abs(its["fp64"] - so.iterations) <= max(1, 0.05 * so.iterations), (its, so.iterations)
-    assert abs(its["mixed"] - its["fp64"]) <= max(2, 0.10 * its["fp64"]), its
+    assert abs(its["mixed"] - so.iterations) <= max(1, 0.05 * so.iterations), (its, so.iterations)
 
 
 def test_mixed_fgmres_inner_tolerance(H, ctx):
@@ -404,4 +429,68 @@ def test_mixed_fgmres_inner_tolerance(H, ctx):
         assert ok and _rel(A, b, x) < 1e-10 and st.true_residual < 1e-10, inner
         runs[inner] = st.iterations
     assert runs[1e-5] <= runs[0.0], runs
-    assert abs(runs[1e-5] - so.iterations) <= max(2, 0.15 * so.iterations), (runs, so.iterations)
+    assert abs(runs[1e-5] - so.iterations) <= max(1, 0.05 * so.iterations), (runs, so.iterations)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_jacobi_from_other_values_keeps_its_diagonal(H, ctx, precision):
+    """A Jacobi set up before hsps_csr_update_values keeps the D^-1 it was
+    built with, as the reference's CG applies M's stored inverse diagonal
+    (cg_solver.cpp:79, 147, jacobi.cpp:40-51): the one-system engine (which
+    rebuilds D^-1 from A) must not take such a solve, so cg_engine = 1 and 0
+    give the same bits; a Jacobi set up afresh takes the engine again."""
+    A = O.Csr.poisson(64)
+    rp, ci, v = A.arrays()
+    n = A.rows
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    v2 = v + np.where(rows == ci, O.uniform(5, len(v)) + 1.5, 0.0)  # a new diagonal in [4.5, 6.5]
+    A2 = O.Csr.from_csr(n, n, rp, ci, v2)
+    b = A2.spmv(O.uniform(6, n))
+    Ad = dev_csr(H, ctx, A)
+    M_old = H.Precond(Ad, "jacobi")
+    Ad.update_values(v2)
+    res = {}
+    for engine in (0, 1):
+        p = H.default_params(tol=1e-8, max_iter=5000, precision=precision, cg_engine=engine)
+        bd, xd = ctx.array(b), ctx.array(n=n)
+        ok, st = H.solve(Ad, bd, xd, "cg", M_old, p)
+        assert ok and _rel(A2, b, xd.download()) < 1e-8
+        res[engine] = (st.iterations, xd.download())
+    assert res[0][0] == res[1][0]
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    M_new = H.Precond(Ad, "jacobi")
+    p = H.default_params(tol=1e-8, max_iter=5000, precision=precision)
+    bd, xd = ctx.array(b), ctx.array(n=n)
+    ok, st = H.solve(Ad, bd, xd, "cg", M_new, p)
+    xo, so = O.solve(A2, b, "cg", "jacobi", tol=1e-8, max_iter=5000)
+    assert ok and abs(st.iterations - so.iterations) <= max(1, 0.05 * so.iterations)
+    assert np.linalg.norm(xd.download() - xo) <= 1e-6 * np.linalg.norm(xo)
+    M_old.close()
+    M_new.close()
+
+
+def test_preconditioner_keeps_its_matrix_alive(H, ctx):
+    """hsps_precond_create retains the matrix handle: destroying the matrix
+    first leaves the preconditioner usable (the AMG V-cycle reads level 0 at
+    apply time); after hsps_csr_update_values an AMG apply is refused (its
+    coarse levels were built from the old values)."""
+    A = O.Csr.poisson(48)
+    Ad = dev_csr(H, ctx, A)
+    M = H.Precond(Ad, "amg")
+    r = O.uniform(5, A.rows)
+    rd, zd = ctx.array(r), ctx.array(n=A.rows)
+    M.apply(rd, zd)
+    z0 = zd.download()
+    Ad.close()  # the caller's handle goes; the preconditioner's reference remains
+    zd2 = ctx.array(n=A.rows)
+    M.apply(rd, zd2)
+    np.testing.assert_array_equal(zd2.download(), z0)
+    M.close()  # the last reference: the matrix is freed here
+    B = dev_csr(H, ctx, A)
+    MB = H.Precond(B, "amg")
+    B.update_values(A.arrays()[2] * 2.0)
+    with pytest.raises(H.HspsError) as e:
+        MB.apply(rd, zd)
+    assert e.value.code == -1 and "values changed" in str(e.value)
+    MB.close()
+    B.close()

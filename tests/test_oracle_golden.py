@@ -32,6 +32,9 @@ def test_readme_iteration_table(golden):
         x, st = O.solve(A, b, solver, pre, tol=1e-8, max_iter=2000, restart_size=50, drop_tol=0.0)
         assert st.converged
         assert st.iterations == ref[key]["iters"] == published, key
+        # SolverStats::flop_count / mem_bytes of the reference build (cg_solver.cpp:98-179,
+        # fgmres_solver.cpp:196-198 over energy_monitor.hpp:163-177)
+        assert (st.flop_count, st.mem_bytes) == (ref[key]["flop_count"], ref[key]["mem_bytes"]), key
         np.testing.assert_allclose(x[:8], ref[key]["x_head"], rtol=1e-6)
         assert abs(np.linalg.norm(x) - ref[key]["x_norm"]) <= 1e-6 * ref[key]["x_norm"]
 
@@ -52,6 +55,8 @@ def test_seeded_cases(golden, idx):
         # exact for these deterministic sizes; the reference build uses
         # -ffast-math, so allow one iteration of slack
         assert abs(st.iterations - ref["iters"]) <= 1, (key, st.iterations, ref["iters"])
+        if st.iterations == ref["iters"]:
+            assert (st.flop_count, st.mem_bytes) == (ref["flop_count"], ref["mem_bytes"]), key
         np.testing.assert_allclose(x[:8], ref["x_head"], rtol=1e-6, atol=1e-9)
         r = b - A.spmv(x)
         assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
