@@ -29,6 +29,8 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <limits>
 #include <type_traits>
 
@@ -479,31 +481,77 @@ static double true_rel(const Csr& A, const double* b, const double* x, double* r
     return bnorm > 0 ? std::sqrt(h) / bnorm : std::sqrt(h);
 }
 
+// Buffers of one PCG solve: device workspace, two host slots for the state
+// and, optionally, two events (without them the host polls synchronously).
+struct CgWork {
+    void *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;  // working precision, n each
+    double* r64 = nullptr;       // FP64 residual, n
+    float* xlo = nullptr;        // mixed: FP32 low-order part of x, n
+    double* partials = nullptr;  // 2 x max(reduction blocks, SpMV row blocks)
+    CgState* st = nullptr;
+    double* scal = nullptr;      // 8 doubles
+    CgState* host = nullptr;     // 2 host slots
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+struct WorkView {
+    void* p;
+    template <class U> U* as() const { return static_cast<U*>(p); }
+};
+// 0 none, 1 Jacobi (inline), 2 any other preconditioner (z buffer + apply)
+static int cg_prec(const Precond* M) {
+    const int kind = M ? M->kind : HSPS_PRECOND_NONE;
+    return kind == HSPS_PRECOND_NONE ? 0 : kind == HSPS_PRECOND_JACOBI ? 1 : 2;
+}
+// Device bytes of a PCG solve's workspace; with base != null, also points
+// w's device buffers into [base, base + bytes), each 256-byte aligned.
+static size_t cg_work_layout(const Csr& A, bool mixed, int prec, unsigned char* base, CgWork* w) {
+    const size_t nb = (size_t)std::max(A.rows, 1);
+    const size_t vs = mixed ? sizeof(float) : sizeof(double);
+    const size_t np = 2 * (size_t)std::max(std::max(reduction_blocks(A.rows), A.n_blocks), 1);
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* ptr = base && bytes ? base + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return ptr;
+    };
+    void* r = take(nb * vs);
+    void* p = take(nb * vs);
+    void* q = take(nb * vs);
+    void* z = take(prec == 2 ? nb * vs : 0);
+    void* r64 = take(nb * sizeof(double));
+    void* xlo = take(mixed ? nb * sizeof(float) : 0);
+    void* part = take(np * sizeof(double));
+    void* st = take(sizeof(CgState));
+    void* scal = take(8 * sizeof(double));
+    if (base) {
+        w->r = r; w->p = p; w->q = q; w->z = z;
+        w->r64 = static_cast<double*>(r64);
+        w->xlo = static_cast<float*>(xlo);
+        w->partials = static_cast<double*>(part);
+        w->st = static_cast<CgState*>(st);
+        w->scal = static_cast<double*>(scal);
+    }
+    return off;
+}
+
 template <class T>
-static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p, hsps_stats* out) {
+static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p, hsps_stats* out,
+                  const CgWork& w) {
     constexpr bool MIXED = std::is_same<T, float>::value;
     Ctx* ctx = A.ctx;
     hipStream_t s = ctx->stream;
     const int n = A.rows;
     const double t0 = now_s();
     hsps_stats st{};
-    const int kind = M ? M->kind : HSPS_PRECOND_NONE;
-    const int PREC = kind == HSPS_PRECOND_NONE ? 0 : kind == HSPS_PRECOND_JACOBI ? 1 : 2;
+    const int PREC = cg_prec(M);
     if (MIXED && !A.v32.p && A.nnz) throw Error(HSPS_ERR_UNSUPPORTED, "matrix has no FP32 copy");
 
     const size_t nb = (size_t)std::max(n, 1);
-    DevBuf r(ctx, nb * sizeof(T)), pv(ctx, nb * sizeof(T)), q(ctx, nb * sizeof(T));
-    DevBuf z(ctx, PREC == 2 ? nb * sizeof(T) : 0);
-    DevBuf r64(ctx, nb * sizeof(double));
-    DevBuf xlo(ctx, MIXED ? nb * sizeof(float) : 0);
+    const WorkView r{w.r}, pv{w.p}, q{w.q}, z{w.z}, r64{w.r64}, xlo{w.xlo}, scal{w.scal};
     const int g = reduction_blocks(n);
-    DevBuf partials(ctx, sizeof(double) * 2 * (size_t)std::max(std::max(g, A.n_blocks), 1));
-    DevBuf dst(ctx, sizeof(CgState));
-    DevBuf scal(ctx, 64);
-    Pinned pin(sizeof(CgState) * 2);
-    CgState* hst = static_cast<CgState*>(pin.p);
-    CgState* d = dst.as<CgState>();
-    double* part = partials.as<double>();
+    CgState* hst = w.host;
+    CgState* d = w.st;
+    double* part = w.partials;
 
     // r = b - A x, ||r||^2, ||b||^2  (cg_solver.cpp:52-57)
     residual_f64(A, b, x, r64.as<double>(), s, &d->rr0, part, nullptr);
@@ -612,7 +660,6 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     };
 
     const int chunk = std::max(1, p.check_every > 0 ? p.check_every : 32);
-    Event ev[2];
     int k = 0;
     bool pending = false;
     long long pq_breaks = 0, rho_breaks = 0;
@@ -628,11 +675,16 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
         }
         CgState* slot = hst + (k & 1);
         HSPS_HIP(hipMemcpyAsync(slot, d, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        HSPS_HIP(hipEventRecord(ev[k & 1].e, s));
         bool halted = false;
-        if (pending) {
-            HSPS_HIP(hipEventSynchronize(ev[(k - 1) & 1].e));
-            halted = hst[(k - 1) & 1].halt != 0;
+        if (!w.ev[0]) {  // no events: read this chunk's state at once
+            HSPS_HIP(hipStreamSynchronize(s));
+            halted = slot->halt != 0;
+        } else {  // read the previous chunk's state while this one runs
+            HSPS_HIP(hipEventRecord(w.ev[k & 1], s));
+            if (pending) {
+                HSPS_HIP(hipEventSynchronize(w.ev[(k - 1) & 1]));
+                halted = hst[(k - 1) & 1].halt != 0;
+            }
         }
         pending = true;
         ++k;
@@ -697,6 +749,69 @@ static int run_cg(const Csr& A, const Precond* M, const double* b, double* x, co
     const CgCost cc = cg_cost(n, A.nnz, fin.iter, fin.done != 0, pq_breaks, rho_breaks, MIXED ? 4 : 8);
     finish_stats(&st, cc.flops, cc.bytes);
     return finish(conv);
+}
+
+// PCG with its workspace allocated for the call (one device buffer, pinned
+// host slots, two events).
+template <class T>
+static int run_cg_alloc(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p,
+                        hsps_stats* out) {
+    constexpr bool MIXED = std::is_same<T, float>::value;
+    const int prec = cg_prec(M);
+    DevBuf ws(A.ctx, cg_work_layout(A, MIXED, prec, nullptr, nullptr));
+    CgWork w;
+    cg_work_layout(A, MIXED, prec, ws.as<unsigned char>(), &w);
+    Pinned pin(sizeof(CgState) * 2);
+    w.host = static_cast<CgState*>(pin.p);
+    Event ev[2];
+    w.ev[0] = ev[0].e;
+    w.ev[1] = ev[1].e;
+    return run_cg<T>(A, M, b, x, p, out, w);
+}
+
+// ---- PCG without allocation ---------------------------------------------------------
+// The workspace of hsps_cg_solve_noalloc: one array per device, reserved when
+// the code object loads and held until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns on each device (the
+// symbol's name in host code is not a device address).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch address on `device` (current), looked up on the device's first call.
+static unsigned char* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(HSPS_ERR_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            throw Error(HSPS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
+template <class T>
+static int run_cg_noalloc(const Csr& A, const Precond* M, const double* b, double* x, const hsps_params& p,
+                          hsps_stats* out) {
+    constexpr bool MIXED = std::is_same<T, float>::value;
+    const int prec = cg_prec(M);
+    const size_t need = cg_work_layout(A, MIXED, prec, nullptr, nullptr);
+    if (need > kNoAllocScratchBytes)
+        throw Error(HSPS_ERR_NOMEM, "hsps_cg_solve_noalloc: the solve needs " + std::to_string(need) +
+                                        " bytes of workspace, more than the " +
+                                        std::to_string(kNoAllocScratchBytes) + "-byte scratch array");
+    CgWork w;
+    cg_work_layout(A, MIXED, prec, scratch_on(A.ctx->device), &w);
+    CgState host[2];  // pageable: the state is read back synchronously (no events)
+    w.host = host;
+    return run_cg<T>(A, M, b, x, p, out, w);
 }
 
 // pipe: the pipelined variant (pipelined_fgmres_solver.cpp:56-243): CGS2
@@ -911,7 +1026,20 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const 
                 return r.converged;
             }
         }
-        return p.precision == HSPS_MIXED ? run_cg<float>(*A, M, b, x, p, st) : run_cg<double>(*A, M, b, x, p, st);
+        return p.precision == HSPS_MIXED ? run_cg_alloc<float>(*A, M, b, x, p, st)
+                                         : run_cg_alloc<double>(*A, M, b, x, p, st);
+    });
+}
+
+int hsps_cg_solve_noalloc(hsps_csr A, hsps_precond M, const double* b, double* x, const hsps_params* pp,
+                          hsps_stats* st) {
+    return guarded([&] {
+        check_solve_args(A, M, b, x);
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        A->ctx->set_current();
+        return p.precision == HSPS_MIXED ? run_cg_noalloc<float>(*A, M, b, x, p, st)
+                                         : run_cg_noalloc<double>(*A, M, b, x, p, st);
     });
 }
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     "hsps_amg_level_matrix": (_i, [_vp, _i, _i, _ip, _ip, _dp]),
     "hsps_amg_setup_profile": (_i, [_vp, _dp]),
     "hsps_cg_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
+    "hsps_cg_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_pipelined_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_sstep_cg_solve": (_i, [_vp, _vp, _vp, _PP, _SP]),

@@ -184,6 +184,22 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x, co
                   hsps_stats* stats);
 int hsps_fgmres_solve(hsps_csr A, hsps_precond M, const double* d_b, double* d_x,
                       const hsps_params* p, hsps_stats* stats);
+/* PCG that allocates nothing.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same solve as hsps_cg_solve on its
+ * plain kernel sequence (p->cg_engine is ignored; the same bits as
+ * hsps_cg_solve with cg_engine = 0), with its workspace (r, p, q, z, the FP64
+ * residual, the FP32 low-order x, reduction partials and the device state)
+ * in one static 64 MiB array per device, reserved when the library loads and
+ * held until the process ends; the host reads the state back synchronously
+ * into stack memory.  Limit: unpreconditioned / Jacobi PCG needs 32 B per
+ * row in FP64 (2.0 M rows) and 24 B in mixed precision (2.7 M rows); other
+ * preconditioners add one vector (40 / 28 B per row).  A larger system
+ * returns HSPS_ERR_NOMEM before any device work.  Every call on a device
+ * shares that device's array: calls that overlap on one device race (run
+ * them on one stream, or serialise them). */
+int hsps_cg_solve_noalloc(hsps_csr A, hsps_precond M, const double* d_b, double* d_x, const hsps_params* p,
+                          hsps_stats* stats);
 /* PipelinedFGMRESSolver::solve (pipelined_fgmres_solver.cpp:165-243): the
  * same Krylov space as FGMRES; here each iteration orthogonalises with two
  * fused classical Gram-Schmidt passes (a fixed number of launches and
