@@ -158,54 +158,4 @@ std::vector<int> tri_levels(const HostCsr& F, bool lower, int* n_levels) {
     return lev;
 }
 
-std::vector<double> coarse_operator(const HostCsr& A) {
-    const int n = A.rows;
-    // Row-major dense copy; row swaps tracked so the same elimination can be
-    // replayed on each unit right-hand side.
-    std::vector<double> M((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i)
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) M[(size_t)i * n + A.ci[k]] = A.v[k];
-    std::vector<int> piv(n);
-    std::vector<char> skip(n, 0);
-    std::vector<double> factors((size_t)n * n, 0.0);  // factor[r][col] used in elimination
-    for (int col = 0; col < n; ++col) {
-        int p = col;
-        for (int r = col + 1; r < n; ++r)
-            if (std::abs(M[(size_t)r * n + col]) > std::abs(M[(size_t)p * n + col])) p = r;
-        piv[col] = p;
-        if (p != col)
-            for (int c = 0; c < n; ++c) std::swap(M[(size_t)col * n + c], M[(size_t)p * n + c]);
-        // Factors of earlier columns stay where they were recorded: the
-        // replay below interleaves swaps and eliminations in the same order.
-        if (std::abs(M[(size_t)col * n + col]) < 1e-14) { skip[col] = 1; continue; }
-        const double ip = 1.0 / M[(size_t)col * n + col];
-#pragma omp parallel for schedule(static)
-        for (int r = col + 1; r < n; ++r) {
-            const double f = M[(size_t)r * n + col] * ip;
-            factors[(size_t)r * n + col] = f;
-            for (int c = col; c < n; ++c) M[(size_t)r * n + c] -= f * M[(size_t)col * n + c];
-        }
-    }
-    // Replay on unit vectors: column e of G = solve(e_e).
-    std::vector<double> G((size_t)n * n, 0.0);
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int e = 0; e < n; ++e) {
-        std::vector<double> b(n, 0.0), x(n, 0.0);
-        b[e] = 1.0;
-        for (int col = 0; col < n; ++col) {
-            std::swap(b[col], b[piv[col]]);
-            if (skip[col]) continue;
-            for (int r = col + 1; r < n; ++r) b[r] -= factors[(size_t)r * n + col] * b[col];
-        }
-        for (int i = n - 1; i >= 0; --i) {
-            double s = b[i];
-            for (int j = i + 1; j < n; ++j) s -= M[(size_t)i * n + j] * x[j];
-            const double d = M[(size_t)i * n + i];
-            x[i] = std::abs(d) > 1e-14 ? s / d : 0.0;
-        }
-        for (int i = 0; i < n; ++i) G[(size_t)i * n + e] = x[i];
-    }
-    return G;
-}
-
 }  // namespace hsps_host

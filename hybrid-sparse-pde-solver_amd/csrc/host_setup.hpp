@@ -33,10 +33,4 @@ HostCsr ilut(const HostCsr& A, double tau, int fill_p);
 // CSR.  Returns level of each row; *n_levels receives the count.
 std::vector<int> tri_levels(const HostCsr& LU, bool lower, int* n_levels);
 
-// Dense operator G (row-major n x n) of the reference's coarsest solve
-// (amg.cpp:260-291): Gaussian elimination with partial pivoting on [A | rhs],
-// pivots < 1e-14 skipped.  The solve is linear in rhs for a fixed A, so G is
-// formed once (columns = solves of unit vectors) and applied as z = G rhs.
-std::vector<double> coarse_operator(const HostCsr& A);
-
 }  // namespace hsps_host

@@ -13,6 +13,7 @@
 #include <limits>
 #include <type_traits>
 
+#include "dense_lu.hpp"
 #include "device_common.hpp"
 #include "host_setup.hpp"
 #include "precond.hpp"
@@ -45,16 +46,19 @@ __global__ void k_copy(int n, const T* __restrict__ a, T* __restrict__ b, const 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
 }
 
-// Dense z = G rhs (row-major n x n), one wave64 per row.
+// Dense z = G rhs (FP64 G, row-major n x n with row stride ld), one wave64
+// per row, FP64 arithmetic in both precisions (the coarse solve of the mixed
+// V-cycle stays FP64).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-k_dense_gemv(int n, const T* __restrict__ G, const T* __restrict__ rhs, T* __restrict__ z, const int* skip) {
+k_dense_gemv(int n, int ld, const double* __restrict__ G, const T* __restrict__ rhs, T* __restrict__ z,
+             const int* skip) {
     if (skip && *skip) return;
     const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     double s = 0.0;
-    const T* g = G + (size_t)row * n;
+    const double* g = G + (size_t)row * ld;
     for (int c = lane; c < n; c += 64) s += (double)g[c] * (double)rhs[c];
     s = wave_sum(s);
     if (lane == 0) z[row] = (T)s;
@@ -552,7 +556,7 @@ static hsps_host::HostCsr host_copy(const Csr& A) {
 }
 
 // Hierarchy on the device (amg_setup.hip), then the coarsest level's dense
-// operator on the host (amg.cpp:260-291).
+// solve operator (amg.cpp:260-291), factored on the device (dense_lu.hip).
 static void setup_amg(Precond& M, const Csr& A, const hsps_params& p) {
     M.amg_omega = p.amg_omega;
     M.pre = p.amg_smooth_pre;
@@ -562,11 +566,9 @@ static void setup_amg(Precond& M, const Csr& A, const hsps_params& p) {
     const int L = (int)M.amg.size();
     for (int lv = 0; lv < L; ++lv) alloc_level_work(M.ctx, M.amg[lv], lv == 0 ? A.rows : M.amg[lv].A->rows);
     const auto t0 = std::chrono::steady_clock::now();
-    const hsps_host::HostCsr Hc = host_copy(L == 1 ? A : *M.amg[L - 1].A);
-    auto G = hsps_host::coarse_operator(Hc);
-    M.coarse_n = Hc.rows;
-    M.coarse_inv64 = upload(M.ctx, G.data(), G.size() * 8);
-    M.coarse_inv32 = upload_f32(M.ctx, G);
+    const Csr& Ac = L == 1 ? A : *M.amg[L - 1].A;
+    M.coarse_n = Ac.rows;
+    dense_inverse_device(M.ctx, Ac, M.coarse_inv, &M.coarse_ld, M.ctx->stream);
     M.amg_times.coarse = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -578,7 +580,6 @@ template <> struct Lv<double> {
     static double* rhs(AmgLevelDev& L) { return L.rhs64.as<double>(); }
     static double* res(AmgLevelDev& L) { return L.res64.as<double>(); }
     static const double* dinv(const AmgLevelDev& L) { return L.dinv64.as<double>(); }
-    static const double* G(const Precond& M) { return M.coarse_inv64.as<double>(); }
 };
 template <> struct Lv<float> {
     static float* x(AmgLevelDev& L) { return L.x32.as<float>(); }
@@ -586,7 +587,6 @@ template <> struct Lv<float> {
     static float* rhs(AmgLevelDev& L) { return L.rhs32.as<float>(); }
     static float* res(AmgLevelDev& L) { return L.res32.as<float>(); }
     static const float* dinv(const AmgLevelDev& L) { return L.dinv32.as<float>(); }
-    static const float* G(const Precond& M) { return M.coarse_inv32.as<float>(); }
 };
 
 static void smooth(const Csr& A, const double* d, double w, const double* rhs, const double* xi, double* xo,
@@ -624,7 +624,8 @@ static void vcycle(const Precond& Mc, const Csr& A0, int lv, const T* rhs, T* ou
     const Csr& A = lv == 0 ? A0 : *Lv_.A;
     const int n = A.rows;
     if (lv == L - 1) {
-        hipLaunchKernelGGL(k_dense_gemv<T>, dim3((n + 3) / 4), dim3(kBlock), 0, s, n, Lv<T>::G(M), rhs, out, skip);
+        hipLaunchKernelGGL(k_dense_gemv<T>, dim3((n + 3) / 4), dim3(kBlock), 0, s, n, M.coarse_ld,
+                           M.coarse_inv.as<double>(), rhs, out, skip);
         HSPS_CHECK_LAUNCH();
         return;
     }

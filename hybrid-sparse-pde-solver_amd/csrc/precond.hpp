@@ -75,7 +75,8 @@ struct Precond {
     double amg_omega = 0.67;
     int pre = 2, post = 2;
     int coarse_n = 0;
-    DevBuf coarse_inv64, coarse_inv32;  // dense inverse of the coarsest operator (row-major)
+    int coarse_ld = 0;   // row stride of coarse_inv (the size padded to 64)
+    DevBuf coarse_inv;   // FP64 solve operator of the coarsest level (row-major; both precisions)
     AmgSetupTimes amg_times;
 };
 

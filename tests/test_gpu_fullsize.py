@@ -146,3 +146,29 @@ def test_convdiff1024_pe100_fgmres_ilu0(H, ctx, full, precision):
     _check(ref, st, xd.download(), b, A, 1e-10, f"C3 FGMRES+ILU(0) {precision}", flops=precision == "fp64")
     M.close()
     Ad.close()
+
+
+def test_poisson2048_coarsest_solve_matches_gepp(H, ctx, poisson2048):
+    """The 861-row coarsest operator of the 2048^2 hierarchy (padded to 896:
+    14 panels), factored on the device, against the oracle's direct_solve
+    restatement; and the time of that factorisation."""
+    _, Ad, _ = poisson2048
+    M = H.Precond(Ad, "amg", H.default_params())
+    L = M.amg_levels()
+    rp, ci, v = M.amg_level_matrix(L - 1, "A")
+    n = len(rp) - 1
+    t_coarse = M.amg_setup_profile()["coarse"]
+    M.close()
+    Ac = O.Csr.from_csr(n, n, rp, ci, v)
+    Cd = H.Csr(ctx, n, n, rp, ci, v)
+    Mc = H.Precond(Cd, "amg", H.default_params(amg_levels=1))
+    r = O.uniform(3, n)
+    rd, zd = ctx.array(r), ctx.array(n=n)
+    Mc.apply(rd, zd)
+    zo = O.gepp_solve(Ac, r)
+    err = np.linalg.norm(zd.download() - zo) / np.linalg.norm(zo)
+    print(f"C2 coarsest level: {n} rows, device factorisation {1e3 * t_coarse:.2f} ms, rel. diff to GEPP {err:.2e}",
+          flush=True)
+    assert err <= 1e-12
+    Mc.close()
+    Cd.close()

@@ -494,3 +494,59 @@ def test_preconditioner_keeps_its_matrix_alive(H, ctx):
     assert e.value.code == -1 and "values changed" in str(e.value)
     MB.close()
     B.close()
+
+
+def _dense_csr(D):
+    r, c = np.nonzero(D)
+    return O.Csr.from_coo(D.shape[0], D.shape[1], r, c, D[r, c])
+
+
+@pytest.mark.parametrize("case", ["poisson5", "one", "rand7", "rand64", "rand65", "pivot130", "rand300", "singular40",
+                                  "rand1300"])
+def test_amg_coarse_solve_matches_gepp(H, ctx, case):
+    """The coarsest level's dense solve, factored on the device (blocked LU
+    with partial pivoting, FP64 MFMA updates), against the oracle's restatement
+    of direct_solve (amg.cpp:260-291).  With amg_levels = 1 (or <= 50 rows) the
+    hierarchy is the matrix alone and the preconditioner is that solve.
+    Sizes around the 64-row padding and the 16-column panels; pivot130 needs a
+    row swap in every column; singular40 has a zero row and column (the pivot
+    below 1e-14 is skipped and that unknown set to 0, as the reference does);
+    rand1300 starts with panels too tall for LDS (the global-memory panel)."""
+    rng = np.random.default_rng(7)
+    if case == "poisson5":
+        A = O.Csr.poisson(5)
+    elif case == "one":
+        A = O.Csr.from_coo(1, 1, [0], [0], [3.0])
+    elif case.startswith("rand"):
+        n = int(case[4:])
+        D = rng.uniform(-1, 1, (n, n)) * (rng.uniform(size=(n, n)) < 0.3)
+        D += np.diag(rng.uniform(2, 4, n) * np.sign(rng.uniform(-1, 1, n)) + 0.0)
+        A = _dense_csr(D)
+    elif case == "pivot130":
+        n = 130
+        D = np.diag(rng.uniform(0.01, 0.02, n)) + np.diag(rng.uniform(3, 4, n - 1), -1)
+        D[0, -1] = 3.5
+        D += 0.05 * rng.uniform(-1, 1, (n, n)) * (rng.uniform(size=(n, n)) < 0.1)
+        A = _dense_csr(D)
+    else:
+        n = 40
+        D = rng.uniform(-1, 1, (n, n)) * (rng.uniform(size=(n, n)) < 0.3) + np.diag(rng.uniform(3, 4, n))
+        D[17, :] = 0.0
+        D[:, 17] = 0.0
+        A = _dense_csr(D)
+    n = A.rows
+    Ad = dev_csr(H, ctx, A)
+    M = H.Precond(Ad, "amg", H.default_params(amg_levels=1))
+    assert M.amg_levels() == 1
+    for seed in (1, 2):
+        r = O.uniform(seed, n)
+        rd, zd = ctx.array(r), ctx.array(n=n)
+        M.apply(rd, zd)
+        z = zd.download()
+        zo = O.gepp_solve(A, r)
+        assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo), (case, np.linalg.norm(z - zo) / np.linalg.norm(zo))
+        if case == "singular40":
+            assert z[17] == 0.0 and zo[17] == 0.0
+    prof = M.amg_setup_profile()
+    print(f"{case}: coarse operator of {n} rows built in {1e3 * prof['coarse']:.2f} ms", flush=True)
+    M.close()
