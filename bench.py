@@ -93,6 +93,62 @@ def spmv_bytes(n, nnz, sv=8, sx=8):
     return nnz * (sv + 4) + 4 * (n + 1) + sx * n + sx * n
 
 
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (lscpu core /
+    socket ids of the affinity set), capped by OMP_NUM_THREADS when it is set
+    (the share of a shared host)."""
+    import subprocess
+    allowed = os.sched_getaffinity(0)
+    cores = set()
+    try:
+        txt = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        for line in txt.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(t) for t in line.split(",")[:3])
+            if cpu in allowed:
+                cores.add((sock, core))
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        cores = set(allowed)
+    n = max(1, len(cores))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return n, len(allowed)
+
+
+def cpu_oracle_runs(specs, b, row_off, sample, threads, warmup, reps, tag, want_x=False):
+    """Times the FP64 OpenMP oracle (oracle/cpu_baseline.py, a child process
+    with its own OpenMP runtime) on the sample systems at `threads` threads
+    (OMP_PROC_BIND=close, OMP_PLACES=cores): `warmup` untimed passes, then the
+    median of `reps` timed ones."""
+    import subprocess
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="hsps_cpu_")
+    inp, out, xs = (os.path.join(tmp, f) for f in ("in.npz", "out.json", "x.npz"))
+    kinds = np.array([0 if specs[s][0] == "poisson" else 1 for s in sample], np.int32)
+    sides = np.array([specs[s][1] for s in sample], np.int32)
+    eps = np.array([specs[s][2] for s in sample], np.float64)
+    bb = [b[row_off[s]:row_off[s + 1]] for s in sample]
+    off = np.zeros(len(sample) + 1, np.int64)
+    np.cumsum([len(v) for v in bb], out=off[1:])
+    np.savez(inp, kinds=kinds, sides=sides, eps=eps, b=np.concatenate(bb), b_off=off, warmup=warmup, reps=reps)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), inp, out] + (["--solutions", xs] if want_x else [])
+    log(f"[bench] cpu baseline ({tag}): {len(sample)} systems, {threads} threads, {warmup} warm-up + {reps} timed passes")
+    subprocess.run(cmd, env=env, check=True)
+    with open(out) as f:
+        res = json.load(f)
+    x = None
+    if want_x:
+        x = np.load(xs)["x"]
+        res["x_off"] = off
+    for path in (inp, out, xs):
+        if os.path.exists(path):
+            os.remove(path)
+    os.rmdir(tmp)
+    return res, x
+
+
 def spmv_roofline(H, ctx, ev, side=2048, reps=50):
     stream = H.lib().hsps_ctx_stream(ctx.h)
     n = side * side
@@ -175,25 +231,26 @@ def bench_batched(H, ctx, ev, args):
             if t1 - t0 >= min_s:
                 return (j1 - j0) / (t1 - t0)
 
-    def batched(precision, steps, warmup):
+    def batched(precision, steps, warmup, batch=None):
+        Bx = batch or B
         p = H.default_params(tol=1e-8, max_iter=20000, precision=precision, check_every=32,
                              batch_tile_rows=args.tile_rows)
         for _ in range(warmup):
             H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
-            B.solve(bd, xd, p)
+            Bx.solve(bd, xd, p)
         times = []
         for _ in range(steps):
             H.check(H.lib().hsps_memset(ctx.h, xd.ptr, 0, xd.nbytes))
             ctx.sync()
             t0 = time.perf_counter()
-            ok, it, cv, tr = B.solve(bd, xd, p)
+            ok, it, cv, tr = Bx.solve(bd, xd, p)
             ctx.sync()
             times.append(time.perf_counter() - t0)
             assert ok and cv.all() and (tr <= 1e-8).all(), (precision, tr.max())
         x = xd.download()
         err = float(np.max([np.linalg.norm(x[row_off[s]:row_off[s + 1]] - xs[row_off[s]:row_off[s + 1]]) /
                             np.linalg.norm(xs[row_off[s]:row_off[s + 1]]) for s in range(len(systems))]))
-        return times, it, tr, err
+        return times, it, tr, err, x
 
     storage = B.storage()
     sizes = np.diff(row_off).astype(np.float64)
@@ -246,7 +303,7 @@ def bench_batched(H, ctx, ev, args):
         Bt.close()
         return out
     log("[bench] batched mixed")
-    t_mx, it_mx, tr_mx, err_mx = batched("mixed", args.steps, args.warmup)
+    t_mx, it_mx, tr_mx, err_mx, x_mx = batched("mixed", args.steps, args.warmup)
     log(f"[bench] batched mixed {np.mean(t_mx):.3f}s")
     row_iters = float(np.dot(sizes, it_mx.astype(np.float64)))
     t_it = (capped(96) - capped(32)) / 64
@@ -263,8 +320,32 @@ def bench_batched(H, ctx, ev, args):
         return {"workload": "batched_only", "value": len(systems) / float(np.mean(t_mx)),
                 "ms_per_step": 1e3 * float(np.mean(t_mx)), "config": {}, "fp64_baseline": None,
                 "speedup_vs_fp64": None, "cpu_baseline": None, "batched_iteration": batched_roof}
-    t_b64, it_b64, tr_b64, _ = batched("fp64", 1, 1)
+    t_b64, it_b64, tr_b64, _, _ = batched("fp64", 1, 1)
     log(f"[bench] batched fp64 {np.mean(t_b64):.3f}s")
+
+    # The same batch with every system stored as CSR, then as DIA (diagonal-major
+    # values, no stencil): the general batched kernels, priced with the
+    # reference's SpMV byte model (energy_monitor.hpp:166-170) in FP32 plus the
+    # PCG iteration's 7 vector passes: per system and iteration
+    # nnz (4 + 4) + 4 (n + 1) + 4 n + 4 n + 7 * 4 n bytes.
+    storage_legs = {}
+    nnz_s = np.array([len(sy[2]) for sy in systems], np.float64)
+    model = nnz_s * 8 + 4 * (sizes + 1) + 8 * sizes + 28 * sizes
+    for cap in ("csr", "dia"):
+        os.environ["HSPS_BATCH_STORAGE"] = cap
+        try:
+            Bc = H.Batch(ctx, row_off, rp, ci, v)
+        finally:
+            del os.environ["HSPS_BATCH_STORAGE"]
+        t_c, it_c, tr_c, _, _ = batched("mixed", max(1, min(args.steps, 2)), 1, Bc)
+        mb = float(np.dot(model, it_c.astype(np.float64)))
+        gbs = mb / float(np.mean(t_c)) / 1e9
+        storage_legs[cap] = {"storage": Bc.storage(), "value": len(systems) / float(np.mean(t_c)), "unit": "solves/s",
+                             "seconds": float(np.mean(t_c)), "total_iterations": int(it_c.sum()),
+                             "max_true_residual": float(tr_c.max()), "model_bytes": mb, "achieved_GBs": gbs,
+                             "frac_of_8TBs": gbs / HBM_PEAK_GBS}
+        log(f"[bench] batched mixed, {cap} storage {np.mean(t_c):.3f}s ({gbs:.0f} GB/s by the CSR byte model)")
+        Bc.close()
 
     # plain FP64 device PCG, one system at a time (cg_engine=0: CSR row-block SpMV and
     # BLAS-1 kernels per iteration); then the same through the one-system batched engine
@@ -296,30 +377,44 @@ def bench_batched(H, ctx, ev, args):
     fp64e_time, iters64e = one_at_a_time(1)
     log(f"[bench] one-at-a-time fp64 (batched engine) {fp64e_time:.3f}s")
 
-    # CPU oracle on a bounded sample, scaled by nnz x iterations
-    cpu = None
+    # CPU oracle: the FP64 OpenMP restatement of the reference on stratified
+    # samples of the same systems (every 16th / 64th by nonzeros), at all
+    # physical cores and at 1 thread; its solutions also check the batched
+    # mixed solutions (every result against the FP64 oracle)
+    cpu, vs_oracle = None, None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        # every 3rd system by size (171 of 512): about 10 s of CPU work on 16 host cores
-        order = np.argsort([len(sy[2]) for sy in systems])
-        step = 3
-        sample = list(order[::step])
-        t0 = time.perf_counter()
-        work_s = 0.0
-        for s in sample:
-            srp, sci, sv = systems[s]
-            n = len(srp) - 1
-            Ao = O.Csr.from_csr(n, n, srp, sci, sv)
-            _, sto = O.solve(Ao, b[row_off[s]:row_off[s + 1]], "cg", "jacobi", tol=1e-8, max_iter=20000)
-            work_s += len(sv) * sto.iterations
-        dt = time.perf_counter() - t0
-        work_all = float(sum(len(sy[2]) * iters64[s] for s, sy in enumerate(systems)))
-        t_all = dt * work_all / work_s
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": len(systems) / t_all, "unit": "solves/s", "cores": cores, "kind": "port",
-               "sample": f"{len(sample)} of the {len(systems)} systems (every {step}rd by size) "
-                         f"solved by the FP64 OpenMP oracle in {dt:.1f} s, scaled by nnz x iterations"}
+        order = np.argsort([len(sy[2]) for sy in systems], kind="stable")
+        sample_p, sample_1 = [int(s) for s in order[8::16]], [int(s) for s in order[32::64]]
+        P, n_cpus = physical_cores()
+        rp_, xo = cpu_oracle_runs(specs, b, row_off, sample_p, P, 1, 3, "all physical cores", want_x=True)
+        r1_, _ = cpu_oracle_runs(specs, b, row_off, sample_1, 1, 1, 3, "1 thread")
+        work = np.array([len(sy[2]) for sy in systems], np.float64) * it_mx.astype(np.float64)
+
+        def ratio(sample):  # the sample's mean work per solve over the batch's
+            return float(work[sample].mean() / work.mean())
+        cpu = {"value": len(sample_p) / rp_["median_s"], "unit": "solves/s", "cores": P, "kind": "port",
+               "sample": f"{len(sample_p)} of the {len(systems)} systems (every 16th by nonzeros), solved by the FP64 "
+                         f"OpenMP oracle at {P} threads (OMP_PROC_BIND=close, OMP_PLACES=cores; {P} physical cores of "
+                         f"the {n_cpus} CPUs this process may use): 1 warm-up pass, median of 3 timed passes "
+                         f"({rp_['median_s']:.2f} s); the sample's mean work per solve (nonzeros x iterations) is "
+                         f"{ratio(sample_p):.3f} x the batch's, and value is the sample's own solves/s (no extrapolation)",
+               "one_thread": {"value": len(sample_1) / r1_["median_s"], "unit": "solves/s", "threads": 1,
+                              "systems": len(sample_1), "median_s": r1_["median_s"], "pass_seconds": r1_["pass_seconds"],
+                              "work_ratio_vs_batch": ratio(sample_1),
+                              "sample": "every 64th system by nonzeros, 1 warm-up + median of 3 passes"},
+               "pass_seconds": rp_["pass_seconds"], "work_ratio_vs_batch": ratio(sample_p)}
+        errs, dits = [], []
+        for i, s in enumerate(sample_p):
+            xo_s = xo[rp_["x_off"][i]:rp_["x_off"][i + 1]]
+            errs.append(float(np.linalg.norm(x_mx[row_off[s]:row_off[s + 1]] - xo_s) / np.linalg.norm(xo_s)))
+            dits.append((int(it_mx[s]), int(rp_["iterations"][i])))
+        vs_oracle = {"systems": len(sample_p), "max_rel_err_vs_oracle": max(errs),
+                     "max_abs_iteration_diff": max(abs(a - o) for a, o in dits),
+                     "max_rel_iteration_diff": max(abs(a - o) / o for a, o in dits),
+                     "iteration_ratio": float(sum(a for a, _ in dits) / sum(o for _, o in dits)),
+                     "what": "batched mixed solution of each sampled system against the FP64 CPU oracle's solution "
+                             "of the same b (relative 2-norm), and the batched mixed iteration counts against the "
+                             "oracle's"}
 
     value = len(systems) / float(np.mean(t_mx))
     fp64_rate = len(systems) / fp64_time
@@ -333,6 +428,9 @@ def bench_batched(H, ctx, ev, args):
         "fp64_baseline": {"value": fp64_rate, "unit": "solves/s", "mode": "plain FP64 device PCG, one system at a time",
                           "total_iterations": int(iters64.sum()), "seconds": fp64_time},
         "batched_fp64": {"value": len(systems) / float(np.mean(t_b64)), "unit": "solves/s"},
+        "batched_mixed_storage": dict(storage_legs, what="the same 512 systems with the storage capped "
+                                      "(HSPS_BATCH_STORAGE=csr | dia) instead of the stencil storage; achieved_GBs by "
+                                      "the reference's CSR SpMV byte model (FP32) + 7 vector passes per iteration"),
         "fp64_engine_one_at_a_time": {"value": len(systems) / fp64e_time, "unit": "solves/s",
                                       "mode": "FP64 PCG+Jacobi, one system at a time, each run as a one-system "
                                               "batch (stencil storage, fused kernels)",
@@ -341,8 +439,15 @@ def bench_batched(H, ctx, ev, args):
                                                                   (len(systems) / fp64e_time)},
         "mixed": {"max_iterations": int(it_mx.max()), "total_iterations": int(it_mx.sum()),
                   "max_true_residual": float(tr_mx.max()), "max_rel_err_vs_xstar": err_mx,
-                  "p50_ms": 1e3 * float(np.percentile(t_mx, 50)), "p99_ms": 1e3 * float(np.percentile(t_mx, 99)),
-                  "iteration_ratio_vs_fp64": float(it_mx.sum() / iters64.sum())},
+                  "batch_time_p50_ms": 1e3 * float(np.percentile(t_mx, 50)),
+                  "batch_time_p99_ms": 1e3 * float(np.percentile(t_mx, 99)),
+                  "iteration_ratio_vs_fp64": float(it_mx.sum() / iters64.sum()),
+                  "vs_oracle": vs_oracle},
+        "speedup_decomposition": {
+            "batched_mixed_over_batched_fp64": (len(systems) / float(np.mean(t_mx))) / (len(systems) / float(np.mean(t_b64))),
+            "batched_fp64_over_plain_fp64_one_at_a_time": (len(systems) / float(np.mean(t_b64))) / (len(systems) / fp64_time),
+            "what": "speedup_vs_fp64 = (mixed over FP64, both batched) x (batched FP64 over the plain FP64 device PCG "
+                    "one system at a time)"},
         "speedup_vs_fp64": value / fp64_rate,
         "cpu_baseline": cpu,
         "batched_iteration": batched_roof,
