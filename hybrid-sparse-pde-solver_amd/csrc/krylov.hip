@@ -883,8 +883,9 @@ static int run_fgmres(const Csr& A, const Precond* M, const double* b, double* x
     // The host reads the cycle state with one poll of lag, so up to two polls
     // of iterations are enqueued past the end of a cycle (their kernels exit
     // at once).  Expensive iterations (an AMG V-cycle, large operators) poll
+    // (and a caller-written apply, whose calls past the end are wasted work)
     // every iteration; cheap ones every 4 to keep the launch queue ahead.
-    const bool costly = (has_m && M->kind == HSPS_PRECOND_AMG) || A.nnz > (1 << 21);
+    const bool costly = (has_m && (M->kind == HSPS_PRECOND_AMG || M->kind == HSPS_PRECOND_USER)) || A.nnz > (1 << 21);
     const int steps_per_poll = costly ? 1 : 4;
     Event ev[2];
     while (total < p.max_iter) {

@@ -40,6 +40,12 @@ __global__ void k_first_sweep(int n, T omega, const T* __restrict__ dinv, const 
         x[i] = omega * dinv[i] * (rhs[i] - T(0));
 }
 
+template <class From, class To>
+__global__ void k_convert(int n, const From* __restrict__ a, To* __restrict__ b, const int* skip) {
+    if (skip && *skip) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (To)a[i];
+}
+
 template <class T>
 __global__ void k_copy(int n, const T* __restrict__ a, T* __restrict__ b, const int* skip) {
     if (skip && *skip) return;
@@ -712,6 +718,23 @@ static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int
                                               "(hsps_csr_update_values); create the preconditioner again");
             vcycle<T>(M, *M.A0, 0, r, z, s, skip);
             break;
+        case HSPS_PRECOND_USER: {  // the caller's apply (preconditioner_base.hpp:32), FP64 vectors
+            int rc;
+            if constexpr (std::is_same<T, double>::value) {
+                rc = M.user_fn(M.user_data, r, z, (void*)s);
+            } else {
+                double* r64 = M.user_r64.as<double>();
+                double* z64 = M.user_z64.as<double>();
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, r64, skip);
+                HSPS_CHECK_LAUNCH();
+                rc = M.user_fn(M.user_data, r64, z64, (void*)s);
+                hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                                   (const double*)z64, z, skip);
+                HSPS_CHECK_LAUNCH();
+            }
+            if (rc < 0) throw Error(rc, "user preconditioner: apply returned status " + std::to_string(rc));
+            break;
+        }
         default:
             throw Error(HSPS_ERR_INVALID, "unknown preconditioner kind");
     }
@@ -764,6 +787,29 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
             *setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         ctx_retain(M->ctx);
         csr_retain(A);
+        *out = M;
+        return HSPS_OK;
+    });
+}
+
+int hsps_precond_create_user(hsps_ctx ctx, int n, hsps_user_apply_fn apply, void* user, hsps_precond* out) {
+    return guarded([&] {
+        if (!ctx || !apply || !out || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_precond_create_user: bad argument");
+        ctx->set_current();
+        auto* M = new hsps_precond_s;
+        M->kind = HSPS_PRECOND_USER;
+        M->ctx = ctx;
+        M->n = n;
+        M->user_fn = apply;
+        M->user_data = user;
+        try {
+            M->user_r64 = DevBuf(ctx, (size_t)std::max(n, 1) * sizeof(double));
+            M->user_z64 = DevBuf(ctx, (size_t)std::max(n, 1) * sizeof(double));
+        } catch (...) {
+            delete M;
+            throw;
+        }
+        ctx_retain(ctx);
         *out = M;
         return HSPS_OK;
     });

@@ -78,6 +78,11 @@ struct Precond {
     int coarse_ld = 0;   // row stride of coarse_inv (the size padded to 64)
     DevBuf coarse_inv;   // FP64 solve operator of the coarsest level (row-major; both precisions)
     AmgSetupTimes amg_times;
+    // caller-written (HSPS_PRECOND_USER): the callback and FP64 staging
+    // vectors for the FP32 loops of mixed precision
+    hsps_user_apply_fn user_fn = nullptr;
+    void* user_data = nullptr;
+    DevBuf user_r64, user_z64;
 };
 
 // Builds M.amg (levels, P, R, D^-1) for A on the device (amg_setup.hip).

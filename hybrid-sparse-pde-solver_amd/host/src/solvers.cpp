@@ -9,6 +9,7 @@
 #include <iomanip>
 #include <iostream>
 
+#include "core/device.hpp"
 #include "energy/energy_monitor.hpp"
 #include "preconditioners/amg.hpp"
 #include "preconditioners/ilu.hpp"
@@ -18,6 +19,7 @@
 #include "solvers/pipelined_fgmres_solver.hpp"
 #include "solvers/sstep_cg_solver.hpp"
 #include "solvers/solver_factory.hpp"
+#include "utils/logger.hpp"
 
 namespace hsps {
 
@@ -46,6 +48,43 @@ hsps_params to_device_params(const SolverParams& p) {
 // ---- preconditioners ------------------------------------------------------------------
 PreconditionerBase::~PreconditionerBase() {
     if (dev_) hsps_precond_destroy(dev_);
+    if (user_dev_) hsps_precond_destroy(user_dev_);
+}
+
+hsps_precond PreconditionerBase::solver_object(Index n) const {
+    if (dev_) return dev_;
+    const hsps_ctx c = DeviceContext::current();
+    if (!user_dev_ || user_n_ != n || user_ctx_ != c) {
+        if (user_dev_) hsps_precond_destroy(user_dev_);
+        user_dev_ = nullptr;
+        dev_check(hsps_precond_create_user(c, n, &PreconditionerBase::user_apply, const_cast<PreconditionerBase*>(this),
+                                           &user_dev_));
+        user_n_ = n;
+        user_ctx_ = c;
+    }
+    return user_dev_;
+}
+
+// The callback the device Krylov loops call for a caller-written subclass:
+// its device_apply(), or its host apply() on host copies of r and z (in
+// stream order: the copies synchronise the context's stream).  Exceptions
+// become a status code that stops the solve.
+int PreconditionerBase::user_apply(void* self, const double* d_r, double* d_z, void* hip_stream) {
+    const auto* P = static_cast<const PreconditionerBase*>(self);
+    try {
+        const Index n = P->user_n_;
+        if (P->device_apply(d_r, d_z, n, hip_stream)) return HSPS_OK;
+        Vector r(n), z(n);
+        dev_check(hsps_memcpy_d2h(P->user_ctx_, r.data(), d_r, sizeof(double) * (size_t)n));
+        P->apply(r, z);
+        if (z.size() != n) throw std::runtime_error("PreconditionerBase::apply: z has the wrong size");
+        const Vector& zc = z;
+        dev_check(hsps_memcpy_h2d(P->user_ctx_, d_z, zc.data(), sizeof(double) * (size_t)n));
+        return HSPS_OK;
+    } catch (const std::exception& e) {
+        HSPS_LOG_WARN("user preconditioner apply failed: ", e.what());
+        return HSPS_ERR_INVALID;
+    }
 }
 
 double PreconditionerBase::setup_device(const SparseMatrix& A, int kind, const hsps_params& p) {
@@ -118,11 +157,7 @@ bool SolverBase::device_solve(int solver, const SparseMatrix& A, const Vector& b
     const Index n = A.rows();
     if (b.size() != n) throw std::runtime_error("solve: b size != rows");
     if (x.size() != n) x.resize(n, REAL_ZERO);  // solver_base.hpp:29-31
-    hsps_precond M = nullptr;
-    if (precond_) {
-        M = precond_->device();
-        if (!M) throw std::invalid_argument("solve: preconditioner has no device object (call setup first)");
-    }
+    hsps_precond M = precond_ ? precond_->solver_object(n) : nullptr;
     hsps_params p = to_device_params(params_);
     if (solver == kSStepCG) p.sstep_k = sstep_k_;
     hsps_stats s{};

@@ -53,6 +53,8 @@ _lib = None
 _vp, _ip, _dp, _fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float)
 _i, _d, _sz = C.c_int, C.c_double, C.c_size_t
 _PP, _SP = C.POINTER(Params), C.POINTER(Stats)
+# hsps_user_apply_fn: int (*)(void* user, const double* d_r, double* d_z, void* hip_stream)
+USER_APPLY = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
 SIGNATURES = {
     "hsps_last_error": (C.c_char_p, []),
     "hsps_params_default": (None, [_PP]),
@@ -82,6 +84,7 @@ SIGNATURES = {
     "hsps_scale_f64": (_i, [_vp, _i, _d, _vp]),
     "hsps_precond_create": (_i, [_vp, _i, _PP, C.POINTER(_vp), _dp]),
     "hsps_precond_destroy": (_i, [_vp]),
+    "hsps_precond_create_user": (_i, [_vp, _i, USER_APPLY, _vp, C.POINTER(_vp)]),
     "hsps_precond_kind": (_i, [_vp]),
     "hsps_precond_apply_f64": (_i, [_vp, _vp, _vp]),
     "hsps_amg_levels": (_i, [_vp]),
@@ -253,6 +256,37 @@ class Csr:
         check(lib().hsps_features(self.h, out.ctypes.data_as(_dp)))
         keys = ["n", "nnz", "dd", "sym", "fro", "cond", "lmax", "spd"]
         return dict(zip(keys, out.tolist()))
+
+
+class UserPrecond:
+    """A caller-written preconditioner (hsps_precond_create_user): apply(d_r,
+    d_z, stream) receives device pointers (ints) of n FP64 entries and must
+    leave z = M^-1 r there (e.g. through hsps_memcpy_* and host numpy)."""
+
+    def __init__(self, ctx, n, apply):
+        self.ctx, self.n = ctx, n
+
+        def tramp(user, r, z, stream):
+            try:
+                apply(r, z, stream)
+                return 0
+            except Exception:  # a negative status stops the solve
+                return -1
+        self._cb = USER_APPLY(tramp)  # kept alive as long as the object
+        h = _vp()
+        check(lib().hsps_precond_create_user(ctx.h, n, self._cb, None, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hsps_precond_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Precond:

@@ -39,7 +39,8 @@ enum {
 /* precision of the Krylov inner loop */
 enum { HSPS_FP64 = 0, HSPS_MIXED = 1 };
 /* preconditioner kinds: PrecondType (types.hpp:40-47) */
-enum { HSPS_PRECOND_NONE = 0, HSPS_PRECOND_JACOBI = 1, HSPS_PRECOND_ILU = 2, HSPS_PRECOND_AMG = 3 };
+enum { HSPS_PRECOND_NONE = 0, HSPS_PRECOND_JACOBI = 1, HSPS_PRECOND_ILU = 2, HSPS_PRECOND_AMG = 3,
+       HSPS_PRECOND_USER = 4 /* caller-written (hsps_precond_create_user) */ };
 /* solver kinds: SolverType (types.hpp:32-38) */
 enum { HSPS_SOLVER_CG = 0, HSPS_SOLVER_FGMRES = 1 };
 
@@ -156,6 +157,18 @@ int hsps_scale_f64(hsps_ctx ctx, int n, double alpha, double* d_x);
 int hsps_precond_create(hsps_csr A, int kind, const hsps_params* p, hsps_precond* out,
                         double* setup_seconds);
 int hsps_precond_destroy(hsps_precond M);
+/* A caller-written preconditioner: the virtual PreconditionerBase::apply
+ * (preconditioner_base.hpp:32) that the reference's CG and FGMRES call each
+ * iteration (cg_solver.cpp:79, 147; fgmres_solver.cpp:112).  The Krylov loops
+ * call apply(user, d_r, d_z, hip_stream) with FP64 device vectors of n
+ * entries (mixed precision converts its FP32 vectors around the call) and
+ * expect z = M^-1 r, the work enqueued on hip_stream (or finished before the
+ * return).  The loops enqueue iterations ahead of their convergence test, so
+ * apply may also be called for iterations whose results are then discarded:
+ * it must be a function of r alone.  A negative return stops the solve with
+ * that status.  The callback runs on the thread that called the solve. */
+typedef int (*hsps_user_apply_fn)(void* user, const double* d_r, double* d_z, void* hip_stream);
+int hsps_precond_create_user(hsps_ctx ctx, int n, hsps_user_apply_fn apply, void* user, hsps_precond* out);
 int hsps_precond_kind(hsps_precond M);
 /* z = M^-1 r in FP64 (z is overwritten; AMG starts from z = 0, amg.cpp:340-343). */
 int hsps_precond_apply_f64(hsps_precond M, const double* d_r, double* d_z);

@@ -551,3 +551,59 @@ def test_amg_coarse_solve_matches_gepp(H, ctx, case):
     prof = M.amg_setup_profile()
     print(f"{case}: coarse operator of {n} rows built in {1e3 * prof['coarse']:.2f} ms", flush=True)
     M.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_user_preconditioner_through_the_c_abi(H, ctx, precision):
+    """hsps_precond_create_user: a caller-written apply (here host numpy code
+    between device copies: z = D^-1 r) inside the device CG and FGMRES loops
+    (cg_solver.cpp:79, 147; fgmres_solver.cpp:112).  FP64: the built-in
+    Jacobi's iterations (the plain kernels) and its solution to 1e-12; mixed
+    (FP32 vectors converted around the FP64 callback): the built-in Jacobi's
+    mixed iterations within 5%; both: the oracle's solution to 1e-6."""
+    import ctypes as C
+    calls = [0]
+    for solver, A in (("cg", O.Csr.aniso(40, 0.1)), ("fgmres", O.Csr.convdiff(40, 0.05, 0.8, 0.6))):
+        n = A.rows
+        rp, ci, v = A.arrays()
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        dinv = 1.0 / v[rows == ci]
+        b = A.spmv(O.uniform(12, n))
+        Ad = dev_csr(H, ctx, A)
+
+        def apply(d_r, d_z, stream, n=n, dinv=dinv):
+            r = np.empty(n)
+            H.check(H.lib().hsps_memcpy_d2h(ctx.h, r.ctypes.data_as(C.c_void_p), C.c_void_p(d_r), 8 * n))
+            z = dinv * r
+            H.check(H.lib().hsps_memcpy_h2d(ctx.h, C.c_void_p(d_z), z.ctypes.data_as(C.c_void_p), 8 * n))
+            calls[0] += 1
+        U = H.UserPrecond(ctx, n, apply)
+        p = H.default_params(tol=1e-8, max_iter=5000, restart_size=30, precision=precision, cg_engine=0)
+        J = H.Precond(Ad, "jacobi", p)
+        res = {}
+        for name, M in (("jacobi", J), ("user", U)):
+            bd, xd = ctx.array(b), ctx.array(n=n)
+            ok, st = H.solve(Ad, bd, xd, solver, M, p)
+            assert ok and st.true_residual < 1e-8, (solver, name)
+            res[name] = (st.iterations, xd.download())
+        xo, so = O.solve(A, b, solver, "jacobi", tol=1e-8, max_iter=5000, restart_size=30)
+        ju, jj = res["user"][0], res["jacobi"][0]
+        print(f"{solver} {precision}: user {ju}, built-in Jacobi {jj}, oracle {so.iterations} iterations", flush=True)
+        if precision == "fp64":  # the same iterations, the same solution to rounding
+            assert ju == jj, (solver, ju, jj)
+            assert np.linalg.norm(res["user"][1] - res["jacobi"][1]) <= 1e-12 * np.linalg.norm(res["jacobi"][1])
+            assert abs(ju - so.iterations) <= max(1, 0.05 * so.iterations), (solver, ju, so.iterations)
+        else:  # FP32 z rounded from the FP64 callback instead of an FP32 product
+            assert abs(ju - jj) <= max(1, 0.05 * jj), (solver, ju, jj)
+        assert np.linalg.norm(res["user"][1] - xo) <= 1e-6 * np.linalg.norm(xo), solver
+        J.close()
+        if solver == "fgmres":
+            def failing(d_r, d_z, stream):
+                raise RuntimeError("no")
+            F = H.UserPrecond(ctx, n, failing)
+            with pytest.raises(H.HspsError) as e:
+                H.solve(Ad, ctx.array(b), ctx.array(n=n), "fgmres", F, H.default_params(tol=1e-8, precision=precision))
+            assert e.value.code == -1 and "user preconditioner" in str(e.value)
+            F.close()
+        U.close()
+    assert calls[0] > 0
