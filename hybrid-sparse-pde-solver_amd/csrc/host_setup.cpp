@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -58,21 +59,43 @@ HostCsr ilu0(const HostCsr& A) {
     return F;
 }
 
+// ILUT(tau, p), ilu.cpp:94-220.  The reference scans every column k < i of
+// a dense work row; the same arithmetic in the same order visits only the
+// active columns, taken from a min-heap (an elimination only activates
+// columns j > k, so the pops come out in increasing order, each column once),
+// and resets only the columns the row touched.  Same factors, bit for bit;
+// O(nnz(L\U) * row length * log) instead of O(n^2).
 HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     const int n = A.rows;
     std::vector<std::vector<std::pair<int, double>>> L(n), U(n);
     std::vector<double> w(n, 0.0), udiag(n, 0.0);
-    std::vector<char> act(n, 0);
+    std::vector<char> act(n, 0), seen(n, 0);
+    std::vector<int> touched, heap, upper;
+    auto touch = [&](int j) {
+        if (!seen[j]) { seen[j] = 1; touched.push_back(j); }
+    };
+    auto push = [&](int j) {  // min-heap of pending pivot columns
+        heap.push_back(j);
+        std::push_heap(heap.begin(), heap.end(), std::greater<int>());
+    };
     for (int i = 0; i < n; ++i) {
+        touched.clear();
         double rn = 0.0;
         for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
-            w[A.ci[p]] = A.v[p];
-            act[A.ci[p]] = 1;
+            const int j = A.ci[p];
+            w[j] = A.v[p];
+            act[j] = 1;
+            touch(j);
             rn += A.v[p] * A.v[p];
+            if (j < i) push(j);
         }
         const double tau = tau_rel * std::sqrt(rn);
-        // Pivot columns k < i in increasing order (ilu.cpp:121-146).
-        for (int k = 0; k < i; ++k) {
+        std::vector<std::pair<int, double>> Lr, Ur;
+        // pivot columns k < i in increasing order (ilu.cpp:121-146)
+        while (!heap.empty()) {
+            std::pop_heap(heap.begin(), heap.end(), std::greater<int>());
+            const int k = heap.back();
+            heap.pop_back();
             if (!act[k]) continue;
             if (std::abs(w[k]) < tau) { w[k] = 0.0; act[k] = 0; continue; }
             const double ukk = udiag[k];
@@ -82,26 +105,30 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
             for (const auto& e : U[k]) {
                 const int j = e.first;
                 if (j <= k) continue;
-                if (!act[j] && std::abs(lik * e.second) >= tau) act[j] = 1;
+                if (!act[j] && std::abs(lik * e.second) >= tau) {
+                    act[j] = 1;
+                    if (j < i) push(j);
+                }
                 w[j] -= lik * e.second;
+                touch(j);
             }
-        }
-        std::vector<std::pair<int, double>> Lr, Ur;
-        for (int j = 0; j < i; ++j) {
-            if (act[j] && std::abs(w[j]) >= tau) Lr.emplace_back(j, w[j]);
-            w[j] = 0.0;
-            act[j] = 0;
+            if (act[k] && std::abs(w[k]) >= tau) Lr.emplace_back(k, w[k]);  // ilu.cpp:151-155, ascending k
         }
         double d = w[i];
         if (std::abs(d) < kEps) d = (d >= 0 ? 1.0 : -1.0) * kEps * 1e6;
         Ur.emplace_back(i, d);
-        w[i] = 0.0;
-        act[i] = 0;
-        for (int j = i + 1; j < n; ++j) {
-            if (act[j] && std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
+        upper.clear();
+        for (int j : touched)
+            if (j > i && act[j] && std::abs(w[j]) >= tau) upper.push_back(j);
+        std::sort(upper.begin(), upper.end());
+        for (int j : upper) Ur.emplace_back(j, w[j]);
+        for (int j : touched) {
             w[j] = 0.0;
             act[j] = 0;
+            seen[j] = 0;
         }
+        w[i] = 0.0;
+        act[i] = 0;
         const int keep = A.rp[i + 1] - A.rp[i] + fill_p;
         auto limit = [keep](std::vector<std::pair<int, double>>& row) {
             if (keep <= 0 || (int)row.size() <= keep) return;
@@ -121,6 +148,10 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     HostCsr F;
     F.rows = F.cols = n;
     F.rp.assign(n + 1, 0);
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += L[i].size() + U[i].size();
+    F.ci.reserve(total);
+    F.v.reserve(total);
     for (int i = 0; i < n; ++i) {
         for (const auto& e : L[i]) { F.ci.push_back(e.first); F.v.push_back(e.second); }
         for (const auto& e : U[i]) { F.ci.push_back(e.first); F.v.push_back(e.second); }

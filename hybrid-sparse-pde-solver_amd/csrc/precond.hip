@@ -792,6 +792,32 @@ int hsps_precond_create(hsps_csr A, int kind, const hsps_params* pp, hsps_precon
     });
 }
 
+int hsps_ilu_factor(int n, const int* rp, const int* ci, const double* v, double drop_tol, int fill_per_row,
+                    int* nnz_out, int* lu_rp, int* lu_ci, double* lu_v) {
+    return guarded([&] {
+        if (n < 0 || !rp || !nnz_out || (rp[n] > 0 && (!ci || !v)))
+            throw Error(HSPS_ERR_INVALID, "hsps_ilu_factor: bad argument");
+        hsps_host::HostCsr A;
+        A.rows = A.cols = n;
+        A.rp.assign(rp, rp + n + 1);
+        A.ci.assign(ci, ci + rp[n]);
+        A.v.assign(v, v + rp[n]);
+        hsps_host::HostCsr F;
+        try {
+            F = (drop_tol > 0.0 || fill_per_row > 0) ? hsps_host::ilut(A, drop_tol, fill_per_row) : hsps_host::ilu0(A);
+        } catch (const std::runtime_error& e) {
+            throw Error(HSPS_ERR_PIVOT, e.what());
+        }
+        *nnz_out = F.nnz();
+        if (lu_rp) std::copy(F.rp.begin(), F.rp.end(), lu_rp);
+        if (lu_ci && lu_v) {
+            std::copy(F.ci.begin(), F.ci.end(), lu_ci);
+            std::copy(F.v.begin(), F.v.end(), lu_v);
+        }
+        return HSPS_OK;
+    });
+}
+
 int hsps_precond_create_user(hsps_ctx ctx, int n, hsps_user_apply_fn apply, void* user, hsps_precond* out) {
     return guarded([&] {
         if (!ctx || !apply || !out || n < 0) throw Error(HSPS_ERR_INVALID, "hsps_precond_create_user: bad argument");

@@ -87,6 +87,7 @@ SIGNATURES = {
     "hsps_precond_create_user": (_i, [_vp, _i, USER_APPLY, _vp, C.POINTER(_vp)]),
     "hsps_precond_kind": (_i, [_vp]),
     "hsps_precond_apply_f64": (_i, [_vp, _vp, _vp]),
+    "hsps_ilu_factor": (_i, [_i, _ip, _ip, _dp, _d, _i, _ip, _ip, _ip, _dp]),
     "hsps_amg_levels": (_i, [_vp]),
     "hsps_amg_level_info": (_i, [_vp, _i, _ip, _ip]),
     "hsps_amg_aggregates": (_i, [_vp, _i, _ip]),
@@ -352,6 +353,24 @@ class Precond:
                                           v.ctypes.data_as(_dp)))
         z = rp[-1]
         return rp, ci[:z], v[:z]
+
+
+def ilu_factor(rp, ci, v, drop_tol=0.0, fill_per_row=0):
+    """L\\U factors (row_ptr, col_idx, values) of a host CSR matrix, computed
+    on the host (hsps_ilu_factor): ILU(0), or ILUT(drop_tol, fill_per_row)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    n = len(rp) - 1
+    nz = C.c_int()
+    args = (n, rp.ctypes.data_as(_ip), ci.ctypes.data_as(_ip), v.ctypes.data_as(_dp), drop_tol, fill_per_row)
+    check(lib().hsps_ilu_factor(*args, C.byref(nz), None, None, None))
+    lrp = np.zeros(n + 1, np.int32)
+    lci = np.zeros(max(nz.value, 1), np.int32)
+    lv = np.zeros(max(nz.value, 1), np.float64)
+    check(lib().hsps_ilu_factor(*args, C.byref(nz), lrp.ctypes.data_as(_ip), lci.ctypes.data_as(_ip),
+                                lv.ctypes.data_as(_dp)))
+    return lrp, lci[:nz.value], lv[:nz.value]
 
 
 def concat_systems(systems):

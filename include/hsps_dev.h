@@ -172,6 +172,15 @@ int hsps_precond_create_user(hsps_ctx ctx, int n, hsps_user_apply_fn apply, void
 int hsps_precond_kind(hsps_precond M);
 /* z = M^-1 r in FP64 (z is overwritten; AMG starts from z = 0, amg.cpp:340-343). */
 int hsps_precond_apply_f64(hsps_precond M, const double* d_r, double* d_z);
+/* The ILU factors hsps_precond_create(HSPS_PRECOND_ILU) uploads, computed on
+ * the host only (no device call): ILU(0) on A's pattern (ilu.cpp:41-89) when
+ * drop_tol == 0 and fill_per_row == 0, else ILUT(drop_tol, fill_per_row)
+ * (ilu.cpp:94-220, the same factors as the reference's dense row scan, in
+ * O(nnz) work) as one L\U CSR (unit L implicit).  *nnz_out receives the
+ * factor's nonzeros; lu_rp (n+1 ints) and lu_ci / lu_v (nnz_out entries) are
+ * filled where not NULL, so a first call with NULL arrays sizes them. */
+int hsps_ilu_factor(int n, const int* h_row_ptr, const int* h_col_idx, const double* h_values, double drop_tol,
+                    int fill_per_row, int* nnz_out, int* h_lu_row_ptr, int* h_lu_col_idx, double* h_lu_values);
 /* AMG introspection: number of levels; rows/nnz of level lv; the host
  * aggregate map of level lv (length rows(lv); lv < levels-1). */
 int hsps_amg_levels(hsps_precond M);

@@ -607,3 +607,20 @@ def test_user_preconditioner_through_the_c_abi(H, ctx, precision):
             F.close()
         U.close()
     assert calls[0] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fgmres_ilut_matches_oracle(H, ctx, precision):
+    """FGMRES(30) + ILUT(1e-4), the reference factory's MODERATE state
+    (solver_factory.cpp:73-79, ilu.cpp:94-220), on conv-diff 128^2: the factors
+    come from the sparse-scan ILUT (bit-identical to the dense scan, checked on
+    the CPU), the sweeps run their fill-in dependencies."""
+    A = O.Csr.convdiff(128, 0.01, CV, SV)
+    b = A.spmv(O.uniform(19, A.rows))
+    kw = dict(tol=1e-8, max_iter=2000, restart_size=30, drop_tol=1e-4)
+    ok, st, x = _solve_dev(H, ctx, A, b, "fgmres", "ilu", precision=precision, **kw)
+    xo, so = O.solve(A, b, "fgmres", "ilu", **kw)
+    print(f"ILUT {precision}: {st.iterations} iterations (oracle {so.iterations})", flush=True)
+    assert ok and _rel(A, b, x) < 1e-8
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert abs(st.iterations - so.iterations) <= max(1, 0.05 * so.iterations), (st.iterations, so.iterations)
