@@ -30,6 +30,7 @@
 // tol to confirm convergence against the FP64 true residual).
 // =============================================================================
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -1052,8 +1053,8 @@ struct OpF2 {
 
 // ---- final: x64 += x_lo and the FP64 true residual per (large) system ----------------------
 __global__ void __launch_bounds__(kBlock)
-k_b_flush_tiles(BArgs B, double* __restrict__ x64, float* __restrict__ xlo) {
-    const int r0 = B.tile_r0[blockIdx.x], r1 = B.tile_r1[blockIdx.x];
+k_b_flush_tiles(BArgs B, int t0, double* __restrict__ x64, float* __restrict__ xlo) {
+    const int r0 = B.tile_r0[t0 + blockIdx.x], r1 = B.tile_r1[t0 + blockIdx.x];
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
         x64[i] += (double)xlo[i];
         xlo[i] = 0.0f;
@@ -1448,6 +1449,99 @@ static void build_waves(Batch& B, size_t budget) {
     if (start < B.n_large) B.waves.push_back({start, B.n_large});
 }
 
+// Storage of one system with rows [a, e) of the global CSR (columns global):
+// its kind and tables, the presence bytes of its rows (mask, e - a entries)
+// and, for SK_DIA, its diagonal-major values (nd * (e - a), zero where absent).
+struct SysStorage {
+    int kind = SK_CSR, nd = 0, cdok = 0;
+    int off[kMaxDiags] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double uv[kMaxDiags] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double cd = 0.0;
+    std::vector<unsigned char> mask;
+    std::vector<double> dia;
+};
+
+static int storage_cap() {
+    // HSPS_BATCH_STORAGE=csr keeps every system in CSR; =dia allows DIA but not stencil
+    const char* cap_env = std::getenv("HSPS_BATCH_STORAGE");
+    return !cap_env ? SK_STENCIL : !std::strcmp(cap_env, "csr") ? SK_CSR : !std::strcmp(cap_env, "dia") ? SK_DIA : SK_STENCIL;
+}
+
+static SysStorage analyze_system(int a, int e, const int* rp, const int* ci, const double* v, const double* dinv,
+                                 int cap) {
+    SysStorage st;
+    st.mask.assign((size_t)(e - a), 0);
+    if (e > a) {
+        bool same = true;
+        for (int i = a + 1; i < e && same; ++i) same = dinv[i] == dinv[a];
+        if (same) {
+            st.cdok = 1;
+            st.cd = dinv[a];
+        }
+    }
+    const long long nz = rp[e] - rp[a];
+    std::vector<int> o;
+    bool ok = e > a;
+    for (int i = a; i < e && ok; ++i)
+        for (int q = rp[i]; q < rp[i + 1] && ok; ++q) {
+            const int d = ci[q] - i;
+            if (std::find(o.begin(), o.end(), d) == o.end()) {
+                if ((int)o.size() == kMaxDiags) ok = false;
+                else o.push_back(d);
+            }
+        }
+    if (!ok || cap == SK_CSR) return st;
+    std::sort(o.begin(), o.end());
+    const int m = (int)o.size();
+    st.nd = m;
+    for (int d = 0; d < m; ++d) st.off[d] = o[d];
+    // one value per diagonal, no repeated column in a row?
+    std::vector<char> seen(m, 0);
+    bool uniform = true;
+    for (int i = a; i < e && uniform; ++i) {
+        unsigned bits = 0;
+        for (int q = rp[i]; q < rp[i + 1] && uniform; ++q) {
+            const int d = (int)(std::lower_bound(o.begin(), o.end(), ci[q] - i) - o.begin());
+            if (bits & (1u << d)) uniform = false;
+            bits |= 1u << d;
+            double& u = st.uv[d];
+            if (!seen[d]) { seen[d] = 1; u = v[q]; }
+            else if (!(v[q] == u) || std::signbit(v[q]) != std::signbit(u)) uniform = false;
+        }
+        st.mask[i - a] = (unsigned char)bits;
+    }
+    if (uniform && cap == SK_STENCIL && st.cdok) {  // (one diagonal value: D^-1 is a scalar)
+        st.kind = SK_STENCIL;
+        return st;
+    }
+    std::fill(st.mask.begin(), st.mask.end(), (unsigned char)0);
+    for (int d = 0; d < kMaxDiags; ++d) st.uv[d] = 0.0;
+    if ((long long)m * (e - a) * 2 > nz * 3) {
+        st.nd = 0;
+        return st;
+    }
+    st.kind = SK_DIA;
+    const int n_s = e - a;
+    st.dia.assign((size_t)m * n_s, 0.0);
+    for (int i = a; i < e; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q) {
+            const int d = (int)(std::lower_bound(o.begin(), o.end(), ci[q] - i) - o.begin());
+            st.dia[(size_t)d * n_s + (i - a)] += v[q];
+        }
+    return st;
+}
+
+// The LDS staging window (doubles) a diagonal-stored system's chunk and halo
+// take in the precision with element size es; 0 when no window up to
+// kStageMaxDoubles holds them (rows then come from memory).
+static int stage_doubles_of(const SysStorage& st, int es) {
+    if (st.kind == SK_CSR) return 0;
+    int h = 0;
+    for (int d = 0; d < st.nd; ++d) h = std::max(h, std::abs(st.off[d]));
+    const long long dbl = ((long long)stage_need(h, es) * es + 7) / 8;
+    return dbl <= kStageMaxDoubles ? (int)dbl : 0;
+}
+
 // Storage selection per system.  A system whose nonzeros lie on <= kMaxDiags
 // diagonals is diagonal-stored: SK_STENCIL when every diagonal carries a
 // single value and no row repeats a column (constant-coefficient stencils:
@@ -1463,93 +1557,39 @@ static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci
     std::vector<long long> base(S, 0);
     std::vector<double> uv((size_t)S * kMaxDiags, 0.0), cd(S, 0.0);
     std::vector<unsigned char> mask((size_t)B.N + kPad, 0);
-    // HSPS_BATCH_STORAGE=csr keeps every system in CSR; =dia allows DIA but not stencil
-    const char* cap_env = std::getenv("HSPS_BATCH_STORAGE");
-    const int cap = !cap_env ? SK_STENCIL : !std::strcmp(cap_env, "csr") ? SK_CSR
-                                          : !std::strcmp(cap_env, "dia") ? SK_DIA : SK_STENCIL;
+    const int cap = storage_cap();
+    std::vector<SysStorage> sto(S);
     long long total = 0;
     for (int k = 0; k < S; ++k) {
         const int s = B.order[k];
         const int a = row_off[s], e = row_off[s + 1];
-        if (e > a) {
-            bool same = true;
-            for (int i = a + 1; i < e && same; ++i) same = dinv[i] == dinv[a];
-            if (same) {
-                cdok[k] = 1;
-                cd[k] = dinv[a];
-            }
+        sto[k] = analyze_system(a, e, rp, ci, v, dinv.data(), cap);
+        const SysStorage& st = sto[k];
+        kind[k] = st.kind;
+        nd[k] = st.nd;
+        cdok[k] = st.cdok;
+        cd[k] = st.cd;
+        for (int d = 0; d < kMaxDiags; ++d) {
+            offs[(size_t)k * kMaxDiags + d] = st.off[d];
+            uv[(size_t)k * kMaxDiags + d] = st.uv[d];
         }
-        const long long nz = rp[e] - rp[a];
-        std::vector<int> o;
-        bool ok = e > a;
-        for (int i = a; i < e && ok; ++i)
-            for (int q = rp[i]; q < rp[i + 1] && ok; ++q) {
-                const int d = ci[q] - i;
-                if (std::find(o.begin(), o.end(), d) == o.end()) {
-                    if ((int)o.size() == kMaxDiags) ok = false;
-                    else o.push_back(d);
-                }
-            }
-        if (!ok || cap == SK_CSR) continue;
-        std::sort(o.begin(), o.end());
-        const int m = (int)o.size();
-        nd[k] = m;
-        for (int d = 0; d < m; ++d) offs[(size_t)k * kMaxDiags + d] = o[d];
-        // one value per diagonal, no repeated column in a row?
-        std::vector<char> seen(m, 0);
-        bool uniform = true;
-        for (int i = a; i < e && uniform; ++i) {
-            unsigned bits = 0;
-            for (int q = rp[i]; q < rp[i + 1] && uniform; ++q) {
-                const int d = (int)(std::lower_bound(o.begin(), o.end(), ci[q] - i) - o.begin());
-                if (bits & (1u << d)) uniform = false;
-                bits |= 1u << d;
-                double& u = uv[(size_t)k * kMaxDiags + d];
-                if (!seen[d]) { seen[d] = 1; u = v[q]; }
-                else if (!(v[q] == u) || std::signbit(v[q]) != std::signbit(u)) uniform = false;
-            }
-            mask[i] = (unsigned char)bits;
+        std::copy(st.mask.begin(), st.mask.end(), mask.begin() + a);
+        if (st.kind == SK_DIA) {
+            base[k] = total;
+            total += (long long)st.dia.size();
         }
-        if (uniform && cap == SK_STENCIL && cdok[k]) {  // (one diagonal value: D^-1 is a scalar)
-            kind[k] = SK_STENCIL;
-            continue;
-        }
-        for (int i = a; i < e; ++i) mask[i] = 0;
-        for (int d = 0; d < kMaxDiags; ++d) uv[(size_t)k * kMaxDiags + d] = 0.0;
-        if ((long long)m * (e - a) * 2 > nz * 3) {
-            nd[k] = 0;
-            continue;
-        }
-        kind[k] = SK_DIA;
-        base[k] = total;
-        total += (long long)m * (e - a);
     }
     // LDS window per precision: the largest halo of a tiled diagonal-stored system
     for (int prec = 0; prec < 2; ++prec) {
-        const int es = prec ? 4 : 8, G = 16 / es;
+        const int es = prec ? 4 : 8;
         int need = kStageMinDoubles;
-        for (int k = 0; k < std::min(B.n_large, S); ++k) {
-            if (kind[k] == SK_CSR) continue;
-            int h = 0;
-            for (int d = 0; d < nd[k]; ++d) h = std::max(h, std::abs(offs[(size_t)k * kMaxDiags + d]));
-            (void)G;
-            const long long dbl = ((long long)stage_need(h, es) * es + 7) / 8;
-            if (dbl <= kStageMaxDoubles) need = (int)std::max<long long>(need, dbl);  // else: rows from memory
-        }
+        for (int k = 0; k < std::min(B.n_large, S); ++k) need = std::max(need, stage_doubles_of(sto[k], es));
         B.stage_doubles[prec] = need;
     }
     std::vector<double> d64((size_t)std::max<long long>(total, 1), 0.0);
     for (int k = 0; k < S; ++k) {
         ++B.n_kind[kind[k]];
-        if (kind[k] != SK_DIA) continue;
-        const int s = B.order[k];
-        const int a = row_off[s], e = row_off[s + 1], n_s = e - a;
-        const int* o = &offs[(size_t)k * kMaxDiags];
-        for (int i = a; i < e; ++i)
-            for (int q = rp[i]; q < rp[i + 1]; ++q) {
-                const int d = (int)(std::lower_bound(o, o + nd[k], ci[q] - i) - o);
-                d64[(size_t)base[k] + (size_t)d * n_s + (i - a)] += v[q];
-            }
+        if (kind[k] == SK_DIA) std::copy(sto[k].dia.begin(), sto[k].dia.end(), d64.begin() + base[k]);
     }
     auto up = [&](DevBuf& dst, const void* src, size_t bytes) {
         dst = DevBuf(B.ctx, std::max<size_t>(bytes, 8));
@@ -1852,7 +1892,7 @@ static std::vector<BState> run_batch(Batch& B, const double* b, double* x, const
         }
         if (B.T > 0) {
             if (prec == HSPS_MIXED)  // every large system's rows, one workgroup per tile
-                hipLaunchKernelGGL(k_b_flush_tiles, dim3(B.T), dim3(kBlock), 0, s, a, x, B.xlo.as<float>());
+                hipLaunchKernelGGL(k_b_flush_tiles, dim3(B.T), dim3(kBlock), 0, s, a, 0, x, B.xlo.as<float>());
             hipLaunchKernelGGL(k_b_final, dim3(B.T), dim3(kBlock), 0, s, a, 0, b, (const double*)x);
             hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(B.n_large), dim3(kFinishThreads), 0, s, a, 0, (const int*)nullptr, OpFinal{});
             HSPS_CHECK_LAUNCH();
@@ -2041,6 +2081,519 @@ bool solo_cg(Csr& A, const double* b, double* x, const hsps_params& p, SoloResul
     return true;
 }
 
+
+// =============================================================================
+// Sessions: systems join a running batch and leave it at chunk boundaries
+// (no reference counterpart; the serving loop's continuous batching).
+// =============================================================================
+// A session holds the engine's device tables at a fixed capacity (rows,
+// nonzeros, systems, tiles, DIA values).  A joining system gets its own
+// ranges from free lists: rows [row0, row0 + n) and at least one gap row
+// after them (its n + 1 row-pointer entries are its own), row0 a multiple of
+// kSessRowAlign (the five-point kernels deal rows to threads by 16-byte
+// groups, so a system's reductions keep their order only at a fixed
+// alignment; a batch places its first system at row 0), a nonzero range, a system
+// slot, a tile range and, if DIA-stored, a value range.  Its storage
+// (analyze_system), tiles and tile records are the ones it would have alone,
+// and every per-system reduction runs over its own tiles in a fixed order,
+// so its iterates and result are the same bits whichever systems share the
+// session and whenever it joins.  Every system is tiled here (the
+// one-workgroup path of hsps_batch_solve would serialise joins on one
+// stream); small systems take one tile.  A join uploads the system and
+// enqueues its initialisation (r = b - A x0, rho) behind the chunk in
+// flight; the next chunk's active list includes it.  The host reads the
+// systems' states one chunk behind, as run_batch does.
+
+constexpr int kSessRowAlign = 64;
+
+class FreeList {
+public:
+    void reset(long long cap) { free_.assign(1, {0, cap}); }
+    // first fit; -1 when no range of len is free (len 0 takes nothing)
+    long long take(long long len) {
+        if (len <= 0) return 0;
+        for (size_t i = 0; i < free_.size(); ++i)
+            if (free_[i].second >= len) {
+                const long long s = free_[i].first;
+                free_[i].first += len;
+                free_[i].second -= len;
+                if (free_[i].second == 0) free_.erase(free_.begin() + (long)i);
+                return s;
+            }
+        return -1;
+    }
+    void give(long long s, long long len) {
+        if (len <= 0) return;
+        size_t i = 0;
+        while (i < free_.size() && free_[i].first < s) ++i;
+        free_.insert(free_.begin() + (long)i, {s, len});
+        if (i + 1 < free_.size() && free_[i].first + free_[i].second == free_[i + 1].first) {
+            free_[i].second += free_[i + 1].second;
+            free_.erase(free_.begin() + (long)i + 1);
+        }
+        if (i > 0 && free_[i - 1].first + free_[i - 1].second == free_[i].first) {
+            free_[i - 1].second += free_[i].second;
+            free_.erase(free_.begin() + (long)i);
+        }
+    }
+
+private:
+    std::vector<std::pair<long long, long long>> free_;  // (start, length), ascending, disjoint
+};
+
+struct SessSlot {
+    int used = 0;      // holds a system (joined, not yet released)
+    int live = 0;      // still iterating as far as the host knows
+    int done = 0;      // its halt has been reported
+    long long join_seq = 0;  // the first state read-back that includes its initialisation
+    int row0 = 0, n = 0, row_len = 0;  // rows [row0, row0 + row_len) held, n of them the system's
+    long long nz0 = 0, nnz = 0, dia0 = 0, dia_len = 0;
+    int t0 = 0, t1 = 0;
+    int need[2] = {0, 0};  // LDS window in doubles, FP64 / FP32
+};
+
+struct Session {
+    Batch B;  // device tables at capacity: B.S slots, B.N rows (gap rows included), B.T tiles
+    FreeList rows, nzs, tiles, dias;
+    std::vector<int> free_slots;
+    std::vector<SessSlot> slots;
+    hsps_params p{};
+    int prec = HSPS_FP64;
+    int tile_rows = 512, tile_rows_dia = 2048;
+    long long nnz_cap = 0, dia_cap = 0;
+    DevBuf b, x;  // right-hand sides and solutions by row (kPad rows of slack)
+    bool dirty = true;
+    Active act;
+    int stage = kStageMinDoubles;
+    BState* pin = nullptr;  // two read-back slots of slots.size() states
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    long long seq = 0;      // read-backs enqueued so far
+    bool pending = false;
+    std::vector<std::pair<std::array<int, 4>, int>> keys;  // chunk shapes seen (a graph on the second use)
+    ~Session() {
+        if (pin) (void)hipHostFree(pin);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// Tile rows of diagonal-stored systems for a session: the batch autotuner's
+// choice on a probe batch of five-point systems (8 of 300^2 and 4 of 600^2
+// rows, Poisson), made once when the session is created.
+static int probe_tile_rows_dia(Ctx* ctx, const hsps_params& p) {
+    std::vector<int> sides = {300, 300, 300, 300, 300, 300, 300, 300, 600, 600, 600, 600};
+    std::vector<int> row_off(1, 0), rp(1, 0), ci;
+    std::vector<double> v;
+    for (int side : sides) {
+        const int base = row_off.back();
+        for (int i = 0; i < side; ++i)
+            for (int j = 0; j < side; ++j) {
+                const int row = i * side + j;
+                auto push = [&](int c, double a) {
+                    ci.push_back(base + c);
+                    v.push_back(a);
+                };
+                if (i > 0) push(row - side, -1.0);
+                if (j > 0) push(row - 1, -1.0);
+                push(row, 4.0);
+                if (j < side - 1) push(row + 1, -1.0);
+                if (i < side - 1) push(row + side, -1.0);
+                rp.push_back((int)ci.size());
+            }
+        row_off.push_back(base + side * side);
+    }
+    const int N = row_off.back();
+    Batch B;
+    fill_batch(B, ctx, (int)sides.size(), row_off.data(), rp.data(), ci.data(), v.data());
+    ensure_work(B, p.precision == HSPS_MIXED ? HSPS_MIXED : HSPS_FP64);
+    DevBuf bd(ctx, sizeof(double) * (size_t)N), xd(ctx, sizeof(double) * (size_t)N);
+    std::vector<double> ones((size_t)N, 1.0);
+    HSPS_HIP(hipMemcpy(bd.p, ones.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice));
+    HSPS_HIP(hipMemset(xd.p, 0, sizeof(double) * (size_t)N));
+    autotune(B, bd.as<double>(), xd.as<double>(), p);
+    HSPS_HIP(hipStreamSynchronize(ctx->stream));
+    return B.tile_rows_dia;
+}
+
+static void session_alloc(Session& S, Ctx* ctx, int max_systems, long long max_rows, long long max_nnz,
+                          const hsps_params& p) {
+    if (max_systems < 1 || max_rows < 1 || max_nnz < 1) throw Error(HSPS_ERR_INVALID, "hsps_session_create: bad capacity");
+    S.p = p;
+    S.prec = p.precision == HSPS_MIXED ? HSPS_MIXED : HSPS_FP64;
+    if (p.batch_tile_rows > 0) {
+        S.tile_rows = S.tile_rows_dia = std::min(p.batch_tile_rows, 1 << 16);
+    } else {
+        S.tile_rows = 512;
+        S.tile_rows_dia = probe_tile_rows_dia(ctx, p);
+    }
+    Batch& B = S.B;
+    const long long N = max_rows + (long long)max_systems * kSessRowAlign;  // gap rows and alignment
+    if (N + kPad >= (1LL << 31) || max_nnz >= (1LL << 31)) throw Error(HSPS_ERR_INVALID, "hsps_session_create: capacity beyond int32");
+    // a CSR tile ends at tile_rows rows or when the next row would take it past
+    // kLdsNnz nonzeros, so two consecutive tiles hold more than kLdsNnz of them
+    const long long T = N / std::min(S.tile_rows, S.tile_rows_dia) + 2 * (max_nnz / kLdsNnz) + 2LL * max_systems + 1;
+    B.ctx = ctx;
+    B.S = max_systems;
+    B.N = (int)N;
+    B.nnz = max_nnz;
+    B.T = (int)T;
+    B.n_large = max_systems;
+    S.nnz_cap = max_nnz;
+    S.dia_cap = max_nnz + max_nnz / 2 + 1;  // a DIA-stored system keeps <= 1.5 x its nonzeros
+    ctx->set_current();
+    HSPS_HIP(hipStreamCreateWithFlags(&B.aux, hipStreamNonBlocking));
+    HSPS_HIP(hipEventCreateWithFlags(&B.join, hipEventDisableTiming));
+    HSPS_HIP(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+    const size_t n = (size_t)N + kPad, Sn = (size_t)max_systems, Tn = (size_t)T;
+    B.rp = DevBuf(ctx, sizeof(int) * (n + 1));
+    B.ci = DevBuf(ctx, sizeof(int) * (size_t)max_nnz);
+    B.v64 = DevBuf(ctx, sizeof(double) * (size_t)max_nnz);
+    B.v32 = DevBuf(ctx, sizeof(float) * (size_t)max_nnz);
+    B.dinv64 = DevBuf(ctx, sizeof(double) * n);
+    B.dinv32 = DevBuf(ctx, sizeof(float) * n);
+    B.umask = DevBuf(ctx, n);
+    B.dia64 = DevBuf(ctx, sizeof(double) * (size_t)S.dia_cap);
+    B.dia32 = DevBuf(ctx, sizeof(float) * (size_t)S.dia_cap);
+    B.dia_kind = DevBuf(ctx, sizeof(int) * Sn);
+    B.dia_nd = DevBuf(ctx, sizeof(int) * Sn);
+    B.dia_off = DevBuf(ctx, sizeof(int) * Sn * kMaxDiags);
+    B.dia_base = DevBuf(ctx, sizeof(long long) * Sn);
+    B.uv64 = DevBuf(ctx, sizeof(double) * Sn * kMaxDiags);
+    B.uv32 = DevBuf(ctx, sizeof(float) * Sn * kMaxDiags);
+    B.cd_ok = DevBuf(ctx, sizeof(int) * Sn);
+    B.cd64 = DevBuf(ctx, sizeof(double) * Sn);
+    B.cd32 = DevBuf(ctx, sizeof(float) * Sn);
+    B.sys_row0 = DevBuf(ctx, sizeof(int) * Sn);
+    B.sys_row1 = DevBuf(ctx, sizeof(int) * Sn);
+    B.sys_t0 = DevBuf(ctx, sizeof(int) * Sn);
+    B.sys_t1 = DevBuf(ctx, sizeof(int) * Sn);
+    B.tile_r0 = DevBuf(ctx, sizeof(int) * Tn);
+    B.tile_r1 = DevBuf(ctx, sizeof(int) * Tn);
+    B.tile_sys = DevBuf(ctx, sizeof(int) * Tn);
+    B.tv32 = DevBuf(ctx, sizeof(TileVec<float>) * Tn);
+    B.tv64 = DevBuf(ctx, sizeof(TileVec<double>) * Tn);
+    B.tva32 = DevBuf(ctx, sizeof(TileVec<float>) * Tn);
+    B.tva64 = DevBuf(ctx, sizeof(TileVec<double>) * Tn);
+    B.act_sys = DevBuf(ctx, sizeof(int) * Sn);
+    B.partials = DevBuf(ctx, sizeof(double) * 2 * Tn);
+    B.h_tv32.assign(Tn, TileVec<float>{});
+    B.h_tv64.assign(Tn, TileVec<double>{});
+    const size_t pb = sizeof(TileVec<double>) * Tn + sizeof(int) * Sn;
+    for (void*& q : B.pin_act) HSPS_HIP(hipHostMalloc(&q, pb, hipHostMallocDefault));
+    B.pin_act_bytes = pb;
+    ensure_work(B, S.prec);
+    S.b = DevBuf(ctx, sizeof(double) * n);
+    S.x = DevBuf(ctx, sizeof(double) * n);
+    HSPS_HIP(hipMemset(S.b.p, 0, sizeof(double) * n));
+    HSPS_HIP(hipMemset(S.x.p, 0, sizeof(double) * n));
+    HSPS_HIP(hipMemset(B.umask.p, 0, n));
+    HSPS_HIP(hipHostMalloc((void**)&S.pin, sizeof(BState) * Sn * 2, hipHostMallocDefault));
+    HSPS_HIP(hipEventCreateWithFlags(&S.ev[0], hipEventDisableTiming));
+    HSPS_HIP(hipEventCreateWithFlags(&S.ev[1], hipEventDisableTiming));
+    S.rows.reset(N);
+    S.nzs.reset(max_nnz);
+    S.tiles.reset(T);
+    S.dias.reset(S.dia_cap);
+    S.slots.assign(Sn, SessSlot{});
+    S.free_slots.clear();
+    for (int k = max_systems - 1; k >= 0; --k) S.free_slots.push_back(k);
+    B.graph_b = S.b.as<double>();
+    B.graph_x = S.x.as<double>();
+}
+
+template <class T>
+static TileVec<T> session_record(const SysStorage& st, int slot, int t, int r0, int r1, int row0, int n) {
+    constexpr int es = (int)sizeof(T);
+    TileVec<T> d;
+    std::memset(&d, 0, sizeof d);
+    d.r0 = r0;
+    d.r1 = r1;
+    d.sys = slot;
+    d.tile = t;
+    d.row0 = row0;
+    d.last = row0 + n - 1;
+    int halo = 0;
+    for (int j = 0; j < st.nd; ++j) halo = std::max(halo, std::abs(st.off[j]));
+    d.H = stage_halo(halo, es);
+    d.C = stage_chunk(halo, es);
+    for (int j = 0; j < 5; ++j) {
+        d.off[j] = j < st.nd ? st.off[j] : 0;
+        d.uv[j] = j < st.nd ? (T)st.uv[j] : T(0);
+    }
+    d.cd = (T)st.cd;
+    // the five-point path when the system's own window fits the largest one a
+    // launch provides (a batch of this system alone decides the same way)
+    const int need = stage_need(halo, es), cap = kStageMaxDoubles * 8 / es;
+    d.vec = st.kind == SK_STENCIL && st.nd <= 5 && st.cdok && need <= cap
+                ? ((long long)need * es > kStageMinDoubles * 8LL ? 2 : 1)
+                : 0;
+    return d;
+}
+
+// Uploads one n x n system (local CSR) with right-hand side h_b and initial
+// guess d_x0 (device, or null for zero) and enqueues its initialisation.
+// Returns the slot, or -1 when the session has no room for it.
+static int session_add(Session& S, int n, const int* rp, const int* ci, const double* v, const double* h_b,
+                       const double* d_x0) {
+    Batch& B = S.B;
+    if (n < 1 || !rp || !ci || !v || !h_b) throw Error(HSPS_ERR_INVALID, "hsps_session_add: bad argument");
+    const long long nnz = rp[n];
+    if (rp[0] != 0 || nnz < 1) throw Error(HSPS_ERR_INVALID, "hsps_session_add: bad row_ptr");
+    std::vector<double> dinv(n);
+    for (int i = 0; i < n; ++i) {
+        if (rp[i + 1] < rp[i]) throw Error(HSPS_ERR_INVALID, "hsps_session_add: row_ptr not monotone");
+        double d = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            if (ci[k] < 0 || ci[k] >= n) throw Error(HSPS_ERR_INVALID, "hsps_session_add: column out of range");
+            if (ci[k] == i) d = v[k];
+        }
+        if (std::abs(d) < kEps)  // jacobi.cpp:28-30
+            throw Error(HSPS_ERR_ZERO_DIAG, "JacobiPreconditioner: zero diagonal at row " + std::to_string(i));
+        dinv[i] = 1.0 / d;
+    }
+    const SysStorage st = analyze_system(0, n, rp, ci, v, dinv.data(), storage_cap());
+    // tiles (build_tiles' rule, local rows)
+    std::vector<int> tr0, tr1;
+    for (int i = 0; i < n;) {
+        int j = i;
+        if (st.kind != SK_CSR) {
+            j = std::min(n, i + S.tile_rows_dia);
+        } else if (rp[i + 1] - rp[i] > kLdsNnz) {
+            j = i + 1;
+        } else {
+            while (j < n && j - i < S.tile_rows && rp[j + 1] - rp[i] <= kLdsNnz) ++j;
+        }
+        tr0.push_back(i);
+        tr1.push_back(j);
+        i = j;
+    }
+    const int nt = (int)tr0.size();
+    if (S.free_slots.empty()) return -1;
+    // every row range is a multiple of kSessRowAlign long, so every start is aligned
+    const long long row_len = ((long long)n + kSessRowAlign) / kSessRowAlign * kSessRowAlign;
+    const long long row0 = S.rows.take(row_len);
+    if (row0 < 0) return -1;
+    const long long nz0 = S.nzs.take(nnz);
+    if (nz0 < 0) { S.rows.give(row0, row_len); return -1; }
+    const long long t0 = S.tiles.take(nt);
+    if (t0 < 0) { S.rows.give(row0, row_len); S.nzs.give(nz0, nnz); return -1; }
+    const long long dia_len = (long long)st.dia.size();
+    const long long dia0 = st.kind == SK_DIA ? S.dias.take(dia_len) : 0;
+    if (dia0 < 0) {
+        S.rows.give(row0, row_len); S.nzs.give(nz0, nnz); S.tiles.give(t0, nt);
+        return -1;
+    }
+    const int slot = S.free_slots.back();
+    S.free_slots.pop_back();
+    // the system's rows, nonzeros and tables (synchronous copies: the chunk in
+    // flight reads none of these ranges)
+    std::vector<int> grp(n + 1), gci(nnz);
+    for (int i = 0; i <= n; ++i) grp[i] = (int)(nz0 + rp[i]);
+    for (long long k = 0; k < nnz; ++k) gci[k] = (int)(row0 + ci[k]);
+    std::vector<float> v32(v, v + nnz), d32(dinv.begin(), dinv.end());
+    auto put = [&](const DevBuf& dst, size_t off_bytes, const void* src, size_t bytes) {
+        if (bytes) HSPS_HIP(hipMemcpy((char*)dst.p + off_bytes, src, bytes, hipMemcpyHostToDevice));
+    };
+    put(B.rp, sizeof(int) * row0, grp.data(), sizeof(int) * (n + 1));
+    put(B.ci, sizeof(int) * nz0, gci.data(), sizeof(int) * nnz);
+    put(B.v64, sizeof(double) * nz0, v, sizeof(double) * nnz);
+    put(B.v32, sizeof(float) * nz0, v32.data(), sizeof(float) * nnz);
+    put(B.dinv64, sizeof(double) * row0, dinv.data(), sizeof(double) * n);
+    put(B.dinv32, sizeof(float) * row0, d32.data(), sizeof(float) * n);
+    put(B.umask, row0, st.mask.data(), (size_t)n);
+    if (st.kind == SK_DIA) {
+        std::vector<float> f(st.dia.begin(), st.dia.end());
+        put(B.dia64, sizeof(double) * dia0, st.dia.data(), sizeof(double) * dia_len);
+        put(B.dia32, sizeof(float) * dia0, f.data(), sizeof(float) * dia_len);
+    }
+    const int kind = st.kind, nd = st.nd, cdok = st.cdok, r0i = (int)row0, r1i = (int)(row0 + n), t0i = (int)t0,
+              t1i = (int)(t0 + nt);
+    float uv32[kMaxDiags], cd32 = (float)st.cd;
+    for (int d = 0; d < kMaxDiags; ++d) uv32[d] = (float)st.uv[d];
+    put(B.dia_kind, sizeof(int) * slot, &kind, sizeof(int));
+    put(B.dia_nd, sizeof(int) * slot, &nd, sizeof(int));
+    put(B.dia_off, sizeof(int) * slot * kMaxDiags, st.off, sizeof(int) * kMaxDiags);
+    put(B.dia_base, sizeof(long long) * slot, &dia0, sizeof(long long));
+    put(B.uv64, sizeof(double) * slot * kMaxDiags, st.uv, sizeof(double) * kMaxDiags);
+    put(B.uv32, sizeof(float) * slot * kMaxDiags, uv32, sizeof(float) * kMaxDiags);
+    put(B.cd_ok, sizeof(int) * slot, &cdok, sizeof(int));
+    put(B.cd64, sizeof(double) * slot, &st.cd, sizeof(double));
+    put(B.cd32, sizeof(float) * slot, &cd32, sizeof(float));
+    put(B.sys_row0, sizeof(int) * slot, &r0i, sizeof(int));
+    put(B.sys_row1, sizeof(int) * slot, &r1i, sizeof(int));
+    put(B.sys_t0, sizeof(int) * slot, &t0i, sizeof(int));
+    put(B.sys_t1, sizeof(int) * slot, &t1i, sizeof(int));
+    std::vector<int> g0(nt), g1(nt), gs(nt, slot);
+    std::vector<TileVec<float>> r32(nt);
+    std::vector<TileVec<double>> r64(nt);
+    for (int t = 0; t < nt; ++t) {
+        g0[t] = r0i + tr0[t];
+        g1[t] = r0i + tr1[t];
+        r32[t] = session_record<float>(st, slot, t0i + t, g0[t], g1[t], r0i, n);
+        r64[t] = session_record<double>(st, slot, t0i + t, g0[t], g1[t], r0i, n);
+        B.h_tv32[t0i + t] = r32[t];
+        B.h_tv64[t0i + t] = r64[t];
+    }
+    put(B.tile_r0, sizeof(int) * t0, g0.data(), sizeof(int) * nt);
+    put(B.tile_r1, sizeof(int) * t0, g1.data(), sizeof(int) * nt);
+    put(B.tile_sys, sizeof(int) * t0, gs.data(), sizeof(int) * nt);
+    put(B.tv32, sizeof(TileVec<float>) * t0, r32.data(), sizeof(TileVec<float>) * nt);
+    put(B.tv64, sizeof(TileVec<double>) * t0, r64.data(), sizeof(TileVec<double>) * nt);
+    put(S.b, sizeof(double) * row0, h_b, sizeof(double) * n);
+    hipStream_t s = B.ctx->stream;
+    double* xr = S.x.as<double>() + row0;
+    if (d_x0) HSPS_HIP(hipMemcpyAsync(xr, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    else HSPS_HIP(hipMemsetAsync(xr, 0, sizeof(double) * n, s));
+    // initialisation behind the chunk in flight (run_batch's init, this system's tiles only)
+    const BArgs a = args_of(B, S.p);
+    const double* bd = S.b.as<double>();
+    hipLaunchKernelGGL(k_b_init1, dim3(nt), dim3(kBlock), 0, s, a, t0i, bd, (const double*)S.x.as<double>(),
+                       B.r64.as<double>());
+    hipLaunchKernelGGL((k_b_finish<2, OpInit1>), dim3(1), dim3(kFinishThreads), 0, s, a, slot, (const int*)nullptr,
+                       OpInit1{});
+    if (S.prec == HSPS_MIXED)
+        hipLaunchKernelGGL((k_b_init2<float, true>), dim3(nt), dim3(kBlock), 0, s, a, t0i,
+                           (const double*)B.r64.as<double>(), B.r.as<float>(), B.p.as<float>(), B.xlo.as<float>());
+    else
+        hipLaunchKernelGGL((k_b_init2<double, false>), dim3(nt), dim3(kBlock), 0, s, a, t0i,
+                           (const double*)B.r64.as<double>(), B.r.as<double>(), B.p.as<double>(), (float*)nullptr);
+    hipLaunchKernelGGL((k_b_finish<1, OpInit2>), dim3(1), dim3(kFinishThreads), 0, s, a, slot, (const int*)nullptr,
+                       OpInit2{});
+    HSPS_CHECK_LAUNCH();
+    SessSlot& sl = S.slots[slot];
+    sl = SessSlot{};
+    sl.used = sl.live = 1;
+    sl.join_seq = S.seq;
+    sl.row0 = r0i;
+    sl.n = n;
+    sl.row_len = (int)row_len;
+    sl.nz0 = nz0;
+    sl.nnz = nnz;
+    sl.dia0 = dia0;
+    sl.dia_len = st.kind == SK_DIA ? dia_len : 0;
+    sl.t0 = t0i;
+    sl.t1 = t1i;
+    sl.need[0] = stage_doubles_of(st, 8);
+    sl.need[1] = stage_doubles_of(st, 4);
+    S.dirty = true;
+    return slot;
+}
+
+// Stages the records of the live systems' tiles and the systems themselves
+// and copies both behind the work on the stream (pinned double buffer).
+template <class T>
+static void session_upload_active(Session& S) {
+    Batch& B = S.B;
+    const std::vector<TileVec<T>>& tv = host_records<T>(B);
+    char* base = (char*)B.pin_act[B.pin_next];
+    B.pin_next ^= 1;
+    TileVec<T>* rec = (TileVec<T>*)base;
+    int* sys = (int*)(base + sizeof(TileVec<T>) * (size_t)B.T);
+    Active A;
+    int need = kStageMinDoubles;
+    for (int k = 0; k < (int)S.slots.size(); ++k) {
+        const SessSlot& sl = S.slots[k];
+        if (!sl.used || !sl.live) continue;
+        sys[A.ns++] = k;
+        need = std::max(need, sl.need[sizeof(T) == 4 ? 1 : 0]);
+        for (int t = sl.t0; t < sl.t1; ++t) {
+            rec[A.nt++] = tv[t];
+            A.kinds |= tv[t].vec == 1 ? 1 : tv[t].vec == 2 ? 4 : 2;
+        }
+    }
+    hipStream_t s = B.ctx->stream;
+    if (A.nt > 0) {
+        HSPS_HIP(hipMemcpyAsync((sizeof(T) == 4 ? B.tva32 : B.tva64).p, rec, sizeof(TileVec<T>) * (size_t)A.nt,
+                                hipMemcpyHostToDevice, s));
+        HSPS_HIP(hipMemcpyAsync(B.act_sys.p, sys, sizeof(int) * (size_t)A.ns, hipMemcpyHostToDevice, s));
+    }
+    S.act = A;
+    S.stage = need;
+}
+
+// One chunk over the live systems; then the states of the chunk before are
+// read and the slots whose solves ended since the last call are returned.
+static std::vector<int> session_step(Session& S) {
+    Batch& B = S.B;
+    hipStream_t s = B.ctx->stream;
+    const int chunk = std::max(1, std::min(S.p.check_every > 0 ? S.p.check_every : 32, 64));
+    if (S.dirty) {
+        if (S.prec == HSPS_MIXED) session_upload_active<float>(S);
+        else session_upload_active<double>(S);
+        S.dirty = false;
+    }
+    const int Sn = (int)S.slots.size();
+    if (S.act.nt > 0) {
+        BArgs a = args_of(B, S.p);
+        a.stage_doubles = S.stage;
+        const std::array<int, 4> key = {S.act.nt, S.act.ns, S.act.kinds, S.stage};
+        int uses = 0;
+        for (auto& kv : S.keys)
+            if (kv.first == key) uses = ++kv.second;
+        if (!uses) {
+            if (S.keys.size() >= 64) S.keys.erase(S.keys.begin());
+            S.keys.push_back({key, 1});
+            uses = 1;
+        }
+        const bool graph = S.p.batch_graphs != 0 && uses >= 2;  // capture a shape seen before
+        launch_chunk(B, a, 0, chunk, S.act, S.b.as<double>(), S.x.as<double>(), S.prec, s, graph);
+    }
+    BState* slotp = S.pin + (size_t)(S.seq & 1) * Sn;
+    HSPS_HIP(hipMemcpyAsync(slotp, B.st.p, sizeof(BState) * (size_t)Sn, hipMemcpyDeviceToHost, s));
+    HSPS_HIP(hipEventRecord(S.ev[S.seq & 1], s));
+    const long long this_seq = S.seq++;
+    std::vector<int> out;
+    if (S.pending) {
+        const long long prev_seq = this_seq - 1;
+        HSPS_HIP(hipEventSynchronize(S.ev[prev_seq & 1]));
+        const BState* prev = S.pin + (size_t)(prev_seq & 1) * Sn;
+        for (int k = 0; k < Sn; ++k) {
+            SessSlot& sl = S.slots[k];
+            if (!sl.used || !sl.live || sl.join_seq > prev_seq) continue;
+            if (prev[k].halt) {
+                sl.live = 0;
+                sl.done = 1;
+                out.push_back(k);
+                S.dirty = true;
+            }
+        }
+    }
+    S.pending = true;
+    return out;
+}
+
+// The result of a slot whose solve ended: x += x_lo, the FP64 true residual,
+// then the state and x (host and / or device copies); the slot is released.
+static BState session_take(Session& S, int slot, double* h_x, double* d_x) {
+    Batch& B = S.B;
+    if (slot < 0 || slot >= (int)S.slots.size() || !S.slots[slot].used || !S.slots[slot].done)
+        throw Error(HSPS_ERR_INVALID, "hsps_session_result: the slot holds no finished solve");
+    SessSlot& sl = S.slots[slot];
+    hipStream_t s = B.ctx->stream;
+    const BArgs a = args_of(B, S.p);
+    const int nt = sl.t1 - sl.t0;
+    double* x = S.x.as<double>();
+    if (S.prec == HSPS_MIXED)
+        hipLaunchKernelGGL(k_b_flush_tiles, dim3(nt), dim3(kBlock), 0, s, a, sl.t0, x, B.xlo.as<float>());
+    hipLaunchKernelGGL(k_b_final, dim3(nt), dim3(kBlock), 0, s, a, sl.t0, (const double*)S.b.as<double>(),
+                       (const double*)x);
+    hipLaunchKernelGGL((k_b_finish<1, OpFinal>), dim3(1), dim3(kFinishThreads), 0, s, a, slot, (const int*)nullptr,
+                       OpFinal{});
+    HSPS_CHECK_LAUNCH();
+    BState st{};
+    HSPS_HIP(hipMemcpyAsync(&st, B.st.as<BState>() + slot, sizeof(BState), hipMemcpyDeviceToHost, s));
+    if (h_x) HSPS_HIP(hipMemcpyAsync(h_x, x + sl.row0, sizeof(double) * sl.n, hipMemcpyDeviceToHost, s));
+    if (d_x) HSPS_HIP(hipMemcpyAsync(d_x, x + sl.row0, sizeof(double) * sl.n, hipMemcpyDeviceToDevice, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+    S.rows.give(sl.row0, sl.row_len);
+    S.nzs.give(sl.nz0, sl.nnz);
+    S.tiles.give(sl.t0, sl.t1 - sl.t0);
+    S.dias.give(sl.dia0, sl.dia_len);
+    sl = SessSlot{};
+    S.free_slots.push_back(slot);
+    return st;
+}
+
 }  // namespace hsps_dev
 
 using namespace hsps_dev;
@@ -2119,6 +2672,93 @@ int hsps_batch_storage(hsps_batch B, int* n_csr, int* n_dia, int* n_stencil) {
         if (n_stencil) *n_stencil = B->n_kind[SK_STENCIL];
         return HSPS_OK;
     });
+}
+
+}  // extern "C"
+
+// ---- sessions: C ABI -------------------------------------------------------------------------
+struct hsps_session_s : Session {
+    std::vector<int> finished;  // slots whose end was observed, not yet handed out by step
+};
+
+extern "C" {
+
+int hsps_session_create(hsps_ctx ctx, int max_systems, long long max_rows, long long max_nnz, const hsps_params* pp,
+                        hsps_session* out) {
+    return guarded([&] {
+        if (!ctx || !out) throw Error(HSPS_ERR_INVALID, "hsps_session_create: null argument");
+        hsps_params p;
+        if (pp) p = *pp; else hsps_params_default(&p);
+        auto* S = new hsps_session_s;
+        try {
+            session_alloc(*S, ctx, max_systems, max_rows, max_nnz, p);
+        } catch (...) {
+            delete S;
+            throw;
+        }
+        ctx_retain(ctx);
+        *out = S;
+        return HSPS_OK;
+    });
+}
+
+int hsps_session_destroy(hsps_session S) {
+    return guarded([&] {
+        if (S) {
+            Ctx* c = S->B.ctx;
+            (void)hipSetDevice(c->device);
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipStreamSynchronize(S->B.aux);
+            delete S;
+            ctx_release(c);
+        }
+        return HSPS_OK;
+    });
+}
+
+int hsps_session_add(hsps_session S, int n, const int* rp, const int* ci, const double* v, const double* b,
+                     const double* d_x0, int* slot) {
+    return guarded([&] {
+        if (!S || !slot) throw Error(HSPS_ERR_INVALID, "hsps_session_add: null argument");
+        S->B.ctx->set_current();
+        const int k = session_add(*S, n, rp, ci, v, b, d_x0);
+        if (k < 0) throw Error(HSPS_ERR_NOMEM, "hsps_session_add: the session has no room for the system now");
+        *slot = k;
+        return HSPS_OK;
+    });
+}
+
+int hsps_session_step(hsps_session S, int* done, int cap, int* n_done) {
+    return guarded([&] {
+        if (!S || !n_done || (cap > 0 && !done)) throw Error(HSPS_ERR_INVALID, "hsps_session_step: null argument");
+        S->B.ctx->set_current();
+        for (int k : session_step(*S)) S->finished.push_back(k);
+        const int m = std::min<int>(std::max(cap, 0), (int)S->finished.size());
+        std::copy(S->finished.begin(), S->finished.begin() + m, done);
+        S->finished.erase(S->finished.begin(), S->finished.begin() + m);
+        *n_done = m;
+        return HSPS_OK;
+    });
+}
+
+int hsps_session_result(hsps_session S, int slot, double* h_x, double* d_x, int* iterations, int* converged,
+                        double* true_res) {
+    return guarded([&] {
+        if (!S) throw Error(HSPS_ERR_INVALID, "hsps_session_result: null session");
+        S->B.ctx->set_current();
+        const BState st = session_take(*S, slot, h_x, d_x);
+        if (iterations) *iterations = st.iter;
+        if (converged) *converged = st.conv;
+        if (true_res) *true_res = st.true_rel;
+        return st.conv ? 1 : 0;
+    });
+}
+
+int hsps_session_live(hsps_session S) {
+    if (!S) return HSPS_ERR_INVALID;
+    int live = 0;
+    for (const auto& sl : S->slots) live += sl.used && sl.live;
+    return live;
 }
 
 }  // extern "C"

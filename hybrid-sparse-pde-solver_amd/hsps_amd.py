@@ -108,6 +108,12 @@ SIGNATURES = {
     "hsps_batch_destroy": (_i, [_vp]),
     "hsps_batch_solve": (_i, [_vp, _vp, _vp, _PP, _ip, _ip, _dp]),
     "hsps_batch_storage": (_i, [_vp, _ip, _ip, _ip]),
+    "hsps_session_create": (_i, [_vp, _i, C.c_longlong, C.c_longlong, _PP, C.POINTER(_vp)]),
+    "hsps_session_destroy": (_i, [_vp]),
+    "hsps_session_add": (_i, [_vp, _i, _ip, _ip, _dp, _dp, _vp, _ip]),
+    "hsps_session_step": (_i, [_vp, _ip, _i, _ip]),
+    "hsps_session_result": (_i, [_vp, _i, _dp, _vp, _ip, _ip, _dp]),
+    "hsps_session_live": (_i, [_vp]),
 }
 
 
@@ -439,6 +445,59 @@ class Batch:
         n = [C.c_int(), C.c_int(), C.c_int()]
         check(lib().hsps_batch_storage(self.h, *[C.byref(c) for c in n]))
         return {"csr": n[0].value, "dia": n[1].value, "stencil": n[2].value}
+
+
+class Session:
+    """Continuous batching (hsps_session_*): systems join a running batched
+    PCG + Jacobi at chunk boundaries and leave when they end."""
+
+    def __init__(self, ctx, max_systems, max_rows, max_nnz, params=None):
+        self.ctx = ctx
+        self.max_systems = max_systems
+        p = params or default_params()
+        h = _vp()
+        check(lib().hsps_session_create(ctx.h, max_systems, int(max_rows), int(max_nnz), C.byref(p), C.byref(h)))
+        self.h = h
+
+    def add(self, rp, ci, v, b, x0=None):
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        slot = C.c_int()
+        check(lib().hsps_session_add(self.h, len(rp) - 1, rp.ctypes.data_as(_ip), ci.ctypes.data_as(_ip),
+                                     v.ctypes.data_as(_dp), b.ctypes.data_as(_dp), x0.ptr if x0 is not None else None,
+                                     C.byref(slot)))
+        return slot.value
+
+    def step(self):
+        done = np.zeros(self.max_systems, np.int32)
+        nd = C.c_int()
+        check(lib().hsps_session_step(self.h, done.ctypes.data_as(_ip), self.max_systems, C.byref(nd)))
+        return done[:nd.value].tolist()
+
+    def result(self, slot, n, d_x=None):
+        """(x on the host, iterations, converged, true relative residual);
+        d_x (a DeviceArray of n doubles) also receives x on the device."""
+        x = np.zeros(n, np.float64)
+        it, cv, tr = C.c_int(), C.c_int(), C.c_double()
+        check(lib().hsps_session_result(self.h, slot, x.ctypes.data_as(_dp), d_x.ptr if d_x is not None else None,
+                                        C.byref(it), C.byref(cv), C.byref(tr)))
+        return x, it.value, bool(cv.value), tr.value
+
+    def live(self):
+        return check(lib().hsps_session_live(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hsps_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def solve(A, b, x, solver="cg", M=None, params=None):

@@ -278,6 +278,37 @@ int hsps_batch_solve(hsps_batch B, const double* d_b, double* d_x, const hsps_pa
  * row).  HSPS_BATCH_STORAGE=csr|dia in the environment caps the choice. */
 int hsps_batch_storage(hsps_batch B, int* n_csr, int* n_dia, int* n_stencil);
 
+/* ---- sessions (no reference counterpart): systems join a running batched
+ *      PCG + Jacobi and leave it at chunk boundaries — continuous batching.
+ *      A session holds device tables for at most max_systems systems of
+ *      max_rows rows and max_nnz nonzeros in total, all solved with the
+ *      params given at creation (tol, max_iter, precision, batch_tile_rows:
+ *      0 = the batch autotuner's choice on a probe batch, batch_graphs).  A
+ *      system's iterates and result are the same bits whichever systems share
+ *      the session and whenever it joins.  One thread at a time per session. -- */
+typedef struct hsps_session_s* hsps_session;
+int hsps_session_create(hsps_ctx ctx, int max_systems, long long max_rows, long long max_nnz, const hsps_params* p,
+                        hsps_session* out);
+int hsps_session_destroy(hsps_session S);
+/* Uploads an n x n system (host CSR, host b; d_x0 = device initial guess or
+ * NULL for zero) and starts it at the next step; *slot identifies it.
+ * HSPS_ERR_NOMEM when the session has no room for it now (retry after a
+ * result has been taken), HSPS_ERR_ZERO_DIAG as the Jacobi setup. */
+int hsps_session_add(hsps_session S, int n, const int* h_row_ptr, const int* h_col_idx, const double* h_values,
+                     const double* h_b, const double* d_x0, int* slot);
+/* Runs one chunk (p->check_every iterations) of every live system; the
+ * slots whose solves ended (reported once each, one chunk after the fact)
+ * go to done[0 .. *n_done), at most cap of them per call (the rest follow). */
+int hsps_session_step(hsps_session S, int* done, int cap, int* n_done);
+/* The result of a finished slot: x to h_x (host) and / or d_x (device, n
+ * doubles; either may be NULL), iterations, convergence (FP64 true residual
+ * below tol) and the FP64 true relative residual; the slot is then free.
+ * Returns 1 converged, 0 not. */
+int hsps_session_result(hsps_session S, int slot, double* h_x, double* d_x, int* iterations, int* converged,
+                        double* true_res);
+/* Systems joined and still iterating. */
+int hsps_session_live(hsps_session S);
+
 #ifdef __cplusplus
 }
 #endif
