@@ -37,6 +37,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 
@@ -1276,6 +1278,7 @@ struct Batch {
     int N = 0;  // rows
     long long nnz = 0;
     int n_large = 0;  // internal systems [0, n_large) are tiled; the rest are small
+    int capture_failures = 0;  // chunk captures that failed and ran as direct launches
     std::vector<int> h_row_off;  // user order, S+1
     std::vector<int> h_rp;       // global row_ptr, N+1
     std::vector<int> order;      // internal position -> user system id (by nnz, descending)
@@ -1366,7 +1369,7 @@ static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::
                     : 0;
     }
     out = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
-    HSPS_HIP(hipMemcpy(out.p, tv.data(), tv.size() * sizeof(TileVec<T>), hipMemcpyHostToDevice));
+    upload_on(B.ctx->stream, out.p, tv.data(), tv.size() * sizeof(TileVec<T>));
     (es == 4 ? B.tva32 : B.tva64) = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
     if constexpr (sizeof(T) == 4) B.h_tv32 = std::move(tv);
     else B.h_tv64 = std::move(tv);
@@ -1401,7 +1404,7 @@ static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     B.T = (int)r0.size();
     auto up = [&](DevBuf& d, const std::vector<int>& h) {
         d = DevBuf(B.ctx, std::max<size_t>(h.size(), 1) * sizeof(int));
-        if (!h.empty()) HSPS_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+        upload_on(B.ctx->stream, d.p, h.data(), h.size() * sizeof(int));
     };
     up(B.tile_r0, r0);
     up(B.tile_r1, r1);
@@ -1593,7 +1596,7 @@ static void build_dia(Batch& B, const int* row_off, const int* rp, const int* ci
     }
     auto up = [&](DevBuf& dst, const void* src, size_t bytes) {
         dst = DevBuf(B.ctx, std::max<size_t>(bytes, 8));
-        if (bytes) HSPS_HIP(hipMemcpy(dst.p, src, bytes, hipMemcpyHostToDevice));
+        upload_on(B.ctx->stream, dst.p, src, bytes);
     };
     B.h_kind = kind;
     B.h_nd = nd;
@@ -1775,17 +1778,28 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const Activ
             (void)hipGraphExecDestroy(B.graphs.front().exec);
             B.graphs.erase(B.graphs.begin());
         }
+        // A capture that fails (another thread of the process may invalidate
+        // it, e.g. by synchronising the device) records nothing that runs:
+        // the chunk is then launched directly, and the failure counted.
         hipGraph_t graph = nullptr;
         HSPS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        bool captured = true;
         try {
             direct();
-        } catch (...) {
-            (void)hipStreamEndCapture(s, &graph);
-            if (graph) (void)hipGraphDestroy(graph);
-            throw;
+        } catch (const Error&) {
+            captured = false;
         }
-        HSPS_HIP(hipStreamEndCapture(s, &graph));
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
         Batch::G ng;
+        hipError_t ei = hipErrorUnknown;
+        if (captured && ec == hipSuccess && graph) ei = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            ++B.capture_failures;
+            direct();
+            return;
+        }
         ng.wave = w;
         ng.prec = prec;
         ng.chunk = chunk;
@@ -1793,9 +1807,6 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const Activ
         ng.ns = act.ns;
         ng.kinds = act.kinds;
         ng.args = a;
-        const hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HSPS_HIP(e);
         B.graphs.push_back(ng);
         g = &B.graphs.back();
     }
@@ -2001,17 +2012,17 @@ static void fill_batch(Batch& B, Ctx* ctx, int S, const int* row_off, const int*
     B.v32 = DevBuf(ctx, sizeof(float) * (size_t)std::max<long long>(nnz, 1));
     B.dinv64 = DevBuf(ctx, sizeof(double) * (size_t)std::max(N, 1));
     B.dinv32 = DevBuf(ctx, sizeof(float) * (size_t)std::max(N, 1));
-    HSPS_HIP(hipMemcpy(B.rp.p, rp, sizeof(int) * (size_t)(N + 1), hipMemcpyHostToDevice));
+    upload_on(ctx->stream, B.rp.p, rp, sizeof(int) * (size_t)(N + 1));
     if (nnz) {
-        HSPS_HIP(hipMemcpy(B.ci.p, ci, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
-        HSPS_HIP(hipMemcpy(B.v64.p, v, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
+        upload_on(ctx->stream, B.ci.p, ci, sizeof(int) * (size_t)nnz);
+        upload_on(ctx->stream, B.v64.p, v, sizeof(double) * (size_t)nnz);
         std::vector<float> f(v, v + nnz);
-        HSPS_HIP(hipMemcpy(B.v32.p, f.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice));
+        upload_on(ctx->stream, B.v32.p, f.data(), sizeof(float) * (size_t)nnz);
     }
     if (N) {
-        HSPS_HIP(hipMemcpy(B.dinv64.p, d64.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice));
+        upload_on(ctx->stream, B.dinv64.p, d64.data(), sizeof(double) * (size_t)N);
         std::vector<float> f(d64.begin(), d64.end());
-        HSPS_HIP(hipMemcpy(B.dinv32.p, f.data(), sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+        upload_on(ctx->stream, B.dinv32.p, f.data(), sizeof(float) * (size_t)N);
     }
     B.order.resize(S);
     std::iota(B.order.begin(), B.order.end(), 0);
@@ -2029,8 +2040,8 @@ static void fill_batch(Batch& B, Ctx* ctx, int S, const int* row_off, const int*
     B.sys_row0 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
     B.sys_row1 = DevBuf(ctx, sizeof(int) * (size_t)std::max(S, 1));
     if (S) {
-        HSPS_HIP(hipMemcpy(B.sys_row0.p, r0.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-        HSPS_HIP(hipMemcpy(B.sys_row1.p, r1.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+        upload_on(ctx->stream, B.sys_row0.p, r0.data(), sizeof(int) * S);
+        upload_on(ctx->stream, B.sys_row1.p, r1.data(), sizeof(int) * S);
     }
     build_dia(B, row_off, rp, ci, v, d64);
     build_tiles(B, 256, 1024);
@@ -2122,6 +2133,11 @@ public:
             }
         return -1;
     }
+    bool fits(long long len) const {
+        for (const auto& f : free_)
+            if (f.second >= len) return true;
+        return len <= 0;
+    }
     void give(long long s, long long len) {
         if (len <= 0) return;
         size_t i = 0;
@@ -2145,6 +2161,7 @@ struct SessSlot {
     int used = 0;      // holds a system (joined, not yet released)
     int live = 0;      // still iterating as far as the host knows
     int done = 0;      // its halt has been reported
+    int conv = 0;      // converged, as read back at the halt
     long long join_seq = 0;  // the first state read-back that includes its initialisation
     int row0 = 0, n = 0, row_len = 0;  // rows [row0, row0 + row_len) held, n of them the system's
     long long nz0 = 0, nnz = 0, dia0 = 0, dia_len = 0;
@@ -2177,9 +2194,10 @@ struct Session {
     }
 };
 
-// Tile rows of diagonal-stored systems for a session: the batch autotuner's
+// Tile rows of diagonal-stored systems for sessions: the batch autotuner's
 // choice on a probe batch of five-point systems (8 of 300^2 and 4 of 600^2
-// rows, Poisson), made once when the session is created.
+// rows, Poisson), made once per device and precision in a process, so every
+// session of the process tiles a system the same way (same bits).
 static int probe_tile_rows_dia(Ctx* ctx, const hsps_params& p) {
     std::vector<int> sides = {300, 300, 300, 300, 300, 300, 300, 300, 600, 600, 600, 600};
     std::vector<int> row_off(1, 0), rp(1, 0), ci;
@@ -2208,11 +2226,23 @@ static int probe_tile_rows_dia(Ctx* ctx, const hsps_params& p) {
     ensure_work(B, p.precision == HSPS_MIXED ? HSPS_MIXED : HSPS_FP64);
     DevBuf bd(ctx, sizeof(double) * (size_t)N), xd(ctx, sizeof(double) * (size_t)N);
     std::vector<double> ones((size_t)N, 1.0);
-    HSPS_HIP(hipMemcpy(bd.p, ones.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice));
-    HSPS_HIP(hipMemset(xd.p, 0, sizeof(double) * (size_t)N));
+    upload_on(ctx->stream, bd.p, ones.data(), sizeof(double) * (size_t)N);
+    fill_on(ctx->stream, xd.p, 0, sizeof(double) * (size_t)N);
     autotune(B, bd.as<double>(), xd.as<double>(), p);
     HSPS_HIP(hipStreamSynchronize(ctx->stream));
     return B.tile_rows_dia;
+}
+
+static int session_tile_rows_dia(Ctx* ctx, const hsps_params& p) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, int> chosen;  // (device, precision) -> tile rows
+    const std::pair<int, int> key = {ctx->device, p.precision == HSPS_MIXED ? HSPS_MIXED : HSPS_FP64};
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = chosen.find(key);
+    if (it != chosen.end()) return it->second;
+    const int t = probe_tile_rows_dia(ctx, p);
+    chosen[key] = t;
+    return t;
 }
 
 static void session_alloc(Session& S, Ctx* ctx, int max_systems, long long max_rows, long long max_nnz,
@@ -2224,7 +2254,7 @@ static void session_alloc(Session& S, Ctx* ctx, int max_systems, long long max_r
         S.tile_rows = S.tile_rows_dia = std::min(p.batch_tile_rows, 1 << 16);
     } else {
         S.tile_rows = 512;
-        S.tile_rows_dia = probe_tile_rows_dia(ctx, p);
+        S.tile_rows_dia = session_tile_rows_dia(ctx, p);
     }
     Batch& B = S.B;
     const long long N = max_rows + (long long)max_systems * kSessRowAlign;  // gap rows and alignment
@@ -2284,9 +2314,9 @@ static void session_alloc(Session& S, Ctx* ctx, int max_systems, long long max_r
     ensure_work(B, S.prec);
     S.b = DevBuf(ctx, sizeof(double) * n);
     S.x = DevBuf(ctx, sizeof(double) * n);
-    HSPS_HIP(hipMemset(S.b.p, 0, sizeof(double) * n));
-    HSPS_HIP(hipMemset(S.x.p, 0, sizeof(double) * n));
-    HSPS_HIP(hipMemset(B.umask.p, 0, n));
+    fill_on(ctx->stream, S.b.p, 0, sizeof(double) * n);
+    fill_on(ctx->stream, S.x.p, 0, sizeof(double) * n);
+    fill_on(ctx->stream, B.umask.p, 0, n);
     HSPS_HIP(hipHostMalloc((void**)&S.pin, sizeof(BState) * Sn * 2, hipHostMallocDefault));
     HSPS_HIP(hipEventCreateWithFlags(&S.ev[0], hipEventDisableTiming));
     HSPS_HIP(hipEventCreateWithFlags(&S.ev[1], hipEventDisableTiming));
@@ -2339,6 +2369,10 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
     if (n < 1 || !rp || !ci || !v || !h_b) throw Error(HSPS_ERR_INVALID, "hsps_session_add: bad argument");
     const long long nnz = rp[n];
     if (rp[0] != 0 || nnz < 1) throw Error(HSPS_ERR_INVALID, "hsps_session_add: bad row_ptr");
+    // every range is a multiple of kSessRowAlign rows long, so every start is aligned
+    const long long row_len = ((long long)n + kSessRowAlign) / kSessRowAlign * kSessRowAlign;
+    // no room: said before the O(nnz) checks and analysis (a caller retries as systems end)
+    if (S.free_slots.empty() || !S.rows.fits(row_len) || !S.nzs.fits(nnz)) return -1;
     std::vector<double> dinv(n);
     for (int i = 0; i < n; ++i) {
         if (rp[i + 1] < rp[i]) throw Error(HSPS_ERR_INVALID, "hsps_session_add: row_ptr not monotone");
@@ -2368,9 +2402,6 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
         i = j;
     }
     const int nt = (int)tr0.size();
-    if (S.free_slots.empty()) return -1;
-    // every row range is a multiple of kSessRowAlign long, so every start is aligned
-    const long long row_len = ((long long)n + kSessRowAlign) / kSessRowAlign * kSessRowAlign;
     const long long row0 = S.rows.take(row_len);
     if (row0 < 0) return -1;
     const long long nz0 = S.nzs.take(nnz);
@@ -2385,14 +2416,15 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
     }
     const int slot = S.free_slots.back();
     S.free_slots.pop_back();
-    // the system's rows, nonzeros and tables (synchronous copies: the chunk in
-    // flight reads none of these ranges)
+    // the system's rows, nonzeros and tables, copied on the side stream while
+    // a chunk may run (it reads none of these ranges); all sources stay alive
+    // until the one wait below
     std::vector<int> grp(n + 1), gci(nnz);
     for (int i = 0; i <= n; ++i) grp[i] = (int)(nz0 + rp[i]);
     for (long long k = 0; k < nnz; ++k) gci[k] = (int)(row0 + ci[k]);
     std::vector<float> v32(v, v + nnz), d32(dinv.begin(), dinv.end());
     auto put = [&](const DevBuf& dst, size_t off_bytes, const void* src, size_t bytes) {
-        if (bytes) HSPS_HIP(hipMemcpy((char*)dst.p + off_bytes, src, bytes, hipMemcpyHostToDevice));
+        if (bytes) HSPS_HIP(hipMemcpyAsync((char*)dst.p + off_bytes, src, bytes, hipMemcpyHostToDevice, B.aux));
     };
     put(B.rp, sizeof(int) * row0, grp.data(), sizeof(int) * (n + 1));
     put(B.ci, sizeof(int) * nz0, gci.data(), sizeof(int) * nnz);
@@ -2440,6 +2472,7 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
     put(B.tv32, sizeof(TileVec<float>) * t0, r32.data(), sizeof(TileVec<float>) * nt);
     put(B.tv64, sizeof(TileVec<double>) * t0, r64.data(), sizeof(TileVec<double>) * nt);
     put(S.b, sizeof(double) * row0, h_b, sizeof(double) * n);
+    HSPS_HIP(hipStreamSynchronize(B.aux));  // the copies are done before the initialisation is enqueued
     hipStream_t s = B.ctx->stream;
     double* xr = S.x.as<double>() + row0;
     if (d_x0) HSPS_HIP(hipMemcpyAsync(xr, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -2535,7 +2568,9 @@ static std::vector<int> session_step(Session& S) {
             S.keys.push_back({key, 1});
             uses = 1;
         }
-        const bool graph = S.p.batch_graphs != 0 && uses >= 2;  // capture a shape seen before
+        // capture a shape seen before; after repeated failed captures (other
+        // threads of the process interfering) the session launches directly
+        const bool graph = S.p.batch_graphs != 0 && uses >= 2 && B.capture_failures < 4;
         launch_chunk(B, a, 0, chunk, S.act, S.b.as<double>(), S.x.as<double>(), S.prec, s, graph);
     }
     BState* slotp = S.pin + (size_t)(S.seq & 1) * Sn;
@@ -2553,6 +2588,7 @@ static std::vector<int> session_step(Session& S) {
             if (prev[k].halt) {
                 sl.live = 0;
                 sl.done = 1;
+                sl.conv = prev[k].conv;
                 out.push_back(k);
                 S.dirty = true;
             }
@@ -2728,13 +2764,15 @@ int hsps_session_add(hsps_session S, int n, const int* rp, const int* ci, const 
     });
 }
 
-int hsps_session_step(hsps_session S, int* done, int cap, int* n_done) {
+int hsps_session_step(hsps_session S, int* done, int* converged, int cap, int* n_done) {
     return guarded([&] {
         if (!S || !n_done || (cap > 0 && !done)) throw Error(HSPS_ERR_INVALID, "hsps_session_step: null argument");
         S->B.ctx->set_current();
         for (int k : session_step(*S)) S->finished.push_back(k);
         const int m = std::min<int>(std::max(cap, 0), (int)S->finished.size());
         std::copy(S->finished.begin(), S->finished.begin() + m, done);
+        if (converged)
+            for (int i = 0; i < m; ++i) converged[i] = S->slots[done[i]].conv;
         S->finished.erase(S->finished.begin(), S->finished.begin() + m);
         *n_done = m;
         return HSPS_OK;

@@ -94,13 +94,13 @@ void Csr::build_schedule() {
     std::vector<int> b = make_row_blocks(h_rp.data(), rows, kMaxBlockRows, kLdsNnz);
     n_blocks = (int)b.size() - 1;
     blocks = DevBuf(ctx, b.size() * sizeof(int));
-    HSPS_HIP(hipMemcpy(blocks.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+    upload_on(ctx->stream, blocks.p, b.data(), b.size() * sizeof(int));
     std::vector<int> b32 = make_row_blocks(h_rp.data(), rows, kMaxBlockRows32, kLdsNnz32);
     n_blocks32 = (int)b32.size() - 1;
     // partial-sum buffers are sized by n_blocks for both precisions
     if (n_blocks32 > n_blocks) throw Error(HSPS_ERR_HIP, "csr: FP32 schedule has more blocks than FP64");
     blocks32 = DevBuf(ctx, b32.size() * sizeof(int));
-    HSPS_HIP(hipMemcpy(blocks32.p, b32.data(), b32.size() * sizeof(int), hipMemcpyHostToDevice));
+    upload_on(ctx->stream, blocks32.p, b32.data(), b32.size() * sizeof(int));
 }
 
 __global__ void k_f64_to_f32(int n, const double* __restrict__ x, float* __restrict__ y) {
@@ -200,8 +200,8 @@ std::unique_ptr<Csr> make_csr(Ctx* ctx, int rows, int cols, std::vector<int> rp,
     if (want_f32) A->v32 = DevBuf(ctx, (size_t)A->nnz * sizeof(float));
     A->diag64 = DevBuf(ctx, (size_t)rows * sizeof(double));
     ctx->set_current();
-    HSPS_HIP(hipMemcpy(A->rp.p, A->h_rp.data(), (size_t)(rows + 1) * sizeof(int), hipMemcpyHostToDevice));
-    if (A->nnz) HSPS_HIP(hipMemcpy(A->ci.p, A->h_ci.data(), (size_t)A->nnz * sizeof(int), hipMemcpyHostToDevice));
+    upload_on(ctx->stream, A->rp.p, A->h_rp.data(), (size_t)(rows + 1) * sizeof(int));
+    upload_on(ctx->stream, A->ci.p, A->h_ci.data(), (size_t)A->nnz * sizeof(int));
     A->upload_values();
     A->build_schedule();
     return A;

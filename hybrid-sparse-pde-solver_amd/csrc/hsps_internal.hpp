@@ -89,6 +89,21 @@ struct Ctx {
 void ctx_retain(Ctx* c);
 void ctx_release(Ctx* c);
 
+// Host -> device copies and fills ordered on a stream of ours, returning once
+// done.  Never through the legacy null stream: it orders against every
+// blocking stream of the process, and HIP refuses it while any other thread
+// captures a graph.  Pageable sources must stay alive until the call returns.
+inline void upload_on(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    HSPS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+}
+inline void fill_on(hipStream_t s, void* dst, int value, size_t bytes) {
+    if (!bytes) return;
+    HSPS_HIP(hipMemsetAsync(dst, value, bytes, s));
+    HSPS_HIP(hipStreamSynchronize(s));
+}
+
 // SpMV schedule constants (gfx950: 64-wide waves, 160 KB LDS per CU).
 constexpr int kBlock = 256;       // threads per workgroup (4 waves)
 constexpr int kLdsNnz = 2048;     // FP64 products staged in LDS per row block (16 KB)

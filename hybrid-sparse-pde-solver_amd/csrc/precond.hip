@@ -337,7 +337,7 @@ __global__ void k_tri_gather(int n, const int* __restrict__ order, const T* __re
 // ---- helpers ---------------------------------------------------------------------
 static DevBuf upload(Ctx* c, const void* h, size_t bytes) {
     DevBuf b(c, bytes);
-    if (bytes) HSPS_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+    upload_on(c->stream, b.p, h, bytes);
     return b;
 }
 static DevBuf upload_f32(Ctx* c, const std::vector<double>& v) {

@@ -61,6 +61,10 @@ public:
     // ---- device residency --------------------------------------------------------
     const double* device_data() const;  ///< uploads if the host copy is newer
     double* device_data_mut();          ///< device copy becomes the current one
+    /// The device copy becomes size() doubles copied from d_src (device
+    /// memory on this device, already written), enqueued on this vector's
+    /// context stream; the host copy becomes stale.
+    void assign_device(const double* d_src);
     bool device_current() const { return dev_valid_; }
     bool host_current() const { return host_valid_; }
 

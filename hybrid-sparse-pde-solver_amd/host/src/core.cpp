@@ -101,6 +101,12 @@ double* Vector::device_data_mut() {
     host_valid_ = false;
     return static_cast<double*>(dev_.get());
 }
+void Vector::assign_device(const double* d_src) {
+    ensure_dev();
+    if (!data_.empty()) dev_check(hsps_memcpy_d2d(dev_.ctx(), dev_.get(), d_src, data_.size() * sizeof(Real)));
+    dev_valid_ = true;
+    host_valid_ = false;
+}
 
 void Vector::resize(Index n, Real val) {
     data_.assign(static_cast<Size>(n), val);

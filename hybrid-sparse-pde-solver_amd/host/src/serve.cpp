@@ -25,22 +25,59 @@ double SolveServer::estimate_bytes(Index n, long long nnz, const SolverParams& p
     return matrix + krylov + precond + 64.0 * n;          // + b, x, r and scratch vectors
 }
 
+namespace {
+
+double monotonic_s(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(t.time_since_epoch()).count();
+}
+
+// Requests share a session when the parameters of its batched PCG agree.
+bool same_session_params(const SolverParams& a, const SolverParams& b) {
+    return a.tol == b.tol && a.max_iter == b.max_iter && a.precision == b.precision &&
+           a.reliable_delta == b.reliable_delta && a.check_every == b.check_every;
+}
+
+hsps_params session_params(const SolverParams& p) {
+    hsps_params hp;
+    hsps_params_default(&hp);
+    hp.tol = p.tol;
+    hp.max_iter = p.max_iter;
+    hp.precision = p.precision == Precision::MIXED ? HSPS_MIXED : HSPS_FP64;
+    hp.reliable_delta = p.reliable_delta;
+    hp.check_every = p.check_every;
+    hp.batch_tile_rows = 0;  // the autotuned tile rows (chosen once per process and device)
+    hp.batch_graphs = 1;     // the session thread alone captures, on its own stream
+    return hp;
+}
+
+constexpr int kMaxSessions = 2;  // distinct parameter sets served by sessions at once
+
+}  // namespace
+
 SolveServer::SolveServer(const ServeOptions& opt) : opt_(opt) {
     if (opt_.workers < 1) throw std::invalid_argument("SolveServer: workers must be >= 1");
     if (opt_.batch_max < 1) opt_.batch_max = 1;
+    const long long want = opt_.batch_rows_total > 0
+                               ? opt_.batch_rows_total
+                               : std::min<long long>((long long)opt_.batch_max * opt_.batch_rows_max, 1LL << 24);
+    sess_rows_ = std::max<long long>(want, (long long)opt_.batch_rows_max + 64);
+    sess_nnz_ = std::min<long long>(8 * sess_rows_, (1LL << 31) - 1);
     try {
-        for (int w = 0; w < opt_.workers; ++w) {
+        for (int w = 0; w <= opt_.workers; ++w) {  // the last context is the batch thread's
             hsps_ctx c = nullptr;
             dev_check(hsps_ctx_create(opt_.device, &c));
             ctxs_.push_back(c);
         }
         for (int w = 0; w < opt_.workers; ++w) threads_.emplace_back([this, w] { worker(w); });
+        threads_.emplace_back([this] { batch_loop(); });
     } catch (...) {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
+            batch_done_ = true;
         }
         cv_work_.notify_all();
+        cv_batch_.notify_all();
         for (auto& t : threads_) t.join();
         for (auto c : ctxs_) hsps_ctx_destroy(c);
         throw;
@@ -53,7 +90,9 @@ SolveServer::~SolveServer() {
         stop_ = true;
     }
     cv_work_.notify_all();
+    cv_batch_.notify_all();
     for (auto& t : threads_) t.join();
+    for (auto& q : x0_pool_) hsps_free(ctxs_.back(), q.first);
     for (auto c : ctxs_) hsps_ctx_destroy(c);
 }
 
@@ -65,8 +104,9 @@ SolveServer::Ticket SolveServer::submit(SparseMatrix A, std::vector<Real> b, con
     r->A = std::move(A);
     r->b = std::move(b);
     r->p = params;
-    r->p.device_graphs = false;  // workers share the device: no stream capture
+    r->p.device_graphs = false;  // ladder solves share the device: no stream capture
     r->t_submit = Clock::now();
+    r->res.submit_s = monotonic_s(r->t_submit);
     Ticket id;
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -97,40 +137,17 @@ ServeResult SolveServer::wait(Ticket t, std::vector<Real>* x) {
     return r->res;
 }
 
-bool SolveServer::batch_due(Clock::time_point now) const {
-    if (pool_.empty()) return false;
-    if ((int)pool_.size() >= opt_.batch_max || stop_) return true;
-    return now >= pool_.front()->t_pool + std::chrono::duration_cast<Clock::duration>(
-                                                  std::chrono::duration<double>(opt_.batch_window_s));
-}
-
-// Up to batch_max pooled requests with the oldest one's tolerance, iteration
-// cap and precision (one batched launch shares them), in pool order.
-std::vector<SolveServer::Req> SolveServer::take_batch() {
-    std::vector<Req> out;
-    const SolverParams& k = pool_.front()->p;
-    for (auto it = pool_.begin(); it != pool_.end() && (int)out.size() < opt_.batch_max;) {
-        const SolverParams& q = (*it)->p;
-        if (q.tol == k.tol && q.max_iter == k.max_iter && q.precision == k.precision &&
-            q.reliable_delta == k.reliable_delta && q.check_every == k.check_every) {
-            out.push_back(*it);
-            it = pool_.erase(it);
-        } else {
-            ++it;
-        }
-    }
-    return out;
-}
-
+// Workers: ladder work first (escalations from the sessions, requests the
+// sessions cannot take), then admission in arrival order within the budget.
 void SolveServer::worker(int w) {
     ThreadDevice bind(ctxs_[w]);
     std::unique_lock<std::mutex> lk(mu_);
     for (;;) {
-        const auto now = Clock::now();
-        if (batch_due(now)) {
-            auto batch = take_batch();
+        if (!handoffs_.empty()) {
+            Handoff h = std::move(handoffs_.front());
+            handoffs_.pop_front();
             lk.unlock();
-            run_batch(w, std::move(batch));
+            run_ladder(h);
             lk.lock();
             continue;
         }
@@ -140,24 +157,29 @@ void SolveServer::worker(int w) {
                 Req r = h;
                 pending_.pop_front();
                 in_flight_ += r->bytes;
-                r->t_admit = now;
+                r->t_admit = Clock::now();
+                ++starting_;
                 lk.unlock();
                 start(r);
                 lk.lock();
+                --starting_;
+                cv_batch_.notify_one();
                 continue;
             }
         }
-        if (stop_ && pending_.empty() && pool_.empty()) break;
-        if (!pool_.empty())
-            cv_work_.wait_until(lk, pool_.front()->t_pool + std::chrono::duration_cast<Clock::duration>(
-                                                                 std::chrono::duration<double>(opt_.batch_window_s)));
-        else
-            cv_work_.wait(lk);
+        if (stop_ && pending_.empty() && handoffs_.empty() && batch_done_) break;
+        cv_work_.wait(lk);
     }
 }
 
+bool SolveServer::joins_session(const Req& r) const {
+    const long long n = r->A.rows(), nnz = r->A.nnz();
+    return opt_.batch_max > 1 && n >= 1 && n <= opt_.batch_rows_max && n + 64 <= sess_rows_ && nnz >= 1 &&
+           nnz <= sess_nnz_;
+}
+
 // Classify (adaptive_selector.cpp:225-233 on the features of :38-220); EASY
-// requests small enough go to the batch pool, the others start the ladder.
+// requests small enough join a session, the others start the ladder.
 void SolveServer::start(const Req& r) {
     AdaptiveState state;
     try {
@@ -169,92 +191,204 @@ void SolveServer::start(const Req& r) {
         complete(r);
         return;
     }
-    if (state == AdaptiveState::EASY && opt_.batch_max > 1 && r->A.rows() <= opt_.batch_rows_max) {
-        r->A.release_device();
+    if (state == AdaptiveState::EASY && joins_session(r)) {
+        r->A.release_device();  // the session holds its own copy
         {
             std::lock_guard<std::mutex> lk(mu_);
-            r->t_pool = Clock::now();
             pool_.push_back(r);
         }
+        cv_batch_.notify_one();
+        return;
+    }
+    Handoff h;
+    h.r = r;
+    h.state = state;
+    run_ladder(h);
+}
+
+// The batch thread: joins pooled requests to the sessions (first fit, in
+// arrival order) and steps every session that holds systems, one chunk each.
+void SolveServer::batch_loop() {
+    ThreadDevice bind(ctxs_.back());
+    std::vector<Sess> sessions;
+    std::deque<Req> waiting;  // taken from the pool, no room yet
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        while (!pool_.empty()) {
+            waiting.push_back(std::move(pool_.front()));
+            pool_.pop_front();
+        }
+        bool live = false;
+        for (const Sess& S : sessions) live = live || !S.slots.empty();
+        if (waiting.empty() && !live) {
+            if (stop_ && pending_.empty() && starting_ == 0) break;
+            cv_batch_.wait(lk);
+            continue;
+        }
+        lk.unlock();
+        for (auto it = waiting.begin(); it != waiting.end();) it = join(sessions, *it) ? waiting.erase(it) : it + 1;
+        live = false;
+        for (const Sess& S : sessions) live = live || !S.slots.empty();
+        if (!live) {  // nothing running and still no room: those requests run alone
+            for (Req& r : waiting) {
+                Handoff h;
+                h.r = r;
+                std::lock_guard<std::mutex> g(mu_);
+                handoffs_.push_back(std::move(h));
+            }
+            waiting.clear();
+            cv_work_.notify_all();
+        }
+        for (Sess& S : sessions)
+            if (!S.slots.empty()) step(S);
+        lk.lock();
+    }
+    lk.unlock();
+    for (Sess& S : sessions) hsps_session_destroy(S.h);
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        batch_done_ = true;
+    }
+    cv_work_.notify_all();
+}
+
+bool SolveServer::join(std::vector<Sess>& sessions, const Req& r) {
+    auto to_ladder = [&](const char* why) {
+        if (why) HSPS_LOG_WARN("SolveServer: request ", r->id, " runs alone: ", why);
+        Handoff h;
+        h.r = r;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            handoffs_.push_back(std::move(h));
+        }
+        cv_work_.notify_one();
+        return true;
+    };
+    Sess* S = nullptr;
+    for (Sess& s : sessions)
+        if (same_session_params(s.p, r->p)) S = &s;
+    if (!S) {
+        if ((int)sessions.size() >= kMaxSessions) {
+            auto idle = std::find_if(sessions.begin(), sessions.end(), [](const Sess& s) { return s.slots.empty(); });
+            if (idle == sessions.end()) return to_ladder(nullptr);  // every session busy with other parameters
+            hsps_session_destroy(idle->h);
+            sessions.erase(idle);
+        }
+        const hsps_params hp = session_params(r->p);
+        hsps_session h = nullptr;
+        if (hsps_session_create(ctxs_.back(), opt_.batch_max, sess_rows_, sess_nnz_, &hp, &h) < 0)
+            return to_ladder(hsps_last_error());
+        sessions.push_back(Sess{h, r->p, {}});
+        S = &sessions.back();
+    }
+    const SparseMatrix& A = r->A;
+    int slot = -1;
+    const int rc = hsps_session_add(S->h, A.rows(), A.row_ptr().data(), A.col_idx().data(), A.values().data(),
+                                    r->b.data(), nullptr, &slot);
+    if (rc == HSPS_OK) {
+        S->slots[slot] = r;
+        return true;
+    }
+    if (rc == HSPS_ERR_NOMEM && !S->slots.empty()) return false;  // room frees as systems end
+    // an error the ladder reports the reference's way (e.g. a zero diagonal)
+    return to_ladder(rc == HSPS_ERR_NOMEM ? hsps_last_error() : nullptr);
+}
+
+// One chunk of a session; the systems that ended complete (converged) or
+// continue at MODERATE on a worker from their x, handed over in device memory.
+void SolveServer::step(Sess& S) {
+    const int cap = opt_.batch_max;
+    std::vector<int> done(cap), conv(cap);
+    int nd = 0;
+    if (hsps_session_step(S.h, done.data(), conv.data(), cap, &nd) < 0) {
+        HSPS_LOG_WARN("SolveServer: session step failed: ", hsps_last_error());
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto& kv : S.slots) {  // every system of the session runs alone
+            Handoff h;
+            h.r = kv.second;
+            handoffs_.push_back(std::move(h));
+        }
+        S.slots.clear();
         cv_work_.notify_all();
         return;
     }
-    run_ladder(r, state, {}, 0, false);
-}
-
-void SolveServer::run_batch(int w, std::vector<Req> batch) {
-    const int S = (int)batch.size();
-    std::vector<int> row_off(S + 1, 0);
-    for (int k = 0; k < S; ++k) row_off[k + 1] = row_off[k] + batch[k]->A.rows();
-    const int N = row_off[S];
-    std::vector<int> rp(1, 0), ci;
-    std::vector<double> v, b;
-    for (int k = 0; k < S; ++k) {
-        const SparseMatrix& A = batch[k]->A;
-        const int base = rp.back();
-        for (Index i = 0; i < A.rows(); ++i) rp.push_back(base + A.row_ptr()[i + 1]);
-        for (Index c : A.col_idx()) ci.push_back(c + row_off[k]);
-        v.insert(v.end(), A.values().begin(), A.values().end());
-        b.insert(b.end(), batch[k]->b.begin(), batch[k]->b.end());
-    }
-    std::vector<int> it(S, 0), cv(S, 0);
-    std::vector<double> tr(S, 0.0), x(N, 0.0);
-    hsps_ctx c = ctxs_[w];
-    hsps_batch B = nullptr;
-    void *db = nullptr, *dx = nullptr;
-    bool ok = true;
-    try {
-        dev_check(hsps_batch_create(c, S, row_off.data(), rp.data(), ci.data(), v.data(), &B));
-        dev_check(hsps_malloc(c, sizeof(double) * std::max(N, 1), &db));
-        dev_check(hsps_malloc(c, sizeof(double) * std::max(N, 1), &dx));
-        dev_check(hsps_memcpy_h2d(c, db, b.data(), sizeof(double) * N));
-        dev_check(hsps_memset(c, dx, 0, sizeof(double) * N));
-        const SolverParams& p = batch[0]->p;
-        hsps_params hp;
-        hsps_params_default(&hp);
-        hp.tol = p.tol;
-        hp.max_iter = p.max_iter;
-        hp.precision = p.precision == Precision::MIXED ? HSPS_MIXED : HSPS_FP64;
-        hp.reliable_delta = p.reliable_delta;
-        hp.check_every = p.check_every;
-        hp.batch_tile_rows = 2048;  // a one-shot batch: no tile autotuning
-        hp.batch_graphs = 0;        // other workers use the device concurrently: no stream capture
-        dev_check(hsps_batch_solve(B, (const double*)db, (double*)dx, &hp, it.data(), cv.data(), tr.data()));
-        dev_check(hsps_memcpy_d2h(c, x.data(), dx, sizeof(double) * N));
-    } catch (const std::exception& e) {
-        HSPS_LOG_WARN("SolveServer: batched launch of ", S, " requests failed: ", e.what());
-        ok = false;
-    }
-    if (db) hsps_free(c, db);
-    if (dx) hsps_free(c, dx);
-    if (B) hsps_batch_destroy(B);
-    for (int k = 0; k < S; ++k) {
-        const Req& r = batch[k];
-        if (!ok) {  // the batch could not run: each request takes the ladder alone
-            run_ladder(r, AdaptiveState::EASY, {}, 0, false);
-            continue;
-        }
-        std::vector<Real> xk(x.begin() + row_off[k], x.begin() + row_off[k + 1]);
-        if (cv[k]) {
-            r->res.converged = true;
-            r->res.iterations = it[k];
+    for (int i = 0; i < nd; ++i) {
+        auto it = S.slots.find(done[i]);
+        if (it == S.slots.end()) continue;
+        Req r = it->second;
+        S.slots.erase(it);
+        const Index n = r->A.rows();
+        int iters = 0, cv = 0;
+        double tr = 0.0;
+        if (conv[i]) {
+            r->x.assign((size_t)n, 0.0);
+            if (hsps_session_result(S.h, done[i], r->x.data(), nullptr, &iters, &cv, &tr) < 0) {
+                HSPS_LOG_WARN("SolveServer: request ", r->id, " result failed: ", hsps_last_error());
+                cv = 0;
+            }
+            r->res.converged = cv != 0;
+            r->res.iterations = iters;
             r->res.state = AdaptiveState::EASY;
             r->res.batched = true;
-            r->res.true_residual = tr[k];
-            r->x = std::move(xk);
+            r->res.true_residual = tr;
             complete(r);
-        } else {  // escalate as the ladder does, warm-started from the batch's x
-            run_ladder(r, AdaptiveState::MODERATE, std::move(xk), 1, true);
+            continue;
         }
+        // escalate as the ladder does (adaptive_selector.cpp:350-380), warm
+        // started from the session's x without a round trip through the host
+        Handoff h;
+        h.r = r;
+        h.state = AdaptiveState::MODERATE;
+        h.escalations = 1;
+        h.batched = true;
+        const size_t bytes = sizeof(double) * (size_t)n;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto best = x0_pool_.end();  // the smallest idle buffer that holds x
+            for (auto q = x0_pool_.begin(); q != x0_pool_.end(); ++q)
+                if (q->second >= bytes && (best == x0_pool_.end() || q->second < best->second)) best = q;
+            if (best != x0_pool_.end()) {
+                h.x0 = best->first;
+                h.x0_bytes = best->second;
+                x0_pool_.erase(best);
+            }
+        }
+        if (!h.x0) {
+            if (hsps_malloc(ctxs_.back(), bytes, &h.x0) < 0) h.x0 = nullptr;
+            h.x0_bytes = h.x0 ? bytes : 0;
+        }
+        if (hsps_session_result(S.h, done[i], h.x0 ? nullptr : (r->x.assign((size_t)n, 0.0), r->x.data()),
+                                (double*)h.x0, &iters, &cv, &tr) < 0)
+            HSPS_LOG_WARN("SolveServer: request ", r->id, " result failed: ", hsps_last_error());
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            handoffs_.push_back(std::move(h));
+        }
+        cv_work_.notify_one();
     }
 }
 
-void SolveServer::run_ladder(const Req& r, AdaptiveState state, std::vector<Real> x0, int escalations,
-                             bool batched) {
+void SolveServer::run_ladder(const Handoff& h) {
+    const Req& r = h.r;
+    AdaptiveState state = h.state;
+    int escalations = h.escalations;
+    void* x0 = h.x0;  // device x0 from a session, returned to the pool once copied
+    auto give_back = [&] {
+        std::lock_guard<std::mutex> g(mu_);
+        x0_pool_.push_back({x0, h.x0_bytes});
+        x0 = nullptr;
+    };
     try {
         const Index n = r->A.rows();
         Vector b(r->b);
-        Vector x = x0.empty() ? Vector(n, REAL_ZERO) : Vector(x0);
+        Vector x(n, REAL_ZERO);
+        if (x0) {  // copied on this worker's stream, which is drained before the buffer is reused
+            x.assign_device(static_cast<const double*>(x0));
+            dev_check(hsps_ctx_sync(DeviceContext::current()));
+            give_back();
+        } else if (!r->x.empty()) {
+            x = Vector(r->x);
+        }
         for (;;) {
             double setup = 0.0;
             auto pair = SolverFactory::make_for_state(state, r->A, r->p, &setup);
@@ -269,12 +403,13 @@ void SolveServer::run_ladder(const Req& r, AdaptiveState state, std::vector<Real
             ++escalations;
         }
         r->res.escalations = escalations;
-        r->res.batched = batched;
+        r->res.batched = h.batched;
         const Real* xd = x.data();
         r->x.assign(xd, xd + n);
     } catch (const std::exception& e) {
         HSPS_LOG_WARN("SolveServer: request ", r->id, " failed at state ", (int)state, ": ", e.what());
         r->res.converged = false;
+        if (x0 && hsps_ctx_sync(DeviceContext::current()) >= 0) give_back();  // else leave it: a copy may still read it
     }
     r->A.release_device();
     complete(r);
@@ -286,6 +421,7 @@ void SolveServer::complete(const Req& r) {
         const auto now = Clock::now();
         r->res.queue_s = std::chrono::duration<double>(r->t_admit - r->t_submit).count();
         r->res.latency_s = std::chrono::duration<double>(now - r->t_submit).count();
+        r->res.done_s = monotonic_s(now);
         in_flight_ -= r->bytes;
         r->done = true;
     }
@@ -323,7 +459,7 @@ extern "C" {
 const char* hsps_server_last_error(void) { return g_serve_error.c_str(); }
 
 int hsps_server_create(int device, int workers, double hbm_budget_bytes, int batch_max, int batch_rows_max,
-                       double batch_window_s, hsps_server* out) {
+                       long long batch_rows_total, hsps_server* out) {
     return serve_guarded([&] {
         if (!out) throw std::invalid_argument("hsps_server_create: null out");
         ServeOptions o;
@@ -332,7 +468,7 @@ int hsps_server_create(int device, int workers, double hbm_budget_bytes, int bat
         o.hbm_budget_bytes = hbm_budget_bytes;
         o.batch_max = batch_max;
         o.batch_rows_max = batch_rows_max;
-        o.batch_window_s = batch_window_s;
+        o.batch_rows_total = batch_rows_total;
         *out = new hsps_server_s(o);
         return HSPS_OK;
     });
@@ -403,6 +539,8 @@ int hsps_server_wait(hsps_server S, long long ticket, double* x, hsps_serve_resu
             res->true_residual = r.true_residual;
             res->queue_s = r.queue_s;
             res->latency_s = r.latency_s;
+            res->submit_s = r.submit_s;
+            res->done_s = r.done_s;
         }
         return r.converged ? 1 : 0;
     });

@@ -111,7 +111,7 @@ SIGNATURES = {
     "hsps_session_create": (_i, [_vp, _i, C.c_longlong, C.c_longlong, _PP, C.POINTER(_vp)]),
     "hsps_session_destroy": (_i, [_vp]),
     "hsps_session_add": (_i, [_vp, _i, _ip, _ip, _dp, _dp, _vp, _ip]),
-    "hsps_session_step": (_i, [_vp, _ip, _i, _ip]),
+    "hsps_session_step": (_i, [_vp, _ip, _ip, _i, _ip]),
     "hsps_session_result": (_i, [_vp, _i, _dp, _vp, _ip, _ip, _dp]),
     "hsps_session_live": (_i, [_vp]),
 }
@@ -473,7 +473,7 @@ class Session:
     def step(self):
         done = np.zeros(self.max_systems, np.int32)
         nd = C.c_int()
-        check(lib().hsps_session_step(self.h, done.ctypes.data_as(_ip), self.max_systems, C.byref(nd)))
+        check(lib().hsps_session_step(self.h, done.ctypes.data_as(_ip), None, self.max_systems, C.byref(nd)))
         return done[:nd.value].tolist()
 
     def result(self, slot, n, d_x=None):
@@ -623,12 +623,12 @@ STATE_NAMES = {0: "EASY", 1: "MODERATE", 2: "HARD"}
 class ServeResult(C.Structure):
     _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("state", C.c_int), ("batched", C.c_int),
                 ("escalations", C.c_int), ("true_residual", C.c_double), ("queue_s", C.c_double),
-                ("latency_s", C.c_double)]
+                ("latency_s", C.c_double), ("submit_s", C.c_double), ("done_s", C.c_double)]
 
 
 _llp = C.POINTER(C.c_longlong)
 SERVE_SIGNATURES = {
-    "hsps_server_create": (_i, [_i, _i, _d, _i, _i, _d, C.POINTER(_vp)]),
+    "hsps_server_create": (_i, [_i, _i, _d, _i, _i, C.c_longlong, C.POINTER(_vp)]),
     "hsps_server_destroy": (_i, [_vp]),
     "hsps_server_submit": (_i, [_vp, _i, _ip, _ip, _dp, _dp, _PP, _llp]),
     "hsps_server_poll": (_i, [_vp, C.c_longlong, _ip]),
@@ -664,10 +664,10 @@ class Server:
     """Stream of independent solves through the adaptive ladder (hsps_server_*)."""
 
     def __init__(self, device=0, workers=4, hbm_budget=64e9, batch_max=64, batch_rows_max=262144,
-                 batch_window_s=0.002):
+                 batch_rows_total=0):
         h = _vp()
         serve_check(serve_lib().hsps_server_create(device, workers, float(hbm_budget), batch_max, batch_rows_max,
-                                                   float(batch_window_s), C.byref(h)))
+                                                   int(batch_rows_total), C.byref(h)))
         self.h = h
         self.sizes = {}
 

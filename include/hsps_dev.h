@@ -298,8 +298,9 @@ int hsps_session_add(hsps_session S, int n, const int* h_row_ptr, const int* h_c
                      const double* h_b, const double* d_x0, int* slot);
 /* Runs one chunk (p->check_every iterations) of every live system; the
  * slots whose solves ended (reported once each, one chunk after the fact)
- * go to done[0 .. *n_done), at most cap of them per call (the rest follow). */
-int hsps_session_step(hsps_session S, int* done, int cap, int* n_done);
+ * go to done[0 .. *n_done), at most cap of them per call (the rest follow),
+ * and whether each converged to converged[] (may be NULL). */
+int hsps_session_step(hsps_session S, int* done, int* converged, int cap, int* n_done);
 /* The result of a finished slot: x to h_x (host) and / or d_x (device, n
  * doubles; either may be NULL), iterations, convergence (FP64 true residual
  * below tol) and the FP64 true relative residual; the slot is then free.

@@ -4,7 +4,9 @@
  * (adaptive_selector.cpp:298-394); this serves a stream of independent
  * requests, each through that adaptive ladder, with HBM-budget admission,
  * concurrent workers (one device context and stream each) and continuous
- * batching of small EASY (PCG + Jacobi) requests into hsps_batch launches.
+ * batching: small EASY (PCG + Jacobi) requests join a running session
+ * (hsps_session_*, hsps_dev.h) at its next chunk boundary and leave it when
+ * they end.
  * Status codes and hsps_params are those of hsps_dev.h.
  * ========================================================================== */
 #ifndef HSPS_SERVE_H
@@ -24,19 +26,24 @@ typedef struct {
     int converged;        /* 1 if x meets tol in the FP64 true residual */
     int iterations;       /* of the attempt that produced x */
     int state;            /* ladder state of that attempt: 0 EASY, 1 MODERATE, 2 HARD */
-    int batched;          /* 1 if the EASY attempt ran inside a batched launch */
+    int batched;          /* 1 if the EASY attempt ran inside a session */
     int escalations;      /* ladder steps taken after the first attempt */
     double true_residual; /* ||b - A x|| / ||b|| in FP64 */
     double queue_s;       /* submit -> admission */
     double latency_s;     /* submit -> completion */
+    double submit_s;      /* submission time, CLOCK_MONOTONIC seconds */
+    double done_s;        /* completion time, CLOCK_MONOTONIC seconds */
 } hsps_serve_result;
 
-/* workers: concurrent solves; hbm_budget_bytes: admitted requests' estimated
- * device footprint; batch_max: requests per batched launch (1 disables
- * batching); batch_rows_max: largest batched request; batch_window_s: longest
- * wait of an EASY request for company. */
+/* workers: concurrent ladder solves; hbm_budget_bytes: admitted requests'
+ * estimated device footprint; batch_max: systems a session holds at once (1
+ * disables sessions); batch_rows_max: largest request a session takes;
+ * batch_rows_total: rows a session holds (0 = min(batch_max x batch_rows_max,
+ * 2^24)), nonzeros 8 x that.  Sessions tile systems at the size the batch
+ * autotuner picks once per process and device, and replay their chunks as
+ * HIP graphs. */
 int hsps_server_create(int device, int workers, double hbm_budget_bytes, int batch_max, int batch_rows_max,
-                       double batch_window_s, hsps_server* out);
+                       long long batch_rows_total, hsps_server* out);
 /* Finishes every submitted request, then stops. */
 int hsps_server_destroy(hsps_server S);
 /* Copies the n x n CSR system and b; returns at once with a ticket.  p may be

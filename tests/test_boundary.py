@@ -53,7 +53,7 @@ def test_serve_library_exports_every_declared_symbol(H):
 
 def test_serve_null_arguments_are_status_codes(H):
     L = H.serve_lib()
-    assert L.hsps_server_create(0, 1, 1e9, 8, 1000, 0.001, None) == -1
+    assert L.hsps_server_create(0, 1, 1e9, 8, 1000, 0, None) == -1
     assert b"null" in L.hsps_server_last_error().lower()
     t = H.C.c_longlong()
     assert L.hsps_server_submit(None, 4, None, None, None, None, None, H.C.byref(t)) == -1

@@ -2,7 +2,7 @@
 
 Each request goes through the adaptive ladder of adaptive_selector.cpp:298-394
 (initial state from the features, escalation on failure, warm start); EASY
-requests are solved in batched launches.  Bars: every request's ladder state
+requests join a running session (continuous batching at chunk boundaries).  Bars: every request's ladder state
 and escalation count equal the FP64 oracle's adaptive solve of the same system;
 its x meets tol in the FP64 true residual and lies within 1e-6 relative of
 the oracle's x; a request's result is bit-identical whether it is served in a
@@ -53,8 +53,9 @@ def serve(H, items, params, **kw):
 def test_server_follows_the_oracle_ladder(H, precision):
     items = systems()
     p = H.default_params(tol=1e-8, max_iter=2000, precision=precision)
-    res = serve(H, items, p, workers=3, batch_window_s=0.02)
+    res = serve(H, items, p, workers=3)
     for (A, _, b), (ok, x, r) in zip(items, res):
+        assert r.submit_s <= r.done_s and abs((r.done_s - r.submit_s) - r.latency_s) < 1e-3
         xo, so = O.adaptive(A, b, tol=1e-8, max_iter=2000)
         assert so.converged and ok and r.converged
         assert (r.state, r.escalations) == (so.state, so.escalations), (H.STATE_NAMES[r.state], so.state)
@@ -62,14 +63,14 @@ def test_server_follows_the_oracle_ladder(H, precision):
         assert rel <= 1e-8 and abs(rel - r.true_residual) <= 1e-3 * rel + 1e-16
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
         if r.state == 0:
-            assert r.batched  # EASY requests of this size are served by batched launches
+            assert r.batched  # EASY requests of this size are served by the session
 
 
 def test_server_request_independent_of_the_mix(H):
     items = systems()
     p = H.default_params(tol=1e-8, max_iter=2000, precision="mixed")
-    mix = serve(H, items, p, workers=3, batch_window_s=0.05)
-    tight = serve(H, items, p, workers=2, hbm_budget=1.0, batch_window_s=0.0)  # one request in flight
+    mix = serve(H, items, p, workers=3)
+    tight = serve(H, items, p, workers=2, hbm_budget=1.0)  # one request in flight
     for k in [0, 2, 3, 4, 5]:
         alone = serve(H, [items[k]], p, workers=1)[0]
         for other in (mix[k], tight[k]):
@@ -81,10 +82,11 @@ def test_server_escalates_like_the_reference(H):
     # Too few iterations for CG + Jacobi: the ladder climbs, warm-started.
     # Poisson 20^2 with 15 iterations ends at MODERATE (the oracle needs 8
     # there), 48^2 with 10 at HARD (4); the two iteration caps also keep the
-    # EASY attempts in separate batched launches.
+    # EASY attempts in separate sessions; the session's x reaches MODERATE
+    # through device memory.
     items = systems([("poisson", 20), ("poisson", 48)])
     caps = [15, 10]
-    srv = H.Server(workers=2, batch_window_s=0.02)
+    srv = H.Server(workers=2)
     tickets = [srv.submit(*arr, b, H.default_params(tol=1e-8, max_iter=m, precision="fp64"))
                for (_, arr, b), m in zip(items, caps)]
     res = [srv.wait(t) for t in tickets]
@@ -108,3 +110,28 @@ def test_server_rejects_bad_requests(H):
     ok, x, r = srv.wait(srv.submit(rp, np.array([0, 1], np.int32), np.array([2.0, 4.0]), np.array([1.0, 1.0])))
     assert ok and np.allclose(x, [0.5, 0.25])  # and the server still serves
     srv.close()
+
+
+def test_server_request_joins_a_running_session(H):
+    """Small EASY requests submitted while a large one iterates join the
+    running session at a chunk boundary, leave it first, and give the bits
+    they give alone."""
+    import time
+    big = systems([("poisson", 1000)])[0]
+    small = systems([("poisson", 40), ("aniso", 30, 0.3)])
+    p = H.default_params(tol=1e-10, max_iter=20000, precision="mixed")
+    srv = H.Server(workers=2, batch_rows_max=1 << 21)
+    srv.wait(srv.submit(*small[0][1], small[0][2], p))  # session created, tile rows chosen
+    tb = srv.submit(*big[1], big[2], p)
+    time.sleep(0.05)
+    ts = [srv.submit(*arr, b, p) for _, arr, b in small]
+    rs = [srv.wait(t) for t in ts]
+    rb = srv.wait(tb)
+    srv.close()
+    assert rb[0] and rb[2].batched and rb[2].state == 0
+    for (A, arr, b), (ok, x, r) in zip(small, rs):
+        assert ok and r.batched and r.state == 0
+        assert r.done_s < rb[2].done_s  # left the session while the big one still ran
+        alone = serve(H, [(A, arr, b)], p, workers=1)[0]
+        assert alone[2].iterations == r.iterations
+        np.testing.assert_array_equal(alone[1], x)
