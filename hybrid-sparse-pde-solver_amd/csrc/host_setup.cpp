@@ -61,74 +61,94 @@ HostCsr ilu0(const HostCsr& A) {
 
 // ILUT(tau, p), ilu.cpp:94-220.  The reference scans every column k < i of
 // a dense work row; the same arithmetic in the same order visits only the
-// active columns, taken from a min-heap (an elimination only activates
-// columns j > k, so the pops come out in increasing order, each column once),
-// and resets only the columns the row touched.  Same factors, bit for bit;
-// O(nnz(L\U) * row length * log) instead of O(n^2).
+// active columns, found as the set bits of a bitmap in increasing order (an
+// elimination of k only activates columns j > k, so a forward scan of the
+// bitmap meets every pivot once, in order, including those activated during
+// the scan), and resets only the columns the row touched.  The kept upper
+// entries come out of the same bitmap already sorted.  Same factors, bit for
+// bit; O(nnz(L\U) * row length + band / 64) per row instead of O(n).
 HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     const int n = A.rows;
-    std::vector<std::vector<std::pair<int, double>>> L(n), U(n);
+    // finished rows, flat: L strictly lower, U from the diagonal on
+    std::vector<long long> lp(n + 1, 0), up(n + 1, 0);
+    std::vector<int> lc, uc;
+    std::vector<double> lv, uv;
+    lc.reserve(A.ci.size());
+    lv.reserve(A.ci.size());
+    uc.reserve(A.ci.size());
+    uv.reserve(A.ci.size());
     std::vector<double> w(n, 0.0), udiag(n, 0.0);
-    std::vector<char> act(n, 0), seen(n, 0);
-    std::vector<int> touched, heap, upper;
+    std::vector<unsigned long long> act((size_t)n / 64 + 1, 0);  // active columns of the work row
+    std::vector<char> seen(n, 0);
+    std::vector<int> touched;
+    std::vector<std::pair<int, double>> Lr, Ur;
+    auto on = [&](int j) { return (act[(size_t)j >> 6] >> (j & 63)) & 1ull; };
+    auto set = [&](int j) { act[(size_t)j >> 6] |= 1ull << (j & 63); };
+    auto clr = [&](int j) { act[(size_t)j >> 6] &= ~(1ull << (j & 63)); };
     auto touch = [&](int j) {
         if (!seen[j]) { seen[j] = 1; touched.push_back(j); }
     };
-    auto push = [&](int j) {  // min-heap of pending pivot columns
-        heap.push_back(j);
-        std::push_heap(heap.begin(), heap.end(), std::greater<int>());
-    };
     for (int i = 0; i < n; ++i) {
         touched.clear();
+        Lr.clear();
+        Ur.clear();
         double rn = 0.0;
+        int hi = i;  // largest column touched
         for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
             const int j = A.ci[p];
             w[j] = A.v[p];
-            act[j] = 1;
+            set(j);
             touch(j);
             rn += A.v[p] * A.v[p];
-            if (j < i) push(j);
+            hi = std::max(hi, j);
         }
         const double tau = tau_rel * std::sqrt(rn);
-        std::vector<std::pair<int, double>> Lr, Ur;
         // pivot columns k < i in increasing order (ilu.cpp:121-146)
-        while (!heap.empty()) {
-            std::pop_heap(heap.begin(), heap.end(), std::greater<int>());
-            const int k = heap.back();
-            heap.pop_back();
-            if (!act[k]) continue;
-            if (std::abs(w[k]) < tau) { w[k] = 0.0; act[k] = 0; continue; }
+        int pos = A.rp[i] < A.rp[i + 1] ? std::min(A.ci[A.rp[i]], i) : i;
+        while (pos < i) {
+            const size_t wi = (size_t)pos >> 6;
+            const unsigned long long word = act[wi] & (~0ull << (pos & 63));
+            if (!word) {
+                pos = (int)((wi + 1) << 6);
+                continue;
+            }
+            const int k = (int)(wi << 6) + __builtin_ctzll(word);
+            if (k >= i) break;
+            pos = k + 1;
+            if (std::abs(w[k]) < tau) { w[k] = 0.0; clr(k); continue; }
             const double ukk = udiag[k];
-            if (std::abs(ukk) < kEps) { act[k] = 0; continue; }
+            if (std::abs(ukk) < kEps) { clr(k); continue; }
             const double lik = w[k] / ukk;
             w[k] = lik;
-            for (const auto& e : U[k]) {
-                const int j = e.first;
+            for (long long q = up[k]; q < up[k + 1]; ++q) {
+                const int j = uc[q];
                 if (j <= k) continue;
-                if (!act[j] && std::abs(lik * e.second) >= tau) {
-                    act[j] = 1;
-                    if (j < i) push(j);
-                }
-                w[j] -= lik * e.second;
+                if (!on(j) && std::abs(lik * uv[q]) >= tau) set(j);
+                w[j] -= lik * uv[q];
                 touch(j);
+                hi = std::max(hi, j);
             }
-            if (act[k] && std::abs(w[k]) >= tau) Lr.emplace_back(k, w[k]);  // ilu.cpp:151-155, ascending k
+            if (std::abs(w[k]) >= tau) Lr.emplace_back(k, w[k]);  // ilu.cpp:151-155, ascending k
         }
         double d = w[i];
         if (std::abs(d) < kEps) d = (d >= 0 ? 1.0 : -1.0) * kEps * 1e6;
         Ur.emplace_back(i, d);
-        upper.clear();
-        for (int j : touched)
-            if (j > i && act[j] && std::abs(w[j]) >= tau) upper.push_back(j);
-        std::sort(upper.begin(), upper.end());
-        for (int j : upper) Ur.emplace_back(j, w[j]);
+        for (size_t wi = (size_t)(i + 1) >> 6; i + 1 <= hi && wi <= (size_t)hi >> 6; ++wi) {
+            unsigned long long word = act[wi];
+            if (wi == (size_t)(i + 1) >> 6) word &= ~0ull << ((i + 1) & 63);
+            while (word) {
+                const int j = (int)(wi << 6) + __builtin_ctzll(word);
+                word &= word - 1;
+                if (std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
+            }
+        }
         for (int j : touched) {
             w[j] = 0.0;
-            act[j] = 0;
+            clr(j);
             seen[j] = 0;
         }
         w[i] = 0.0;
-        act[i] = 0;
+        clr(i);
         const int keep = A.rp[i + 1] - A.rp[i] + fill_p;
         auto limit = [keep](std::vector<std::pair<int, double>>& row) {
             if (keep <= 0 || (int)row.size() <= keep) return;
@@ -142,19 +162,21 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
         udiag[i] = 0.0;
         for (const auto& e : Ur)
             if (e.first == i) { udiag[i] = e.second; break; }
-        L[i] = std::move(Lr);
-        U[i] = std::move(Ur);
+        for (const auto& e : Lr) { lc.push_back(e.first); lv.push_back(e.second); }
+        for (const auto& e : Ur) { uc.push_back(e.first); uv.push_back(e.second); }
+        lp[i + 1] = (long long)lc.size();
+        up[i + 1] = (long long)uc.size();
     }
     HostCsr F;
     F.rows = F.cols = n;
     F.rp.assign(n + 1, 0);
-    size_t total = 0;
-    for (int i = 0; i < n; ++i) total += L[i].size() + U[i].size();
-    F.ci.reserve(total);
-    F.v.reserve(total);
+    F.ci.reserve(lc.size() + uc.size());
+    F.v.reserve(lc.size() + uc.size());
     for (int i = 0; i < n; ++i) {
-        for (const auto& e : L[i]) { F.ci.push_back(e.first); F.v.push_back(e.second); }
-        for (const auto& e : U[i]) { F.ci.push_back(e.first); F.v.push_back(e.second); }
+        F.ci.insert(F.ci.end(), lc.begin() + lp[i], lc.begin() + lp[i + 1]);
+        F.v.insert(F.v.end(), lv.begin() + lp[i], lv.begin() + lp[i + 1]);
+        F.ci.insert(F.ci.end(), uc.begin() + up[i], uc.begin() + up[i + 1]);
+        F.v.insert(F.v.end(), uv.begin() + up[i], uv.begin() + up[i + 1]);
         F.rp[i + 1] = (int)F.ci.size();
     }
     return F;

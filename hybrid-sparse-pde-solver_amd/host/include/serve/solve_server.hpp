@@ -56,6 +56,8 @@ struct ServeResult {
     double latency_s = 0;    ///< submit -> completion
     double submit_s = 0;     ///< submission time, CLOCK_MONOTONIC seconds
     double done_s = 0;       ///< completion time, CLOCK_MONOTONIC seconds
+    double setup_s = 0;      ///< preconditioner setups of all attempts (host and device)
+    double solve_s = 0;      ///< Krylov solves of all attempts (a session attempt: join to end)
 };
 
 class SolveServer {
@@ -86,7 +88,7 @@ private:
         std::vector<Real> b, x;
         SolverParams p;
         double bytes = 0;
-        Clock::time_point t_submit, t_admit;
+        Clock::time_point t_submit, t_admit, t_join;
         ServeResult res;
         bool done = false;
     };

@@ -287,6 +287,7 @@ bool SolveServer::join(std::vector<Sess>& sessions, const Req& r) {
                                     r->b.data(), nullptr, &slot);
     if (rc == HSPS_OK) {
         S->slots[slot] = r;
+        r->t_join = Clock::now();
         return true;
     }
     if (rc == HSPS_ERR_NOMEM && !S->slots.empty()) return false;  // room frees as systems end
@@ -317,6 +318,7 @@ void SolveServer::step(Sess& S) {
         if (it == S.slots.end()) continue;
         Req r = it->second;
         S.slots.erase(it);
+        r->res.solve_s += std::chrono::duration<double>(Clock::now() - r->t_join).count();
         const Index n = r->A.rows();
         int iters = 0, cv = 0;
         double tr = 0.0;
@@ -394,6 +396,8 @@ void SolveServer::run_ladder(const Handoff& h) {
             auto pair = SolverFactory::make_for_state(state, r->A, r->p, &setup);
             SolverStats st;
             const bool ok = pair.first->solve(r->A, b, x, st);
+            r->res.setup_s += setup;
+            r->res.solve_s += st.solve_time_s;
             r->res.converged = ok;
             r->res.iterations = st.iterations;
             r->res.true_residual = st.true_residual;
@@ -541,6 +545,8 @@ int hsps_server_wait(hsps_server S, long long ticket, double* x, hsps_serve_resu
             res->latency_s = r.latency_s;
             res->submit_s = r.submit_s;
             res->done_s = r.done_s;
+            res->setup_s = r.setup_s;
+            res->solve_s = r.solve_s;
         }
         return r.converged ? 1 : 0;
     });

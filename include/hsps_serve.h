@@ -33,6 +33,8 @@ typedef struct {
     double latency_s;     /* submit -> completion */
     double submit_s;      /* submission time, CLOCK_MONOTONIC seconds */
     double done_s;        /* completion time, CLOCK_MONOTONIC seconds */
+    double setup_s;       /* preconditioner setups of all attempts */
+    double solve_s;       /* Krylov solves of all attempts (a session attempt: join to end) */
 } hsps_serve_result;
 
 /* workers: concurrent ladder solves; hbm_budget_bytes: admitted requests'
