@@ -223,6 +223,9 @@ int sstep_cg_dev(const Csr& A, const double* b, double* x, const hsps_params& p,
 struct Precond;
 void precond_apply_f64(const Precond& M, const double* r, double* z, hipStream_t s, const int* skip);
 void precond_apply_f32(const Precond& M, const float* r, float* z, hipStream_t s, const int* skip);
+// Raises the error a preconditioner's sweeps flagged on the device since the
+// last check (synchronises the context's stream when there is one to read).
+void precond_check(const Precond* M);
 int precond_kind(const Precond& M);
 const double* precond_jacobi_dinv64(const Precond& M);
 const float* precond_jacobi_dinv32(const Precond& M);

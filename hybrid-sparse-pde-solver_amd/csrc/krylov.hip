@@ -1027,8 +1027,10 @@ int hsps_cg_solve(hsps_csr A, hsps_precond M, const double* b, double* x, const 
                 return r.converged;
             }
         }
-        return p.precision == HSPS_MIXED ? run_cg_alloc<float>(*A, M, b, x, p, st)
-                                         : run_cg_alloc<double>(*A, M, b, x, p, st);
+        const int rc = p.precision == HSPS_MIXED ? run_cg_alloc<float>(*A, M, b, x, p, st)
+                                                 : run_cg_alloc<double>(*A, M, b, x, p, st);
+        precond_check(M);
+        return rc;
     });
 }
 
@@ -1039,8 +1041,10 @@ int hsps_cg_solve_noalloc(hsps_csr A, hsps_precond M, const double* b, double* x
         hsps_params p;
         if (pp) p = *pp; else hsps_params_default(&p);
         A->ctx->set_current();
-        return p.precision == HSPS_MIXED ? run_cg_noalloc<float>(*A, M, b, x, p, st)
-                                         : run_cg_noalloc<double>(*A, M, b, x, p, st);
+        const int rc = p.precision == HSPS_MIXED ? run_cg_noalloc<float>(*A, M, b, x, p, st)
+                                                 : run_cg_noalloc<double>(*A, M, b, x, p, st);
+        precond_check(M);
+        return rc;
     });
 }
 
@@ -1051,8 +1055,10 @@ int hsps_pipelined_fgmres_solve(hsps_csr A, hsps_precond M, const double* b, dou
         hsps_params p;
         if (pp) p = *pp; else hsps_params_default(&p);
         A->ctx->set_current();
-        return p.precision == HSPS_MIXED ? run_fgmres<float>(*A, M, b, x, p, st, true)
-                                         : run_fgmres<double>(*A, M, b, x, p, st, true);
+        const int rc = p.precision == HSPS_MIXED ? run_fgmres<float>(*A, M, b, x, p, st, true)
+                                                 : run_fgmres<double>(*A, M, b, x, p, st, true);
+        precond_check(M);
+        return rc;
     });
 }
 
@@ -1084,8 +1090,10 @@ int hsps_fgmres_solve(hsps_csr A, hsps_precond M, const double* b, double* x, co
         hsps_params p;
         if (pp) p = *pp; else hsps_params_default(&p);
         A->ctx->set_current();
-        return p.precision == HSPS_MIXED ? run_fgmres<float>(*A, M, b, x, p, st)
-                                         : run_fgmres<double>(*A, M, b, x, p, st);
+        const int rc = p.precision == HSPS_MIXED ? run_fgmres<float>(*A, M, b, x, p, st)
+                                                 : run_fgmres<double>(*A, M, b, x, p, st);
+        precond_check(M);
+        return rc;
     });
 }
 

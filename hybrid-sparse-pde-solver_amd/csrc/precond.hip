@@ -472,8 +472,23 @@ static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {
     f->pos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
     f->opos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
     f->opos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
+    f->fwd_band = make_tri_band(M.ctx, F, true, invd);
+    f->bwd_band = make_tri_band(M.ctx, F, false, invd);
+    f->band_err = DevBuf(M.ctx, sizeof(int));
+    fill_on(M.ctx->stream, f->band_err.p, 0, sizeof(int));
     f->lu = to_dev(M.ctx, std::move(F));
     M.ilu = std::move(f);
+}
+
+void precond_check(const Precond* M) {
+    if (!M || M->kind != HSPS_PRECOND_ILU || !M->ilu || !(M->ilu->fwd_band.ok || M->ilu->bwd_band.ok)) return;
+    int e = 0;
+    HSPS_HIP(hipMemcpyAsync(&e, M->ilu->band_err.p, sizeof(int), hipMemcpyDeviceToHost, M->ctx->stream));
+    HSPS_HIP(hipStreamSynchronize(M->ctx->stream));
+    if (e) {
+        fill_on(M->ctx->stream, M->ilu->band_err.p, 0, sizeof(int));
+        throw Error(HSPS_ERR_HIP, "ILU band sweep: a wait between waves gave up");
+    }
 }
 
 template <int K, class T, bool UPPER>
@@ -687,8 +702,11 @@ static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int
             if (f.fwd_ell.K && f.bwd_ell.K) {
                 T* ip = dbl ? (T*)f.pos64.p : (T*)f.pos32.p;
                 T* op = dbl ? (T*)f.opos64.p : (T*)f.opos32.p;
-                tri_ell_sweep<T, false>(f.fwd, f.fwd_ell, r, y, ip, op, s, skip);
-                tri_ell_sweep<T, true>(f.bwd, f.bwd_ell, (const T*)y, z, ip, op, s, skip);
+                int* err = f.band_err.as<int>();
+                if (f.fwd_band.ok) tri_band_sweep<T>(f.fwd_band, false, n, r, y, err, s, skip);
+                else tri_ell_sweep<T, false>(f.fwd, f.fwd_ell, r, y, ip, op, s, skip);
+                if (f.bwd_band.ok) tri_band_sweep<T>(f.bwd_band, true, n, (const T*)y, z, err, s, skip);
+                else tri_ell_sweep<T, true>(f.bwd, f.bwd_ell, (const T*)y, z, ip, op, s, skip);
             } else if (dbl) {
                 hipLaunchKernelGGL((k_tri_lower<double, T>), dim3(1), dim3(kTriThreads), 0, s, f.fwd.n_levels,
                                    f.fwd.level_ptr.as<int>(), f.fwd.order.as<int>(), f.lu->rp.as<int>(),
@@ -863,6 +881,7 @@ int hsps_precond_apply_f64(hsps_precond M, const double* r, double* z) {
         if (!M || !r || !z) throw Error(HSPS_ERR_INVALID, "hsps_precond_apply_f64: null argument");
         M->ctx->set_current();
         precond_apply_f64(*M, r, z, M->ctx->stream, nullptr);
+        precond_check(M);
         return HSPS_OK;
     });
 }
@@ -893,6 +912,16 @@ int hsps_amg_aggregates(hsps_precond M, int lv, int* out) {
         std::copy(a.begin(), a.end(), out);
         return HSPS_OK;
     });
+}
+
+int hsps_ilu_sweep_kinds(hsps_precond M, int* fwd, int* bwd, int* w_fwd, int* w_bwd) {
+    if (!M || M->kind != HSPS_PRECOND_ILU || !M->ilu) return HSPS_ERR_INVALID;
+    const IluFactor& f = *M->ilu;
+    if (fwd) *fwd = f.fwd_band.ok;
+    if (bwd) *bwd = f.bwd_band.ok;
+    if (w_fwd) *w_fwd = f.fwd_band.ok ? f.fwd_band.W : 0;
+    if (w_bwd) *w_bwd = f.bwd_band.ok ? f.bwd_band.W : 0;
+    return HSPS_OK;
 }
 
 int hsps_amg_setup_profile(hsps_precond M, double* out) {

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "hsps_internal.hpp"
+#include "tri_band.hpp"
 
 namespace hsps_dev {
 
@@ -43,6 +44,8 @@ struct IluFactor {
     DevBuf tmp64, tmp32;      // y of L y = r
     DevBuf pos64, pos32;      // a sweep's input in level order
     DevBuf opos64, opos32;    // a narrow sweep's output in level order
+    TriBand fwd_band, bwd_band;  // wavefront sweeps, when the factor is banded (else ok = 0)
+    DevBuf band_err;             // int: set if a band sweep's wait gave up
 };
 
 struct AmgLevelDev {

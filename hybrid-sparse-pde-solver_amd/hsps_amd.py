@@ -93,6 +93,7 @@ SIGNATURES = {
     "hsps_amg_aggregates": (_i, [_vp, _i, _ip]),
     "hsps_amg_level_matrix": (_i, [_vp, _i, _i, _ip, _ip, _dp]),
     "hsps_amg_setup_profile": (_i, [_vp, _dp]),
+    "hsps_ilu_sweep_kinds": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "hsps_cg_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_cg_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
     "hsps_fgmres_solve": (_i, [_vp, _vp, _vp, _vp, _PP, _SP]),
@@ -323,6 +324,12 @@ class Precond:
 
     def amg_levels(self):
         return check(lib().hsps_amg_levels(self.h))
+
+    def ilu_sweeps(self):
+        """((forward kind, W), (backward kind, W)): kind 1 = band wavefront, 0 = levels."""
+        v = [C.c_int() for _ in range(4)]
+        check(lib().hsps_ilu_sweep_kinds(self.h, *[C.byref(x) for x in v]))
+        return (v[0].value, v[2].value), (v[1].value, v[3].value)
 
     def amg_level_info(self, lv):
         r, z = C.c_int(), C.c_int()
