@@ -46,6 +46,12 @@ struct TriBand {
     DevBuf sk_of_row;           // [n] skewed position of natural row i
     DevBuf in64, in32;          // [n_sk] sweep input, skewed; padding positions stay zero
     DevBuf out64, out32;        // [n_sk] sweep output, skewed
+    int mode = 0;               // 1: one workgroup (LDS handoff); 2: a workgroup per 4 waves (global handoff)
+    int five = 0;               // two fixed-source slots (five-point ILU(0)): values in slot order, no codes
+    int NWT = 0;                // waves of the multi-workgroup sweep
+    long long n_slot_words = 0;
+    DevBuf slots;               // multi-workgroup handoff slots: [NWT][2 rounds][W][2] tagged 64-bit words
+    DevBuf con;                 // [NWT] consumer progress: pair-stream round x W + columns read
 };
 
 // The band layout of a sweep of F (lower: the strictly lower entries; upper:

@@ -1,12 +1,14 @@
 """Wavefront ILU sweeps over a banded grid layout (k_tri_band) against the
-level-scheduled sweeps (HSPS_ILU_BAND=0 at setup) and the FP64 oracle.
+level-scheduled sweeps (HSPS_ILU_BAND=0 at setup) and the FP64 oracle, for
+both band kernels (several workgroups with a global-memory handoff, the
+default; one workgroup with LDS rings: HSPS_ILU_BAND=1).
 
 Every row is the same expression in the same order on both paths, so the
 bar is bit equality: the FP64 apply through the C ABI, and a mixed FGMRES
 solve (FP32 sweeps inside) whose iterations and x agree bit for bit.  The
 cases cover ILU(0) of the C3 geometry (1024^2 conv-diff, one round of 16
 waves), 2048^2 (32 bands: two rounds), a grid whose last band is partial
-(130^2), ILUT(1e-4) factors with fill (4 and 8 entry slots, lane skews up
+(150^2), ILUT(1e-4) factors with fill (4 and 8 entry slots, lane skews up
 to 4, an empty L), and the oracle's ILU apply within 1e-12.
 """
 import numpy as np
@@ -20,7 +22,7 @@ CV, SV = 0.7648421872844885, 0.644217687237691
 CASES = {
     "convdiff1024_ilu0": (lambda: O.Csr.convdiff(1024, 0.01, CV, SV), 0.0),
     "poisson2048_ilu0": (lambda: O.Csr.poisson(2048), 0.0),
-    "poisson130_ilu0": (lambda: O.Csr.poisson(130), 0.0),
+    "poisson150_ilu0": (lambda: O.Csr.poisson(150), 0.0),
     "convdiff300_ilut": (lambda: O.Csr.convdiff(300, 0.05, 0.3, 0.95), 1e-4),
     "convdiff512_ilut_strong": (lambda: O.Csr.convdiff(512, 0.001, 1.0, 0.0), 1e-4),
     "aniso300_ilut": (lambda: O.Csr.aniso(300, 1.0), 1e-4),
@@ -32,9 +34,11 @@ def dev_csr(H, ctx, A):
     return H.Csr(ctx, A.rows, A.rows, rp, ci, v)
 
 
-def two_preconds(H, ctx, monkeypatch, A, tau):
+def two_preconds(H, ctx, monkeypatch, A, tau, mode="2"):
+    """The band sweep (mode 2: several workgroups, 1: one workgroup) and the level sweep."""
     Ad = dev_csr(H, ctx, A)
     p = H.default_params(drop_tol=tau)
+    monkeypatch.setenv("HSPS_ILU_BAND", mode)
     Mb = H.Precond(Ad, "ilu", p)
     monkeypatch.setenv("HSPS_ILU_BAND", "0")
     Ml = H.Precond(Ad, "ilu", p)
@@ -42,11 +46,12 @@ def two_preconds(H, ctx, monkeypatch, A, tau):
     return Ad, Mb, Ml
 
 
+@pytest.mark.parametrize("mode", ["2", "1"])
 @pytest.mark.parametrize("case", list(CASES))
-def test_band_apply_equals_level_apply(H, ctx, monkeypatch, case):
+def test_band_apply_equals_level_apply(H, ctx, monkeypatch, case, mode):
     make, tau = CASES[case]
     A = make()
-    Ad, Mb, Ml = two_preconds(H, ctx, monkeypatch, A, tau)
+    Ad, Mb, Ml = two_preconds(H, ctx, monkeypatch, A, tau, mode)
     (fk, fw), (bk, bw) = Mb.ilu_sweeps()
     assert bk == 1 and bw > 0, (case, Mb.ilu_sweeps())  # the backward sweep takes the band path
     assert Ml.ilu_sweeps() == ((0, 0), (0, 0))
