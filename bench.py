@@ -723,12 +723,12 @@ def bench_serve(H, ctx, args):
     warm = [("poisson", 64, 0.0, 0.0, 0.0), ("convdiff", 64, 0.01, 0.8, 0.6), ("aniso", 64, 1e-3, 0.0, 0.0)]
     for k, spec in enumerate(warm):
         srv.wait(srv.submit(*serve_request(H, spec, 10 ** 6 + k), p))
-    pool = ThreadPoolExecutor(4)
+    pool = ThreadPoolExecutor(6)
     ahead, nxt = deque(), 0
 
     def refill():
         nonlocal nxt
-        while nxt < len(specs) and len(ahead) < 16:
+        while nxt < len(specs) and len(ahead) < 32:
             ahead.append(pool.submit(serve_request, H, specs[nxt], nxt))
             nxt += 1
 
@@ -804,8 +804,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-energy", action="store_true", help="skip the device energy measurement")
     ap.add_argument("--workload", choices=["batched", "single", "amg", "convdiff", "serve"], default="batched")
-    ap.add_argument("--requests", type=int, default=300, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--requests", type=int, default=1000, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=8.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="batched workload: requests of the serving leg run after it (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="batched workload: the serving leg's arrival rate")
     ap.add_argument("--workers", type=int, default=8, help="serve: concurrent ladder solves")
     ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: admission budget (bytes)")
     ap.add_argument("--serve-nmax", type=float, default=4e6, help="serve: largest request (rows)")
@@ -846,8 +849,14 @@ def main():
         print(json.dumps({"spmv": roof, "n": rn, "nnz": rnnz, "calibration_copy_bytes": nbytes}), flush=True)
         ctx.close()
         return
+    serve = None
     if args.workload == "batched":
         res = bench_batched(H, ctx, ev, args)
+        if args.serve_requests > 0 and not args.only_batched:  # serving leg: per-request latency
+            sargs = argparse.Namespace(**vars(args))
+            sargs.requests = args.serve_requests
+            sargs.rate = args.serve_rate
+            serve = bench_serve(H, ctx, sargs)
     elif args.workload == "serve":
         res = bench_serve(H, ctx, args)
     elif args.workload == "convdiff":
@@ -904,6 +913,16 @@ def main():
         "roofline": roofline,
         "cpu_baseline": res["cpu_baseline"],
     }
+    if serve is not None:
+        sv = serve["serve"]
+        line["p50_latency_ms"] = sv["p50_ms"]
+        line["p99_latency_ms"] = sv["p99_ms"]
+        line["serve"] = dict(sv, workload=serve["config"]["workload"], requests=serve["config"]["requests"],
+                             rate_per_s=serve["config"]["rate_per_s"], workers=serve["config"]["workers"],
+                             total_rows=serve["config"]["total_rows"], throughput_solves_per_s=serve["value"])
+    elif args.workload == "serve":
+        line["p50_latency_ms"] = res["serve"]["p50_ms"]
+        line["p99_latency_ms"] = res["serve"]["p99_ms"]
     print(json.dumps(line), flush=True)
     ctx.close()
 
