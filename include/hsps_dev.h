@@ -175,8 +175,8 @@ int hsps_precond_apply_f64(hsps_precond M, const double* d_r, double* d_z);
 /* The ILU factors hsps_precond_create(HSPS_PRECOND_ILU) uploads, computed on
  * the host only (no device call): ILU(0) on A's pattern (ilu.cpp:41-89) when
  * drop_tol == 0 and fill_per_row == 0, else ILUT(drop_tol, fill_per_row)
- * (ilu.cpp:94-220, the same factors as the reference's dense row scan, in
- * O(nnz) work) as one L\U CSR (unit L implicit).  *nnz_out receives the
+ * (ilu.cpp:94-220, the same factors as the reference's dense row scan, which
+ * visits only the row's active columns) as one L\U CSR (unit L implicit).  *nnz_out receives the
  * factor's nonzeros; lu_rp (n+1 ints) and lu_ci / lu_v (nnz_out entries) are
  * filled where not NULL, so a first call with NULL arrays sizes them. */
 int hsps_ilu_factor(int n, const int* h_row_ptr, const int* h_col_idx, const double* h_values, double drop_tol,
