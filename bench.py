@@ -12,8 +12,8 @@ inputs resident in HBM.
                  Krylov vectors, FP64 reductions, FP64 reliable-update
                  refinement), all 512 systems in one batch
   fp64_baseline  solves/s of the plain FP64 device PCG, one system at a time
-  roofline       the batched iteration at full load (K1 + K2 + the two
-                 per-system finish kernels over every tiled row): 34 B of
+  roofline       the batched iteration at full load (K1 + K2, each with its
+                 per-system finish fused in, over every tiled row): 34 B of
                  algorithmic traffic per row (stencil storage, FP32), timed as
                  (t(96 iterations) - t(32 iterations)) / 64
   spmv_roofline  the CSR SpMV kernel (FP64) on the 2048^2 Poisson matrix
@@ -47,7 +47,7 @@ SPMV_FP64_HBM_BYTES = (154831 * 2.0 + 32768 * 1.0) * 1024
 # `bench.py --full-load-only --tile-rows 2048`: the median full-load dispatch
 # of each kernel (factors 2.000 / 1.000 from the 256 MiB copy of the same run)
 # divided by the 21,108,314 tiled rows (profiles/batched512_pcg_jacobi_kernels.md).
-BATCHED_HBM_BYTES_PER_ROW_ITER = 35.1
+BATCHED_HBM_BYTES_PER_ROW_ITER = 35.3
 
 
 def log(*a):
@@ -878,8 +878,8 @@ def main():
     if bi and bi.get("full_load_achieved_GBs"):
         alg = bi["bytes_per_row_iteration"] * bi["tiled_rows"]
         roofline = {"bound": "hbm",
-                    "kernel": "batched PCG iteration at full load: k_b_k1v<float> + k_b_k2v<float> + the two "
-                              f"per-system finish kernels over {bi['tiled_rows']:.0f} stencil-stored rows",
+                    "kernel": "batched PCG iteration at full load: k_b_k1v<float> + k_b_k2v<float> (per-system "
+                              f"finishes fused in) over {bi['tiled_rows']:.0f} stencil-stored rows",
                     "achieved": bi["full_load_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": bi["full_load_achieved_GBs"] / HBM_PEAK_GBS,
                     "traffic": BATCHED_HBM_BYTES_PER_ROW_ITER * bi["tiled_rows"],
@@ -888,7 +888,7 @@ def main():
                                       f"({BATCHED_HBM_BYTES_PER_ROW_ITER:.1f} B vs {bi['bytes_per_row_iteration']} B "
                                       "algorithmic; profiles/batched512_pcg_jacobi_kernels.md)",
                     "algorithmic_bytes": alg, "avg_launch_us": bi["full_load_iteration_us"],
-                    "per": "one iteration (one launch each of K1, F1, K2, F2)"}
+                    "per": "one iteration (one launch each of K1 and K2)"}
         res["spmv_roofline"] = spmv_roof
     else:
         roofline = spmv_roof

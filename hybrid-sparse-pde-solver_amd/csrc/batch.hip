@@ -11,13 +11,15 @@
 //    only — no launches per iteration, vectors stay in L2;
 //  * large systems: cut into tiles of <= tile_rows rows (one workgroup per
 //    tile, tile_rows autotuned), grouped into waves by a working-set budget
-//    (default: one wave).  Per iteration four launches over the wave's tiles:
+//    (default: one wave).  Per iteration, over the wave's tiles:
 //      K1  p' = D^-1 r + beta p (two p buffers), q = A p', p'.q partials
 //          (FLUSH state: FP64 true residual r = b - A (x64 + x_lo))
 //      F1  per-system finish: alpha (or breakdown / reliable-update decisions)
 //      K2  x += alpha p', r -= alpha q, r.r / r.z partials
 //          (after a FLUSH: x64 += x_lo)
 //      F2  per-system finish: rel, rho, beta, convergence
+//    F1 and F2 run inside K1 and K2: the system's workgroup that starts last
+//    sums the tiles' partials and decides, so an iteration is two launches.
 //    The chunk of iterations is captured once per wave in a hipGraph and
 //    replayed; the host polls the wave's states one chunk behind.
 // Systems on <= 8 diagonals are stored by diagonal (constant-coefficient
@@ -81,7 +83,7 @@ __host__ __device__ constexpr int stage_need(int h, int es) {
 // FLUSH: K1 replaces r by the FP64 true residual b - A (x64 + x_lo).
 // FLUSHED: after a FLUSH, K2 folds x_lo into x64; K1 then proceeds as NORMAL.
 enum : int { BS_NORMAL = 0, BS_FLUSH = 1, BS_FLUSHED = 2 };
-enum : int { BR_NONE = 0, BR_RELIABLE = 1, BR_CHECK = 2, BR_RESTART = 3 };
+enum : int { BR_NONE = 0, BR_RELIABLE = 1, BR_CHECK = 2, BR_RESTART = 3, BR_STALLED = 4 };
 // storage of a system inside the batch
 enum : int { SK_CSR = 0, SK_DIA = 1, SK_STENCIL = 2 };
 
@@ -90,18 +92,19 @@ struct BState {
     double alpha, beta;
     int halt, flag;
     int pcur;  // which of the two p buffers holds the current search direction
-    int iter;
+    int seq;   // finishes so far: K1 / K2 launches tag their partials seq + 1
     double rho, rho_old, bnorm, rel, rmax, true_rel;
     int done, reason, reliable, restarts, conv;
     int restarts_pq;  // restarts after a p.q breakdown (the rest follow a rho breakdown)
     double r0;  // initial ||r|| (absolute)
+    int iter;
 };
-static_assert(sizeof(BState) == 112, "the counter fills the padding before r0");
+static_assert(sizeof(BState) == 120, "BState layout");
 struct BHot {
     double alpha, beta;
-    int halt, flag, pcur, iter;
+    int halt, flag, pcur, seq;
 };
-static_assert(offsetof(BState, alpha) == offsetof(BHot, alpha) && offsetof(BState, iter) == offsetof(BHot, iter),
+static_assert(offsetof(BState, alpha) == offsetof(BHot, alpha) && offsetof(BState, seq) == offsetof(BHot, seq),
               "BHot is the head of BState");
 // The head of a system's state in one load (the compiler would otherwise
 // issue each field's load after the branch on the one before).
@@ -127,6 +130,7 @@ struct alignas(128) TileVec {
     int row0, last, H, C;  // the system's rows [row0, last]; staging halo and chunk (stage_halo / stage_chunk)
     int off[5];            // diagonal offsets, ascending (padding: offset 0 with value 0, never present)
     int tile;              // the tile's index (its partials slot)
+    int fin;               // in an active list: the system's workgroup the dispatcher starts last (runs the finish)
     T uv[5];               // the diagonals' values
     T cd;                  // D^-1
 };
@@ -136,7 +140,6 @@ struct BArgs {
     const TileVec<double>* tv64;
     const TileVec<float>* tva32;   // records of the active tiles (one per workgroup of K1 / K2)
     const TileVec<double>* tva64;
-    const int* act_sys;            // the active systems (one per workgroup of F1 / F2)
     const int* rp;
     const int* ci;
     const double* v64;
@@ -172,6 +175,7 @@ struct BArgs {
     int stage_doubles;  // dynamic LDS of K1 / K2, in doubles (>= kLdsNnz for the CSR path)
     BState* st;
     double* partials;
+    unsigned long long* tagp;  // [2 phases][T tiles][2 values][2 words] tagged partials of K1 / K2
     int T;  // total tiles
     double tol, delta;
     int max_iter;
@@ -716,6 +720,10 @@ k_b_init1(BArgs B, int t0, const double* __restrict__ b, const double* __restric
     });
     for (int i = r0 + threadIdx.x; i < r1; i += kBlock) red[1] += b[i] * b[i];
     tile_partial<2>(red, B, tile);
+    if (threadIdx.x < 8) {  // the tile's tagged partials of both phases start untagged
+        const int ph = threadIdx.x >> 2, k = threadIdx.x & 3;
+        B.tagp[((size_t)ph * B.T + tile) * 4 + k] = 0ULL;
+    }
 }
 
 struct OpInit1 {
@@ -773,82 +781,7 @@ struct OpInit2 {
     }
 };
 
-// ---- K1 / F1 -------------------------------------------------------------------------------
-// NORMAL / FLUSHED: p_new = D^-1 r + beta p_old into the other p buffer, and
-// q = A p_new with p.q partials.  The p update of cg_solver.cpp:173-176 is
-// evaluated on the fly for every column the tile's rows touch (the same
-// expression on the same inputs as the row's owner, so bit-identical), which
-// removes a pass over r and p per iteration.  Stencil-stored systems do not
-// store q: K2 recomputes it from p_new.
-// FLUSH: FP64 true residual r = b - A (x64 + x_lo).
-template <class T, bool MIXED>
-__global__ void __launch_bounds__(kBlock)
-k_b_k1(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
-       const double* __restrict__ b, const double* __restrict__ x64, const float* __restrict__ xlo) {
-    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
-    // every access to the staging buffer goes through address-space-3 pointers
-    const lds_ptr<double> prod64 = (lds_ptr<double>)stage_lds;
-    const TileVec<T>& D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
-    if (D.vec) return;  // k_b_k1v runs it
-    const int tile = D.tile;
-    const int s = B.tile_sys[tile];
-    const BState* S0 = &B.st[s];
-    if (S0->halt) return;
-    const int flag = S0->flag;
-    const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
-    const bool cdok = B.cd_ok[s] != 0;
-    const T cdv = Prec<T>::cd(B)[s];
-    const T* __restrict__ dinv = Prec<T>::dinv(B);
-    double red[2] = {0.0, 0.0};
-    if (flag == BS_FLUSH) {
-        auto xv = [&](int j) { return MIXED ? x64[j] + (double)xlo[j] : x64[j]; };
-        tile_rows64(B, s, r0, r1, xv, prod64, [&](int i, double sum) {
-            const double rh = b[i] - sum;
-            const T ri = (T)rh;
-            r[i] = ri;
-            red[0] += rh * rh;
-            red[1] += (double)ri * (double)(T)((cdok ? cdv : dinv[i]) * ri);
-        });
-        tile_partial<2>(red, B, tile);
-        return;
-    }
-    const double beta = S0->beta;
-    const T* __restrict__ po = S0->pcur ? p1 : p0;
-    T* __restrict__ pn = S0->pcur ? p0 : p1;
-    auto pnew = [&](int j) -> T {
-        const T dj = cdok ? cdv : dinv[j];
-        return (T)((double)(T)(dj * r[j]) + beta * (double)po[j]);
-    };
-    const int kind = B.dia_kind[s];
-    if (kind != SK_CSR) {
-        const Stencil<T> A(B, s);
-        const lds_ptr<T> lds = (lds_ptr<T>)prod64;
-        const int cap = B.stage_doubles * (int)(sizeof(double) / sizeof(T));
-        auto out = [&](int i, T pi, T sum) {
-            if (kind == SK_DIA) q[i] = sum;
-            red[0] += (double)pi * (double)sum;
-        };
-        if (stage_need(A.halo, (int)sizeof(T)) <= cap) {  // p_new staged in LDS, evaluated once per row
-            A.dispatch([&](auto nd) {
-                stencil_staged<nd()>(A, r0, r1, lds, cap, pnew, [&](int j, T v) { pn[j] = v; }, out);
-            });
-        } else {  // halo wider than the staging buffer: rows straight from memory
-            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
-                const T pi = pnew(i);
-                pn[i] = pi;
-                out(i, pi, A.template row<0>(i, pnew));
-            }
-        }
-    } else {
-        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) pn[i] = pnew(i);
-        tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), pnew, (lds_ptr<T>)prod64, [&](int i, T sum) {
-            q[i] = sum;
-            red[0] += (double)pnew(i) * (double)sum;
-        });
-    }
-    tile_partial<2>(red, B, tile);
-}
-
+// ---- the per-system decisions after K1 (F1) and K2 (F2) -------------------------------------
 struct OpF1 {
     __device__ bool active(const BState& S) const { return !S.halt; }
     __device__ void operator()(BState& S, const double* tot, const BArgs& B) const {
@@ -899,6 +832,232 @@ struct OpF1 {
     }
 };
 
+template <bool MIXED>
+struct OpF2 {
+    __device__ bool active(const BState& S) const { return !S.halt && S.flag == BS_NORMAL; }
+    __device__ void operator()(BState& S, const double* tot, const BArgs& B) const {
+        const double rn = sqrt(tot[0]);
+        S.rel = rn / S.bnorm;
+        S.iter += 1;
+        if (S.rel < B.tol) {  // cg_solver.cpp:143, confirmed on the FP64 true residual
+            S.flag = BS_FLUSH;
+            S.reason = BR_CHECK;
+            return;
+        }
+        if (MIXED) {
+            S.rmax = fmax(S.rmax, rn);
+            if (rn < B.delta * S.rmax) {
+                S.flag = BS_FLUSH;
+                S.reason = BR_RELIABLE;
+                return;
+            }
+        }
+        S.rho_old = S.rho;
+        S.rho = tot[1];
+        if (fabs(S.rho_old) < kEps * 1e8 * fabs(S.rho)) {  // cg_solver.cpp:157-167: restart
+            S.restarts += 1;
+            S.flag = BS_FLUSH;
+            S.reason = BR_RESTART;
+            if (S.iter >= B.max_iter) S.halt = 1;
+            return;
+        }
+        S.beta = S.rho / S.rho_old;
+        if (S.iter >= B.max_iter) S.halt = 1;
+    }
+};
+
+// ---- finishes inside the iteration kernels -------------------------------------------------
+// An iteration kernel's workgroup publishes its tile's partial sums as tagged
+// 64-bit words (tag << 32 | 32 bits of the value, two words a value, written
+// with relaxed device-scope atomics: each word is valid exactly when its tag
+// is the launch's, so no flag, fence or cache writeback orders anything).  One
+// workgroup per system — the one the dispatcher starts last (blocks start in
+// increasing index order; its record carries fin = 1) — then waits for the
+// tags of the system's other tiles, which all started before it and wait on
+// nothing, and runs the system's finish: the sum of k_b_finish in the same
+// order, then the decisions.  No atomics and no extra launch.  The tag is the
+// system's finish count + 1, read with the state at the kernel's start, so
+// every launch tags differently; a system's tags are cleared when it starts.
+// A wait that never sees its tag halts the system after 2 s (BR_STALLED)
+// instead of hanging.
+constexpr long long kFinishWaitTicks = 200000000LL;  // 2 s of s_memrealtime (100 MHz)
+
+__device__ __forceinline__ void tag_st(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long tag_ld(const unsigned long long* p) {
+    return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NV, int BS, class Op>
+__device__ __forceinline__ void tile_finish(double (&v)[NV], const BArgs& B, int tile, int s, int phase,
+                                            unsigned tag, int fin, Op op) {
+    static_assert(BS >= kBlock && BS % kBlock == 0, "the finish uses the first 256 threads");
+    __shared__ double red[NV * (BS / 64)];
+    block_sum<NV, BS>(v, red);
+    if (threadIdx.x == 0) {
+        unsigned long long* w = B.tagp + ((size_t)phase * B.T + tile) * (2 * NV);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(v[k]);
+            tag_st(w + 2 * k, ((unsigned long long)tag << 32) | (bits & 0xffffffffULL));
+            tag_st(w + 2 * k + 1, ((unsigned long long)tag << 32) | (bits >> 32));
+        }
+    }
+    if (!fin) return;  // uniform over the workgroup
+    __syncthreads();   // red[] is reused below
+    // k_b_finish's sum over the tagged words, as a 256-thread tree: thread
+    // 64 w + l adds tiles t0 + 64 w + l + 256 j in order, each wave's sums go
+    // through the xor butterfly, the four wave sums are added in order
+    const int tid = threadIdx.x;
+    const int t0 = B.sys_t0[s], t1 = B.sys_t1[s];
+    constexpr int U = 2;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    const long long t_start = (long long)__builtin_amdgcn_s_memrealtime();
+    int stalled = 0;
+    for (int j0 = t0 + tid; tid < kBlock && j0 < t1; j0 += U * kBlock) {  // threads >= 256 add nothing
+        unsigned long long wd[U][NV][2];
+        for (;;) {
+            bool ok = true;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = j0 + u * kBlock;
+                const unsigned long long* q = B.tagp + ((size_t)phase * B.T + t) * (2 * NV);
+#pragma unroll
+                for (int k = 0; k < NV; ++k)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        wd[u][k][h] = t < t1 ? tag_ld(q + 2 * k + h) : ((unsigned long long)tag << 32);
+                        ok = ok && (unsigned)(wd[u][k][h] >> 32) == tag;
+                    }
+            }
+            if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
+            if ((long long)__builtin_amdgcn_s_memrealtime() - t_start > kFinishWaitTicks) {
+                stalled = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (stalled) break;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                acc[k] += __longlong_as_double((long long)((wd[u][k][0] & 0xffffffffULL) | (wd[u][k][1] << 32)));
+    }
+    __shared__ double wsum[NV * (kBlock / 64)];
+    __shared__ int wstall[kBlock / 64];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
+    if ((tid & 63) == 0 && tid < kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) wsum[k * (kBlock / 64) + (tid >> 6)] = acc[k];
+        wstall[tid >> 6] = stalled;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        BState S = B.st[s];
+        double tot[NV];
+        int st = 0;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double sum = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < kBlock / 64; ++ww) sum += wsum[k * (kBlock / 64) + ww];
+            tot[k] = sum;
+        }
+#pragma unroll
+        for (int ww = 0; ww < kBlock / 64; ++ww) st |= wstall[ww];
+        if (st) {
+            S.halt = 1;
+            S.reason = BR_STALLED;
+        } else {
+            op(S, tot, B);
+        }
+        S.seq += 1;
+        B.st[s] = S;
+    }
+}
+
+// ---- K1 / F1 -------------------------------------------------------------------------------
+// NORMAL / FLUSHED: p_new = D^-1 r + beta p_old into the other p buffer, and
+// q = A p_new with p.q partials.  The p update of cg_solver.cpp:173-176 is
+// evaluated on the fly for every column the tile's rows touch (the same
+// expression on the same inputs as the row's owner, so bit-identical), which
+// removes a pass over r and p per iteration.  Stencil-stored systems do not
+// store q: K2 recomputes it from p_new.
+// FLUSH: FP64 true residual r = b - A (x64 + x_lo).
+template <class T, bool MIXED>
+__global__ void __launch_bounds__(kBlock)
+k_b_k1(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
+       const double* __restrict__ b, const double* __restrict__ x64, const float* __restrict__ xlo) {
+    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles
+    // every access to the staging buffer goes through address-space-3 pointers
+    const lds_ptr<double> prod64 = (lds_ptr<double>)stage_lds;
+    const TileVec<T>& D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    if (D.vec) return;  // k_b_k1v runs it
+    const int tile = D.tile;
+    const int s = B.tile_sys[tile];
+    const BState* S0 = &B.st[s];
+    if (S0->halt) return;
+    const int flag = S0->flag;
+    const unsigned tag = (unsigned)S0->seq + 1u;
+    const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
+    const bool cdok = B.cd_ok[s] != 0;
+    const T cdv = Prec<T>::cd(B)[s];
+    const T* __restrict__ dinv = Prec<T>::dinv(B);
+    double red[2] = {0.0, 0.0};
+    if (flag == BS_FLUSH) {
+        auto xv = [&](int j) { return MIXED ? x64[j] + (double)xlo[j] : x64[j]; };
+        tile_rows64(B, s, r0, r1, xv, prod64, [&](int i, double sum) {
+            const double rh = b[i] - sum;
+            const T ri = (T)rh;
+            r[i] = ri;
+            red[0] += rh * rh;
+            red[1] += (double)ri * (double)(T)((cdok ? cdv : dinv[i]) * ri);
+        });
+        tile_finish<2, kBlock>(red, B, tile, s, 0, tag, D.fin, OpF1{});
+        return;
+    }
+    const double beta = S0->beta;
+    const T* __restrict__ po = S0->pcur ? p1 : p0;
+    T* __restrict__ pn = S0->pcur ? p0 : p1;
+    auto pnew = [&](int j) -> T {
+        const T dj = cdok ? cdv : dinv[j];
+        return (T)((double)(T)(dj * r[j]) + beta * (double)po[j]);
+    };
+    const int kind = B.dia_kind[s];
+    if (kind != SK_CSR) {
+        const Stencil<T> A(B, s);
+        const lds_ptr<T> lds = (lds_ptr<T>)prod64;
+        const int cap = B.stage_doubles * (int)(sizeof(double) / sizeof(T));
+        auto out = [&](int i, T pi, T sum) {
+            if (kind == SK_DIA) q[i] = sum;
+            red[0] += (double)pi * (double)sum;
+        };
+        if (stage_need(A.halo, (int)sizeof(T)) <= cap) {  // p_new staged in LDS, evaluated once per row
+            A.dispatch([&](auto nd) {
+                stencil_staged<nd()>(A, r0, r1, lds, cap, pnew, [&](int j, T v) { pn[j] = v; }, out);
+            });
+        } else {  // halo wider than the staging buffer: rows straight from memory
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+                const T pi = pnew(i);
+                pn[i] = pi;
+                out(i, pi, A.template row<0>(i, pnew));
+            }
+        }
+    } else {
+        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) pn[i] = pnew(i);
+        tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), pnew, (lds_ptr<T>)prod64, [&](int i, T sum) {
+            q[i] = sum;
+            red[0] += (double)pnew(i) * (double)sum;
+        });
+    }
+    tile_finish<2, kBlock>(red, B, tile, s, 0, tag, D.fin, OpF1{});
+}
+
 // ---- K2 / F2 -------------------------------------------------------------------------------
 // NORMAL: x += alpha p, r -= alpha q (q recomputed from p for stencil-stored
 // systems), r.r / r.z partials.  FLUSHED: x64 += x_lo, x_lo = 0.
@@ -914,6 +1073,7 @@ k_b_k2(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __r
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
     const int flag = S0->flag;
+    const unsigned tag = (unsigned)S0->seq + 1u;
     const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
     if (flag == BS_FLUSHED) {
         if (MIXED)
@@ -952,7 +1112,7 @@ k_b_k2(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __r
     } else {
         for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], q[i]);
     }
-    tile_partial<2>(red, B, tile);
+    tile_finish<2, kBlock>(red, B, tile, s, 1, tag, D.fin, OpF2<MIXED>{});
 }
 
 // ---- K1 / K2 of the 16-byte five-point path ------------------------------------------------
@@ -990,7 +1150,7 @@ k_b_k1v(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ r, cons
         k1_stencil_vec<T, BS>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, r, h.pcur ? p1 : p0, h.pcur ? p0 : p1,
                        h.beta, red[0]);
     }
-    tile_partial<2, BS>(red, B, tile);
+    tile_finish<2, BS>(red, B, tile, D.sys, 0, (unsigned)h.seq + 1u, D.fin, OpF1{});
 }
 
 // K2: NORMAL as k_b_k2's stencil path; FLUSHED: x64 += x_lo, x_lo = 0.
@@ -1016,42 +1176,8 @@ k_b_k2v(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, T* __restri
     double red[2] = {0.0, 0.0};
     k2_stencil_vec<T, MIXED, BS>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, h.pcur ? p1 : p0, r, x64, xlo,
                              h.alpha, red);
-    tile_partial<2, BS>(red, B, tile);
+    tile_finish<2, BS>(red, B, tile, D.sys, 1, (unsigned)h.seq + 1u, D.fin, OpF2<MIXED>{});
 }
-
-template <bool MIXED>
-struct OpF2 {
-    __device__ bool active(const BState& S) const { return !S.halt && S.flag == BS_NORMAL; }
-    __device__ void operator()(BState& S, const double* tot, const BArgs& B) const {
-        const double rn = sqrt(tot[0]);
-        S.rel = rn / S.bnorm;
-        S.iter += 1;
-        if (S.rel < B.tol) {  // cg_solver.cpp:143, confirmed on the FP64 true residual
-            S.flag = BS_FLUSH;
-            S.reason = BR_CHECK;
-            return;
-        }
-        if (MIXED) {
-            S.rmax = fmax(S.rmax, rn);
-            if (rn < B.delta * S.rmax) {
-                S.flag = BS_FLUSH;
-                S.reason = BR_RELIABLE;
-                return;
-            }
-        }
-        S.rho_old = S.rho;
-        S.rho = tot[1];
-        if (fabs(S.rho_old) < kEps * 1e8 * fabs(S.rho)) {  // cg_solver.cpp:157-167: restart
-            S.restarts += 1;
-            S.flag = BS_FLUSH;
-            S.reason = BR_RESTART;
-            if (S.iter >= B.max_iter) S.halt = 1;
-            return;
-        }
-        S.beta = S.rho / S.rho_old;
-        if (S.iter >= B.max_iter) S.halt = 1;
-    }
-};
 
 // ---- final: x64 += x_lo and the FP64 true residual per (large) system ----------------------
 __global__ void __launch_bounds__(kBlock)
@@ -1296,11 +1422,11 @@ struct Batch {
     int T = 0;
     DevBuf tile_r0, tile_r1, tile_sys, sys_t0, sys_t1, tv32, tv64;
     // active-tile compaction: host copies of the records, the compacted device
-    // copies K1 / K2 read, the active systems F1 / F2 read, and two pinned
-    // staging buffers (an upload never overwrites the one still being copied)
+    // copies K1 / K2 read, and two pinned staging buffers (an upload never
+    // overwrites the one still being copied)
     std::vector<TileVec<float>> h_tv32;
     std::vector<TileVec<double>> h_tv64;
-    DevBuf tva32, tva64, act_sys;
+    DevBuf tva32, tva64;
     void* pin_act[2] = {nullptr, nullptr};
     size_t pin_act_bytes = 0;
     int pin_next = 0;
@@ -1308,6 +1434,7 @@ struct Batch {
     std::vector<std::pair<int, int>> waves;  // internal system ranges within [0, n_large)
     // per-solve work (reused)
     DevBuf st, partials, r64, r, p, p1, q, xlo;
+    DevBuf tagp;  // tagged partials of the iteration kernels
     int work_prec = -1;
     // a captured chunk replays the kernel arguments it was captured with
     // (tolerance, iteration cap, tiles, ...), so the whole argument block is
@@ -1411,9 +1538,8 @@ static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     up(B.tile_sys, sys);
     build_tile_records<float>(B, r0, r1, sys, B.tv32);
     build_tile_records<double>(B, r0, r1, sys, B.tv64);
-    B.act_sys = DevBuf(B.ctx, sizeof(int) * (size_t)std::max(B.S, 1));
     static_assert(sizeof(TileVec<float>) == sizeof(TileVec<double>), "one staging size for both precisions");
-    const size_t pb = sizeof(TileVec<double>) * (size_t)std::max(B.T, 1) + sizeof(int) * (size_t)std::max(B.S, 1);
+    const size_t pb = sizeof(TileVec<double>) * (size_t)std::max(B.T, 1);
     if (pb > B.pin_act_bytes) {
         for (void*& q : B.pin_act) {
             if (q) HSPS_HIP(hipHostFree(q));
@@ -1426,6 +1552,7 @@ static void build_tiles(Batch& B, int tile_rows, int tile_rows_dia) {
     up(B.sys_t0, B.h_sys_t0);
     up(B.sys_t1, B.h_sys_t1);
     B.partials = DevBuf(B.ctx, sizeof(double) * 2 * (size_t)std::max(B.T, 1));
+    B.tagp = DevBuf(B.ctx, sizeof(unsigned long long) * 8 * (size_t)std::max(B.T, 1));
     B.tile_rows = tile_rows;
     B.tile_rows_dia = tile_rows_dia;
     B.drop_graphs();
@@ -1627,7 +1754,6 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     a.tv64 = B.tv64.as<TileVec<double>>();
     a.tva32 = B.tva32.as<TileVec<float>>();
     a.tva64 = B.tva64.as<TileVec<double>>();
-    a.act_sys = B.act_sys.as<int>();
     a.rp = B.rp.as<int>();
     a.ci = B.ci.as<int>();
     a.v64 = B.v64.as<double>();
@@ -1656,6 +1782,7 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
     a.stage_doubles = B.stage_doubles[p.precision == HSPS_MIXED ? 1 : 0];
     a.st = B.st.as<BState>();
     a.partials = B.partials.as<double>();
+    a.tagp = B.tagp.as<unsigned long long>();
     a.T = B.T;
     a.tol = p.tol;
     a.delta = p.reliable_delta > 0 ? p.reliable_delta : 0.1;
@@ -1664,7 +1791,7 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
 }
 
 // The active set of a wave: the tiles K1 / K2 run (their records, compacted)
-// and the systems F1 / F2 run.  kinds: bit 0 / bit 2 = some tile takes the
+// and the number of their systems.  kinds: bit 0 / bit 2 = some tile takes the
 // narrow / wide five-point kernels, bit 1 = some tile takes the general ones.
 struct Active {
     int nt = 0, ns = 0, kinds = 0;
@@ -1676,8 +1803,29 @@ static const std::vector<TileVec<T>>& host_records(const Batch& B) {
     else return B.h_tv64;
 }
 
+// The dispatcher starts a launch's workgroups in block order and block b
+// takes active record xcd_tile(b): mark, per system, the record of its
+// last-started block (the one that runs the fused finish).
+static int xcd_tile_host(int b, int nt) {
+    constexpr int span = kXcds * kXcdRun;
+    if (b >= nt / span * span) return b;
+    const int x = b % kXcds, slot = b / kXcds;
+    return ((slot / kXcdRun) * kXcds + x) * kXcdRun + slot % kXcdRun;
+}
+template <class T>
+static void mark_finishers(TileVec<T>* rec, int nt, int n_sys) {
+    std::vector<int> last((size_t)n_sys, -1);  // system -> record of its last-started block
+    for (int b = 0; b < nt; ++b) {
+        const int p = xcd_tile_host(b, nt);
+        rec[p].fin = 0;
+        last[rec[p].sys] = p;
+    }
+    for (int p : last)
+        if (p >= 0) rec[p].fin = 1;
+}
+
 // Stages the records of the tiles of the live systems of [k0, k1) (live =
-// nullptr: all of them) and the systems themselves, and copies both to the
+// nullptr: all of them), marks each system's finisher, and copies them to the
 // device on stream s behind the work already queued there.
 template <class T>
 static Active upload_active(Batch& B, int k0, int k1, const std::vector<char>* live, hipStream_t s) {
@@ -1685,20 +1833,19 @@ static Active upload_active(Batch& B, int k0, int k1, const std::vector<char>* l
     char* base = (char*)B.pin_act[B.pin_next];
     B.pin_next ^= 1;
     TileVec<T>* rec = (TileVec<T>*)base;
-    int* sys = (int*)(base + sizeof(TileVec<T>) * (size_t)std::max(B.T, 1));
     Active A;
     for (int k = k0; k < k1; ++k) {
         if (live && !(*live)[k - k0]) continue;
-        sys[A.ns++] = k;
+        ++A.ns;
         for (int t = B.h_sys_t0[k]; t < B.h_sys_t1[k]; ++t) {
             rec[A.nt++] = tv[t];
             A.kinds |= tv[t].vec == 1 ? 1 : tv[t].vec == 2 ? 4 : 2;
         }
     }
+    mark_finishers(rec, A.nt, B.S);
     if (A.nt > 0) {
         HSPS_HIP(hipMemcpyAsync((sizeof(T) == 4 ? B.tva32 : B.tva64).p, rec, sizeof(TileVec<T>) * (size_t)A.nt,
                                 hipMemcpyHostToDevice, s));
-        HSPS_HIP(hipMemcpyAsync(B.act_sys.p, sys, sizeof(int) * (size_t)A.ns, hipMemcpyHostToDevice, s));
     }
     return A;
 }
@@ -1711,7 +1858,7 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
     T* p1 = B.p1.as<T>();
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
-    const int nt = act.nt, ns = act.ns;
+    const int nt = act.nt;
     const bool narrow = act.kinds & 1, any_gen = act.kinds & 2, wide = act.kinds & 4;
     for (int it = 0; it < iters; ++it) {
         const size_t lds = sizeof(double) * (size_t)a.stage_doubles;
@@ -1724,7 +1871,6 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
         if (any_gen)
             hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, q, r, b,
                                (const double*)x, (const float*)xl);
-        hipLaunchKernelGGL((k_b_finish<2, OpF1>), dim3(ns), dim3(kFinishThreads), 0, s, a, 0, a.act_sys, OpF1{});
         if (narrow)
             hipLaunchKernelGGL((k_b_k2v<T, MIXED, kVecNarrow>), dim3(nt), dim3(kVecNarrow), lds, s, a, (const T*)p0,
                                (const T*)p1, r, x, xl);
@@ -1734,8 +1880,6 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
         if (any_gen)
             hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, (const T*)p0, (const T*)p1,
                                (const T*)q, r, x, xl);
-        hipLaunchKernelGGL((k_b_finish<2, OpF2<MIXED>>), dim3(ns), dim3(kFinishThreads), 0, s, a, 0, a.act_sys,
-                           OpF2<MIXED>{});
     }
     HSPS_CHECK_LAUNCH();
 }
@@ -2304,11 +2448,11 @@ static void session_alloc(Session& S, Ctx* ctx, int max_systems, long long max_r
     B.tv64 = DevBuf(ctx, sizeof(TileVec<double>) * Tn);
     B.tva32 = DevBuf(ctx, sizeof(TileVec<float>) * Tn);
     B.tva64 = DevBuf(ctx, sizeof(TileVec<double>) * Tn);
-    B.act_sys = DevBuf(ctx, sizeof(int) * Sn);
     B.partials = DevBuf(ctx, sizeof(double) * 2 * Tn);
+    B.tagp = DevBuf(ctx, sizeof(unsigned long long) * 8 * Tn);
     B.h_tv32.assign(Tn, TileVec<float>{});
     B.h_tv64.assign(Tn, TileVec<double>{});
-    const size_t pb = sizeof(TileVec<double>) * Tn + sizeof(int) * Sn;
+    const size_t pb = sizeof(TileVec<double>) * Tn;
     for (void*& q : B.pin_act) HSPS_HIP(hipHostMalloc(&q, pb, hipHostMallocDefault));
     B.pin_act_bytes = pb;
     ensure_work(B, S.prec);
@@ -2512,8 +2656,9 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
     return slot;
 }
 
-// Stages the records of the live systems' tiles and the systems themselves
-// and copies both behind the work on the stream (pinned double buffer).
+// Stages the records of the live systems' tiles, marks each system's
+// finisher, and copies them behind the work on the stream (pinned double
+// buffer).
 template <class T>
 static void session_upload_active(Session& S) {
     Batch& B = S.B;
@@ -2521,24 +2666,23 @@ static void session_upload_active(Session& S) {
     char* base = (char*)B.pin_act[B.pin_next];
     B.pin_next ^= 1;
     TileVec<T>* rec = (TileVec<T>*)base;
-    int* sys = (int*)(base + sizeof(TileVec<T>) * (size_t)B.T);
     Active A;
     int need = kStageMinDoubles;
     for (int k = 0; k < (int)S.slots.size(); ++k) {
         const SessSlot& sl = S.slots[k];
         if (!sl.used || !sl.live) continue;
-        sys[A.ns++] = k;
+        ++A.ns;
         need = std::max(need, sl.need[sizeof(T) == 4 ? 1 : 0]);
         for (int t = sl.t0; t < sl.t1; ++t) {
             rec[A.nt++] = tv[t];
             A.kinds |= tv[t].vec == 1 ? 1 : tv[t].vec == 2 ? 4 : 2;
         }
     }
+    mark_finishers(rec, A.nt, (int)S.slots.size());
     hipStream_t s = B.ctx->stream;
     if (A.nt > 0) {
         HSPS_HIP(hipMemcpyAsync((sizeof(T) == 4 ? B.tva32 : B.tva64).p, rec, sizeof(TileVec<T>) * (size_t)A.nt,
                                 hipMemcpyHostToDevice, s));
-        HSPS_HIP(hipMemcpyAsync(B.act_sys.p, sys, sizeof(int) * (size_t)A.ns, hipMemcpyHostToDevice, s));
     }
     S.act = A;
     S.stage = need;
