@@ -22,8 +22,10 @@
 //   * dense window  (B has <= kDenseCols columns, or a row too wide for the
 //     hash): acc indexed by column; the row is emitted by scanning the window
 //     in column order;
-//   * hash          (otherwise): open-addressing table of 512 or 2048 slots,
-//     by the row's product count; kept entries are compacted and ranked.
+//   * hash          (otherwise): open-addressing table of 512 slots (rows of
+//     <= 256 products) or 1024, a row that overflows it retried in 2048 slots;
+//     kept entries are compacted and ranked.  The B-row loads of the next few
+//     steps are in flight while one is accumulated.
 // A count pass sizes each row (drop rule applied), a scan gives row pointers,
 // and a fill pass recomputes the same accumulation and writes.
 // =============================================================================
@@ -47,15 +49,20 @@ constexpr double kEpsD = std::numeric_limits<double>::epsilon();
 constexpr double kDropP = 1e-14;              // amg.cpp:160
 constexpr double kDropGalerkin = kEpsD * kEpsD;  // sparse_matrix.cpp:209
 constexpr int kDenseCols = 8192;              // dense LDS window: one FP64 accumulator per column (64 KB)
-constexpr int kPShort = 256;                  // P rows from A rows up to this length: one thread each
+constexpr int kPThread = 12;                  // P rows from A rows up to this length: one thread each
+constexpr int kPShort = 256;                  // longer ones up to this length: one wave each (k_p_wave)
+constexpr int kStrongThread = 12;             // strength rows: one thread each up to this average length
 constexpr int kPWindow = kDenseCols - 128;    // long P rows: window + 64-entry staging in 64 KB
-constexpr int kHashSmall = 512;               // rows with <= 256 products
-constexpr int kHashLarge = 2048;              // rows with more (<= 1024 distinct columns; 36 KB)
+constexpr int kHashSmall = 512;               // rows with <= 256 products (never overflows)
+constexpr int kHashMid = 1024;                // rows with more (<= 512 distinct columns; 20 KB)
+constexpr int kHashLarge = 2048;              // rows the mid table overflows (<= 1024 distinct; 40 KB)
 constexpr int kSmallProducts = kHashSmall / 2;
+constexpr int kGemmAhead = 8;                 // B-row loads in flight ahead of the accumulation
 constexpr int kMaxWaves = 8192;               // persistent one-wave workgroups per launch
 
 struct CsrRef {
     int rows = 0, cols = 0;
+    long long nnz = 0;
     const int* rp = nullptr;
     const int* ci = nullptr;
     const double* v = nullptr;
@@ -112,24 +119,68 @@ __global__ void k_strong(int n, const int* __restrict__ rp, const int* __restric
     }
 }
 
+// The same, one wave per row (levels whose rows average more than
+// kStrongThread entries): the row maximum is a wave reduction (max is exact
+// in any order) and the strong columns are written in row order by ballot.
+__global__ __launch_bounds__(64) void k_strong_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                    const double* __restrict__ v, double theta,
+                                                    int* __restrict__ cnt, const int* __restrict__ srp,
+                                                    int* __restrict__ sci) {
+    const int lane = threadIdx.x;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int k0 = rp[i], k1 = rp[i + 1];
+        double m = 0.0;
+        for (int k = k0 + lane; k < k1; k += 64)
+            if (ci[k] != i) {
+                const double a = fabs(v[k]);
+                m = m < a ? a : m;
+            }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(m, off);
+            m = m < o ? o : m;
+        }
+        const double thr = theta * m;
+        int o = srp ? srp[i] : 0;
+        for (int kb = k0; kb < k1; kb += 64) {
+            const int k = kb + lane;
+            int j = -1;
+            bool strong = false;
+            if (k < k1) {
+                j = ci[k];
+                strong = j != i && fabs(v[k]) >= thr;
+            }
+            const unsigned long long b = __ballot(strong);
+            if (srp && strong) sci[o + __popcll(b & lanes_below(lane))] = j;
+            o += __popcll(b);
+        }
+        if (!srp && lane == 0) cnt[i] = o;
+    }
+}
+
 // ---- smoothed prolongation (amg.cpp:101-178) -----------------------------------
 // Row i of P: entry (agg_i, 1) minus, for each off-diagonal a_ij in row order,
 // t = (omega / a_ii) * a_ij at column agg_j (nothing when |omega / a_ii| <
 // eps); entries with |v| <= 1e-14 are dropped, columns ascending.
 //
-// Rows of A up to kPShort entries: one thread each; the row's list lives in
+// Rows of A up to kPThread entries: one thread each; the row's list lives in
 // scratch at rp[i] + i (at most row length + 1 entries) and is sorted by
-// column at the end.  Longer rows: k_p_long.
+// column at the end.  Rows up to kPShort entries: one wave each (k_p_wave),
+// same scratch layout.  Longer rows, and wave rows with more than 64 distinct
+// aggregate columns: k_p_long.
 __device__ __forceinline__ double p_weight(int i, int k0, int k1, const int* ci, const double* v, double omega) {
     for (int k = k0; k < k1; ++k)
         if (ci[k] == i) return fabs(v[k]) > kEpsD ? omega / v[k] : 0.0;
     return 0.0;
 }
 
-__global__ void k_p_classify(int n, const int* __restrict__ rp, int* __restrict__ list, int* __restrict__ n_list) {
+__global__ void k_p_classify(int n, const int* __restrict__ rp, int* __restrict__ list, int* __restrict__ n_list,
+                             int* __restrict__ wlist, int* __restrict__ n_wlist) {
     for (int b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
         const int i = b + threadIdx.x;
-        wave_append(i < n && rp[i + 1] - rp[i] > kPShort, i, list, n_list);
+        const int L = i < n ? rp[i + 1] - rp[i] : 0;
+        wave_append(i < n && L > kPShort, i, list, n_list);
+        wave_append(i < n && L > kPThread && L <= kPShort, i, wlist, n_wlist);
     }
 }
 
@@ -139,7 +190,7 @@ __global__ void k_p_rows(int n, const int* __restrict__ rp, const int* __restric
 #pragma clang fp contract(off)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int k0 = rp[i], k1 = rp[i + 1];
-        if (k1 - k0 > kPShort) continue;
+        if (k1 - k0 > kPThread) continue;
         int* c_ = lc + k0 + i;
         double* v_ = lv + k0 + i;
         int L = 1;
@@ -178,6 +229,99 @@ __global__ void k_p_rows(int n, const int* __restrict__ rp, const int* __restric
         for (int q = 0; q < L; ++q) kept += fabs(v_[q]) > kDropP;
         len[i] = L;
         cnt[i] = kept;
+    }
+}
+
+// One wave per row.  Lane q owns the row's q-th distinct column (in order of
+// first appearance, the reference's list order; lane 0: (agg_i, 1)) and its
+// value.  For each 64-entry chunk the entries' (column, t) are broadcast in
+// row order and each owner subtracts the t of its column: the same
+// subtractions in the same order per column as the reference's list.  Owners
+// are then ranked by column.  A row with more than 64 distinct columns is
+// appended to `longs` (len 0) for k_p_long.
+__global__ __launch_bounds__(64) void k_p_wave(const int* __restrict__ rows, int n_rows, const int* __restrict__ rp,
+                                               const int* __restrict__ ci, const double* __restrict__ v,
+                                               const int* __restrict__ agg, double omega, int* __restrict__ lc,
+                                               double* __restrict__ lv, int* __restrict__ len, int* __restrict__ cnt,
+                                               int* __restrict__ longs, int* __restrict__ n_long) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x;
+    for (int r = blockIdx.x; r < n_rows; r += gridDim.x) {
+        const int i = rows[r];
+        const int k0 = rp[i], k1 = rp[i + 1];
+        // the first diagonal entry in row order
+        double w = 0.0;
+        for (int kb = k0; kb < k1; kb += 64) {
+            const int k = kb + lane;
+            const unsigned long long m = __ballot(k < k1 && ci[k] == i);
+            if (m) {
+                const int kd = kb + __ffsll((long long)m) - 1;
+                w = fabs(v[kd]) > kEpsD ? omega / v[kd] : 0.0;
+                break;
+            }
+        }
+        int ucol = lane == 0 ? agg[i] : -1;
+        double acc = lane == 0 ? 1.0 : 0.0;
+        int nu = 1;
+        bool over = false;
+        if (!(fabs(w) < kEpsD)) {
+            for (int kb = k0; kb < k1 && !over; kb += 64) {
+                const int k = kb + lane;
+                int c = -1;
+                double t = 0.0;
+                if (k < k1) {
+                    const int j = ci[k];
+                    if (j != i) {
+                        c = agg[j];
+                        t = w * v[k];
+                    }
+                }
+                const int nk = min(64, k1 - kb);
+                // new columns: not owned yet and first in this chunk
+                bool fresh = c >= 0;
+                for (int q = 0; q < 64; ++q) {
+                    const int oc = __builtin_amdgcn_readlane(ucol, q);
+                    const int cc = __builtin_amdgcn_readlane(c, q);
+                    if (oc == c || (q < lane && cc == c)) fresh = false;
+                }
+                unsigned long long fm = __ballot(fresh);
+                if (nu + __popcll(fm) > 64) {
+                    over = true;
+                    break;
+                }
+                while (fm) {
+                    const int e = __ffsll((long long)fm) - 1;
+                    fm &= fm - 1;
+                    const int ce = __builtin_amdgcn_readlane(c, e);
+                    if (lane == nu) ucol = ce;
+                    ++nu;
+                }
+                for (int e = 0; e < nk; ++e) {
+                    const int ce = __builtin_amdgcn_readlane(c, e);
+                    const double te = bcast(t, e);
+                    if (ce >= 0 && ucol == ce) acc = acc - te;
+                }
+            }
+        }
+        if (over) {
+            if (lane == 0) {
+                len[i] = 0;
+                longs[atomicAdd(n_long, 1)] = i;
+            }
+            continue;
+        }
+        int rank = 0;
+        for (int q = 0; q < nu; ++q) rank += __builtin_amdgcn_readlane(ucol, q) < ucol;
+        const bool own = lane < nu;
+        const unsigned long long km = __ballot(own && fabs(acc) > kDropP);
+        if (own) {
+            lc[k0 + i + rank] = ucol;
+            lv[k0 + i + rank] = acc;
+        }
+        if (lane == 0) {
+            len[i] = nu;
+            cnt[i] = __popcll(km);
+        }
     }
 }
 
@@ -310,59 +454,182 @@ __global__ void k_t_gather(int nnz, const int* __restrict__ order, const int* __
 }
 
 // ---- sparse product C = A B (sparse_matrix.cpp:171-237) --------------------------
-// Product count of each row (clamped to INT_MAX), rows split into the small-
-// and large-hash lists.
-__global__ void k_gemm_classify(CsrRef A, const int* __restrict__ brp, int* __restrict__ list_small,
-                                int* __restrict__ list_large, int* __restrict__ n_lists) {
+// Rows travel as RowRef {i, A.rp[i], A.rp[i + 1], 0} (one 16-byte load), and
+// each A entry p carries its B row's bounds bb[p] = {B.rp[k], B.rp[k + 1]}
+// (k = A.ci[p]), so a row's first loads are two independent ones instead of a
+// chain of four.
+__global__ void k_gemm_bounds(long long nnz, const int* __restrict__ aci, const int* __restrict__ brp,
+                              int2* __restrict__ bb) {
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nnz;
+         p += (long long)gridDim.x * blockDim.x) {
+        const int k = aci[p];
+        bb[p] = make_int2(brp[k], brp[k + 1]);
+    }
+}
+
+// Appends r to list where pred holds: one atomic per wave (wave_append for
+// RowRefs).
+__device__ __forceinline__ void wave_append_ref(bool pred, int4 r, int4* list, int* n) {
+    const unsigned long long m = __ballot(pred);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(n, __popcll(m));
+    base = __shfl(base, leader);
+    if (pred) list[base + __popcll(m & lanes_below(lane))] = r;
+}
+
+// Product count of each row, rows split into the small- and large-hash lists.
+__global__ void k_gemm_classify(CsrRef A, const int2* __restrict__ bb, int4* __restrict__ list_small,
+                                int4* __restrict__ list_large, int* __restrict__ n_lists) {
     for (int b = blockIdx.x * blockDim.x; b < A.rows; b += gridDim.x * blockDim.x) {
         const int i = b + threadIdx.x;
         const bool valid = i < A.rows;
         long long ub = 0;
-        if (valid)
-            for (int pa = A.rp[i]; pa < A.rp[i + 1]; ++pa) {
-                const int k = A.ci[pa];
-                ub += brp[k + 1] - brp[k];
+        int a0 = 0, a1 = 0;
+        if (valid) {
+            a0 = A.rp[i];
+            a1 = A.rp[i + 1];
+            for (int pa = a0; pa < a1; ++pa) {
+                const int2 e = bb[pa];
+                ub += e.y - e.x;
             }
-        wave_append(valid && ub <= kSmallProducts, i, list_small, &n_lists[0]);
-        wave_append(valid && ub > kSmallProducts, i, list_large, &n_lists[1]);
+        }
+        const int4 r = make_int4(i, a0, a1, 0);
+        wave_append_ref(valid && ub <= kSmallProducts, r, list_small, &n_lists[0]);
+        wave_append_ref(valid && ub > kSmallProducts, r, list_large, &n_lists[1]);
+    }
+}
+
+// A 64-entry chunk of A's row starting at p0 (row end a1): lane e holds entry
+// p0 + e's value and B-row bounds.  Every lane loads (a lane past the row
+// loads entry 0, A having entries, and is never read), so no load sits under
+// a branch.
+__device__ __forceinline__ void load_chunk(const CsrRef& A, const int2* bb, int p0, int a1, double& av, int2& b) {
+    const int p = p0 + (int)(threadIdx.x & 63);
+    const int pc = p < a1 ? p : 0;
+    av = A.v[pc];
+    b = bb[pc];
+}
+
+// The rows of the list, one wave each (r = blockIdx.x, += gridDim.x).  While a
+// row runs, the next row's RowRef and first A chunk and the RowRef after it
+// are in flight.  body(row, av, b) gets the row's first chunk.
+template <class Body>
+__device__ __forceinline__ void for_each_row(const int4* rows, int n_rows, const CsrRef& A, const int2* bb,
+                                             Body body) {
+    int r = blockIdx.x;
+    if (r >= n_rows) return;
+    const int g = gridDim.x;
+    int4 cur = rows[r];
+    double av;
+    int2 b;
+    load_chunk(A, bb, cur.y, cur.z, av, b);
+    int4 nxt = rows[min(r + g, n_rows - 1)];
+    for (; r < n_rows; r += g) {
+        double nav;
+        int2 nb;
+        load_chunk(A, bb, nxt.y, nxt.z, nav, nb);
+        const int4 nxt2 = rows[min(r + 2 * g, n_rows - 1)];
+        body(cur, av, b);
+        cur = nxt;
+        av = nav;
+        b = nb;
+        nxt = nxt2;
+    }
+}
+
+// The steps of one 64-entry chunk of A's row i (lane e holds entry e: its
+// B-row bounds [b0, b1)): a step is one 64-entry run of one B row, in A's
+// order, runs in B's order; an empty B row is one empty step.  f(j, l, v)
+// consumes a step in every lane: j = the chunk entry, (l, v) this lane's B
+// entry (l = -1 past the run).  The B entries of the next D steps are loaded
+// while one is consumed, so the consumption order is the product order and
+// global-load latency overlaps the accumulation.  Every lane loads in every
+// step (a lane past its run, or a step past the chunk, loads B entry 0 and
+// ignores it), so no load sits under a branch and the compiler's wait counts
+// stay exact.  Stops early once stop is set (by f).
+template <int D, class F>
+__device__ __forceinline__ void for_each_step(const CsrRef& B, int b0, int b1, int na, const bool& stop, F f) {
+    constexpr int safe = 0;  // callers launch only when B has entries
+    const int lane = threadIdx.x & 63;
+    int pj = 0, pp = bcast(b0, 0), pe = bcast(b1, 0);  // producer cursor (wave-uniform)
+    int rl[D], rj[D];
+    double rv[D];
+    bool rin[D];
+    auto issue = [&](int d) {
+        const bool live = pj < na;
+        const int p = pp + lane;
+        rin[d] = live && p < pe;
+        const int pc = rin[d] ? p : safe;
+        rl[d] = B.ci[pc];
+        rv[d] = B.v[pc];
+        rj[d] = live ? pj : -1;
+        if (live) {
+            pp += 64;
+            if (pp >= pe && ++pj < na) {
+                pp = bcast(b0, pj);
+                pe = bcast(b1, pj);
+            }
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d) issue(d);
+    while (rj[0] >= 0 && !stop) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (rj[d] >= 0 && !stop) f(rj[d], rin[d] ? rl[d] : -1, rv[d]);
+            issue(d);
+        }
+    }
+}
+
+// The chunks of a row (the first one preloaded), each through for_each_step.
+template <class F>
+__device__ __forceinline__ void for_each_product(const CsrRef& A, const int2* bb, const CsrRef& B, int4 row,
+                                                 double av, int2 b, const bool& stop, F f) {
+    const int a1 = row.z;
+    for (int base = row.y; base < a1 && !stop; base += 64) {
+        if (base != row.y) load_chunk(A, bb, base, a1, av, b);
+        const int na = min(64, a1 - base);
+        for_each_step<kGemmAhead>(B, b.x, b.y, na, stop, [&](int j, int l, double v) { f(bcast(av, j), l, v); });
     }
 }
 
 // One wave per row, open-addressing hash of CAP slots in LDS.  PASS 0 writes
 // the kept count (rows whose distinct columns exceed CAP / 2 are appended to
-// `spill` and left to the dense-window kernel); PASS 1 writes the row at
-// crp[i] in ascending columns.  Occupancy is checked after every step of <= 64
-// insertions, so the table never exceeds CAP / 2 + 64 slots and probing ends.
-// Occupied slots are listed as they fill, so emitting and clearing a row
-// visits only those.
-//
-// The wave loads 64 entries of A's row at a time (column, value and B-row
-// bounds, one per lane) and broadcasts them in order; the first 64 entries
-// of the next B row are loaded before the current one is accumulated.
+// `spill` and left to the next larger table or the dense-window kernel); PASS 1
+// writes the row at crp[i] in ascending columns (and skips rows that spilled).
+// Occupancy is checked after every step of <= 64 insertions, so the table
+// never exceeds CAP / 2 + 64 slots and probing ends.  Occupied slots are
+// listed as they fill, so emitting and clearing a row visits only those.
+// Products are accumulated in the row's order (for_each_product).
 template <int CAP, int PASS>
-__global__ __launch_bounds__(64) void k_gemm_hash(const int* __restrict__ rows, int n_rows, CsrRef A, CsrRef B,
-                                                  int* __restrict__ cnt, const int* __restrict__ crp,
-                                                  int* __restrict__ cci, double* __restrict__ cv,
-                                                  int* __restrict__ spill, int* __restrict__ n_spill) {
+__global__ __launch_bounds__(64) void k_gemm_hash(const int4* __restrict__ rows, int n_rows, CsrRef A,
+                                                  const int2* __restrict__ bb, CsrRef B, int* __restrict__ cnt,
+                                                  const int* __restrict__ crp, int* __restrict__ cci,
+                                                  double* __restrict__ cv, int4* __restrict__ spill,
+                                                  int* __restrict__ n_spill) {
     constexpr int HALF = CAP / 2;
     constexpr int BITS = __builtin_ctz(CAP);
     __shared__ int key[CAP];
     __shared__ double val[CAP];
     __shared__ int occl[HALF + 64];
-    __shared__ int lk[HALF];
-    __shared__ double lvv[HALF];
+    __shared__ int lk[PASS == 1 ? HALF : 1];
+    __shared__ double lvv[PASS == 1 ? HALF : 1];
     const int lane = threadIdx.x;
     for (int s = lane; s < CAP; s += 64) {
         key[s] = -1;
         val[s] = 0.0;
     }
     __syncthreads();
-    for (int r = blockIdx.x; r < n_rows; r += gridDim.x) {
-        const int i = rows[r];
+    for_each_row(rows, n_rows, A, bb, [&](int4 row, double av, int2 b) {
+        const int i = row.x;
         int occ = 0;
         bool over = false;
-        // insert (l, av * bv) from this lane when l >= 0; false once over
-        auto add = [&](int l, double av, double bv) {
+        // insert (l, av * bv) from this lane when l >= 0; over once the table is half full
+        for_each_product(A, bb, B, row, av, b, over, [&](double avj, int l, double bv) {
             bool fresh = false;
             int s = 0;
             if (l >= 0) {
@@ -373,54 +640,13 @@ __global__ __launch_bounds__(64) void k_gemm_hash(const int* __restrict__ rows, 
                     if (old == l) break;
                     s = (s + 1) & (CAP - 1);
                 }
-                val[s] = fma(av, bv, val[s]);
+                val[s] = fma(avj, bv, val[s]);
             }
             const unsigned long long m = __ballot(fresh);
             if (fresh) occl[occ + __popcll(m & lanes_below(lane))] = s;
             occ += __popcll(m);
             if (occ > HALF) over = true;
-        };
-        const int a1 = A.rp[i + 1];
-        for (int base = A.rp[i]; base < a1 && !over; base += 64) {
-            const int pa = base + lane;
-            int b0 = 0, b1 = 0;
-            double av = 0.0;
-            if (pa < a1) {
-                const int k = A.ci[pa];
-                av = A.v[pa];
-                b0 = B.rp[k];
-                b1 = B.rp[k + 1];
-            }
-            const int na = min(64, a1 - base);
-            int nl = -1;
-            double nbv = 0.0;
-            {
-                const int c0 = bcast(b0, 0), c1 = bcast(b1, 0);
-                if (c0 + lane < c1) {
-                    nl = B.ci[c0 + lane];
-                    nbv = B.v[c0 + lane];
-                }
-            }
-            for (int j = 0; j < na && !over; ++j) {
-                const double avj = bcast(av, j);
-                const int bj0 = bcast(b0, j), bj1 = bcast(b1, j);
-                const int l = nl;
-                const double bv = nbv;
-                if (j + 1 < na) {
-                    const int c0 = bcast(b0, j + 1), c1 = bcast(b1, j + 1);
-                    nl = -1;
-                    if (c0 + lane < c1) {
-                        nl = B.ci[c0 + lane];
-                        nbv = B.v[c0 + lane];
-                    }
-                }
-                add(l, avj, bv);
-                for (int pb0 = bj0 + 64; pb0 < bj1 && !over; pb0 += 64) {
-                    const int pb = pb0 + lane;
-                    add(pb < bj1 ? B.ci[pb] : -1, avj, pb < bj1 ? B.v[pb] : 0.0);
-                }
-            }
-        }
+        });
         __syncthreads();
         int u = 0;
         for (int e0 = 0; e0 < occ; e0 += 64) {
@@ -448,7 +674,7 @@ __global__ __launch_bounds__(64) void k_gemm_hash(const int* __restrict__ rows, 
         if (PASS == 0) {
             if (lane == 0) {
                 cnt[i] = over ? 0 : u;
-                if (over) spill[atomicAdd(n_spill, 1)] = i;
+                if (over) spill[atomicAdd(n_spill, 1)] = row;
             }
         } else if (!over) {
             const int o = crp[i];
@@ -461,48 +687,34 @@ __global__ __launch_bounds__(64) void k_gemm_hash(const int* __restrict__ rows, 
             }
         }
         __syncthreads();
-    }
+    });
 }
 
 // One wave per row, dense accumulator over a window of W columns in LDS
-// (W doubles, dynamic).  Rows wider than the window are processed window by
-// window, each a full pass over the row's products.  An untouched column
-// holds 0 and falls under the drop rule, so no touched flags are kept.
+// (W doubles, dynamic), products in the row's order (for_each_product).
+// Rows wider than the window are processed window by window, each a full
+// pass over the row's products.  An untouched column holds 0 and falls under
+// the drop rule, so no touched flags are kept.
 template <int PASS>
-__global__ __launch_bounds__(64) void k_gemm_dense(const int* __restrict__ rows, int n_rows, CsrRef A, CsrRef B,
-                                                   int W, int* __restrict__ cnt, const int* __restrict__ crp,
+__global__ __launch_bounds__(64) void k_gemm_dense(const int4* __restrict__ rows, int n_rows, CsrRef A,
+                                                   const int2* __restrict__ bb, CsrRef B, int W,
+                                                   int* __restrict__ cnt, const int* __restrict__ crp,
                                                    int* __restrict__ cci, double* __restrict__ cv) {
     extern __shared__ double acc[];
     const int lane = threadIdx.x;
     for (int c = lane; c < W; c += 64) acc[c] = 0.0;
     __syncthreads();
-    for (int r = blockIdx.x; r < n_rows; r += gridDim.x) {
-        const int i = rows ? rows[r] : r;
+    const bool never = false;
+    for_each_row(rows, n_rows, A, bb, [&](int4 row, double av, int2 b) {
+        const int i = row.x;
         int u = 0;
         const int o = PASS == 1 ? crp[i] : 0;
         for (int c0 = 0; c0 < B.cols; c0 += W) {
             const int wn = min(W, B.cols - c0);
-            const int a1 = A.rp[i + 1];
-            for (int base = A.rp[i]; base < a1; base += 64) {
-                const int pa = base + lane;
-                int b0 = 0, b1 = 0;
-                double av = 0.0;
-                if (pa < a1) {
-                    const int k = A.ci[pa];
-                    av = A.v[pa];
-                    b0 = B.rp[k];
-                    b1 = B.rp[k + 1];
-                }
-                const int na = min(64, a1 - base);
-                for (int j = 0; j < na; ++j) {
-                    const double avj = bcast(av, j);
-                    const int bj1 = bcast(b1, j);
-                    for (int pb = bcast(b0, j) + lane; pb < bj1; pb += 64) {
-                        const int l = B.ci[pb] - c0;
-                        if (l >= 0 && l < wn) acc[l] = fma(avj, B.v[pb], acc[l]);
-                    }
-                }
-            }
+            for_each_product(A, bb, B, row, av, b, never, [&](double avj, int l, double v) {
+                l -= c0;
+                if (l >= 0 && l < wn) acc[l] = fma(avj, v, acc[l]);
+            });
             __syncthreads();
             for (int cb = 0; cb < wn; cb += 64) {
                 const int c = cb + lane;
@@ -520,7 +732,7 @@ __global__ __launch_bounds__(64) void k_gemm_dense(const int* __restrict__ rows,
             __syncthreads();
         }
         if (PASS == 0 && lane == 0) cnt[i] = u;
-    }
+    });
 }
 
 // ---- 1 / diagonal (amg.cpp:189-196, 224-228) and the raw diagonal --------------
@@ -570,6 +782,7 @@ CsrRef ref_of(const Csr& A) {
     CsrRef r;
     r.rows = A.rows;
     r.cols = A.cols;
+    r.nnz = A.nnz;
     r.rp = A.rp.as<int>();
     r.ci = A.ci.as<int>();
     r.v = A.v64.as<double>();
@@ -583,15 +796,20 @@ std::unique_ptr<Csr> spgemm_dev(Ctx* c, const CsrRef& A, const CsrRef& B, bool f
     const bool dense = B.cols <= kDenseCols;
     const int W = std::min(std::max(B.cols, 1), kDenseCols);
     const size_t dense_lds = (size_t)W * sizeof(double);
-    DevBuf lists, counters, spill;
-    int n_small = 0, n_large = 0, n_spill = 0;
-    if (!dense && m > 0) {
-        lists = DevBuf(c, (size_t)m * 2 * sizeof(int));
-        spill = DevBuf(c, (size_t)m * sizeof(int));
+    const bool any = m > 0 && A.nnz > 0 && B.nnz > 0;  // else every row is empty
+    DevBuf lists, counters, spill, bbuf;
+    int n_small = 0, n_large = 0, n_spill = 0, n_spill2 = 0;
+    if (any) {
+        bbuf = DevBuf(c, (size_t)A.nnz * sizeof(int2));
+        hipLaunchKernelGGL(k_gemm_bounds, dim3(blocks_for((int)std::min<long long>(A.nnz, 1 << 30))), dim3(kBlock), 0, s,
+                           A.nnz, A.ci, B.rp, bbuf.as<int2>());
+        HSPS_CHECK_LAUNCH();
+        lists = DevBuf(c, (size_t)m * 2 * sizeof(int4));
+        spill = DevBuf(c, (size_t)m * 2 * sizeof(int4));
         counters = DevBuf(c, 4 * sizeof(int));
         HSPS_HIP(hipMemsetAsync(counters.p, 0, 4 * sizeof(int), s));
-        hipLaunchKernelGGL(k_gemm_classify, dim3(blocks_for(m)), dim3(kBlock), 0, s, A, B.rp, lists.as<int>(),
-                           lists.as<int>() + m, counters.as<int>());
+        hipLaunchKernelGGL(k_gemm_classify, dim3(blocks_for(m)), dim3(kBlock), 0, s, A, bbuf.as<int2>(),
+                           lists.as<int4>(), lists.as<int4>() + m, counters.as<int>());
         HSPS_CHECK_LAUNCH();
         int h[2];
         HSPS_HIP(hipMemcpyAsync(h, counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -599,35 +817,56 @@ std::unique_ptr<Csr> spgemm_dev(Ctx* c, const CsrRef& A, const CsrRef& B, bool f
         n_small = h[0];
         n_large = h[1];
     }
+    const int2* bb = bbuf.as<int2>();
+    const int4* small = lists.as<int4>();
+    const int4* large = lists.as<int4>() + m;
+    int4* spill1 = spill.as<int4>();
+    int4* spill2 = spill.as<int4>() + m;
+    // dense: both lists through the window kernel.  Otherwise the small
+    // table for small rows, the mid table for large ones; rows the mid table
+    // overflows (spill1, counters[2]) go to the large table, rows that one
+    // overflows (spill2, counters[3]) to the window kernel.
     auto run = [&](auto pass_tag, const DevBuf* crp, int* cci, double* cv) {
         constexpr int PASS = decltype(pass_tag)::value;
-        if (m == 0) return;
-        if (dense) {
-            hipLaunchKernelGGL(k_gemm_dense<PASS>, dim3(std::min(m, kMaxWaves)), dim3(64), dense_lds, s, nullptr, m, A, B,
-                               W, cnt.as<int>(), crp ? crp->as<int>() : nullptr, cci, cv);
-            HSPS_CHECK_LAUNCH();
+        if (!any) {
+            if (PASS == 0 && m > 0) HSPS_HIP(hipMemsetAsync(cnt.p, 0, (size_t)m * sizeof(int), s));
             return;
         }
-        int* ns = counters.as<int>() + 2;
+        int* out_rp = crp ? crp->as<int>() : nullptr;
+        auto window = [&](const int4* list, int nl) {
+            if (nl)
+                hipLaunchKernelGGL(k_gemm_dense<PASS>, dim3(std::min(nl, kMaxWaves)), dim3(64), dense_lds, s, list, nl,
+                                   A, bb, B, W, cnt.as<int>(), out_rp, cci, cv);
+            HSPS_CHECK_LAUNCH();
+        };
+        if (dense) {
+            window(small, n_small);
+            window(large, n_large);
+            return;
+        }
+        int* ns1 = counters.as<int>() + 2;
+        int* ns2 = counters.as<int>() + 3;
         if (n_small)
             hipLaunchKernelGGL((k_gemm_hash<kHashSmall, PASS>), dim3(std::min(n_small, kMaxWaves)), dim3(64), 0, s,
-                               lists.as<int>(), n_small, A, B, cnt.as<int>(), crp ? crp->as<int>() : nullptr, cci, cv,
-                               spill.as<int>(), ns);
+                               small, n_small, A, bb, B, cnt.as<int>(), out_rp, cci, cv, spill1, ns1);
         HSPS_CHECK_LAUNCH();
         if (n_large)
-            hipLaunchKernelGGL((k_gemm_hash<kHashLarge, PASS>), dim3(std::min(n_large, kMaxWaves)), dim3(64), 0, s,
-                               lists.as<int>() + m, n_large, A, B, cnt.as<int>(), crp ? crp->as<int>() : nullptr, cci,
-                               cv, spill.as<int>(), ns);
+            hipLaunchKernelGGL((k_gemm_hash<kHashMid, PASS>), dim3(std::min(n_large, kMaxWaves)), dim3(64), 0, s,
+                               large, n_large, A, bb, B, cnt.as<int>(), out_rp, cci, cv, spill1, ns1);
         HSPS_CHECK_LAUNCH();
         if (PASS == 0) {
-            HSPS_HIP(hipMemcpyAsync(&n_spill, ns, sizeof(int), hipMemcpyDeviceToHost, s));
+            HSPS_HIP(hipMemcpyAsync(&n_spill, ns1, sizeof(int), hipMemcpyDeviceToHost, s));
             HSPS_HIP(hipStreamSynchronize(s));
         }
         if (n_spill)
-            hipLaunchKernelGGL(k_gemm_dense<PASS>, dim3(std::min(n_spill, kMaxWaves)), dim3(64), dense_lds, s,
-                               spill.as<int>(), n_spill, A, B, W, cnt.as<int>(), crp ? crp->as<int>() : nullptr, cci,
-                               cv);
+            hipLaunchKernelGGL((k_gemm_hash<kHashLarge, PASS>), dim3(std::min(n_spill, kMaxWaves)), dim3(64), 0, s,
+                               spill1, n_spill, A, bb, B, cnt.as<int>(), out_rp, cci, cv, spill2, ns2);
         HSPS_CHECK_LAUNCH();
+        if (PASS == 0 && n_spill) {
+            HSPS_HIP(hipMemcpyAsync(&n_spill2, ns2, sizeof(int), hipMemcpyDeviceToHost, s));
+            HSPS_HIP(hipStreamSynchronize(s));
+        }
+        window(spill2, n_spill2);
     };
     run(std::integral_constant<int, 0>{}, nullptr, nullptr, nullptr);
     DevBuf crp;
@@ -668,16 +907,22 @@ void amg_setup_device(Precond& M, const Csr& A, const hsps_params& p, AmgSetupTi
         auto t0 = clk::now();
         DevBuf cnt(c, (size_t)n * sizeof(int)), srp;
         std::vector<int> h_srp;
-        hipLaunchKernelGGL(k_strong, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, F->rp.as<int>(), F->ci.as<int>(),
-                           F->v64.as<double>(), p.amg_strong, cnt.as<int>(), nullptr, nullptr);
-        HSPS_CHECK_LAUNCH();
+        const bool strong_wave = F->nnz > (long long)kStrongThread * n;
+        auto strong = [&](int* cnt_out, const int* srp_in, int* sci_out) {
+            if (strong_wave)
+                hipLaunchKernelGGL(k_strong_wave, dim3(std::min(n, kMaxWaves)), dim3(64), 0, s, n, F->rp.as<int>(),
+                                   F->ci.as<int>(), F->v64.as<double>(), p.amg_strong, cnt_out, srp_in, sci_out);
+            else
+                hipLaunchKernelGGL(k_strong, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, F->rp.as<int>(),
+                                   F->ci.as<int>(), F->v64.as<double>(), p.amg_strong, cnt_out, srp_in, sci_out);
+            HSPS_CHECK_LAUNCH();
+        };
+        strong(cnt.as<int>(), nullptr, nullptr);
         scan_rows(c, cnt.as<int>(), n, srp, h_srp, s, "strength pattern");
         std::vector<int> h_sci(h_srp[n]);
         {
             DevBuf sci(c, (size_t)std::max(h_srp[n], 1) * sizeof(int));
-            hipLaunchKernelGGL(k_strong, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, F->rp.as<int>(), F->ci.as<int>(),
-                               F->v64.as<double>(), p.amg_strong, nullptr, srp.as<int>(), sci.as<int>());
-            HSPS_CHECK_LAUNCH();
+            strong(nullptr, srp.as<int>(), sci.as<int>());
             if (h_srp[n])
                 HSPS_HIP(hipMemcpyAsync(h_sci.data(), sci.p, (size_t)h_srp[n] * sizeof(int), hipMemcpyDeviceToHost, s));
             HSPS_HIP(hipStreamSynchronize(s));
@@ -691,23 +936,35 @@ void amg_setup_device(Precond& M, const Csr& A, const hsps_params& p, AmgSetupTi
         t0 = clk::now();
         DevBuf dagg(c, (size_t)n * sizeof(int));
         HSPS_HIP(hipMemcpyAsync(dagg.p, agg.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
-        // P: short rows one thread each, long rows one wave each
+        // P: short rows one thread each, medium rows one wave each (owner
+        // lanes), long rows one wave each over LDS windows
         std::unique_ptr<Csr> P;
         {
             const size_t slots = (size_t)F->nnz + n;
             DevBuf lc(c, slots * sizeof(int)), lvals(c, slots * sizeof(double)), len(c, (size_t)n * sizeof(int));
-            DevBuf longs(c, (size_t)n * sizeof(int)), n_long_d(c, sizeof(int));
-            HSPS_HIP(hipMemsetAsync(n_long_d.p, 0, sizeof(int), s));
+            DevBuf longs(c, (size_t)n * sizeof(int)), waves(c, (size_t)n * sizeof(int)),
+                n_lists(c, 2 * sizeof(int));
+            HSPS_HIP(hipMemsetAsync(n_lists.p, 0, 2 * sizeof(int), s));
             hipLaunchKernelGGL(k_p_classify, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, F->rp.as<int>(),
-                               longs.as<int>(), n_long_d.as<int>());
+                               longs.as<int>(), n_lists.as<int>(), waves.as<int>(), n_lists.as<int>() + 1);
             HSPS_CHECK_LAUNCH();
             hipLaunchKernelGGL(k_p_rows, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, F->rp.as<int>(), F->ci.as<int>(),
                                F->v64.as<double>(), dagg.as<int>(), M.amg_omega, lc.as<int>(), lvals.as<double>(),
                                len.as<int>(), cnt.as<int>());
             HSPS_CHECK_LAUNCH();
-            int n_long = 0;
-            HSPS_HIP(hipMemcpyAsync(&n_long, n_long_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            int h_n[2] = {0, 0};
+            HSPS_HIP(hipMemcpyAsync(h_n, n_lists.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             HSPS_HIP(hipStreamSynchronize(s));
+            if (h_n[1]) {
+                hipLaunchKernelGGL(k_p_wave, dim3(std::min(h_n[1], kMaxWaves)), dim3(64), 0, s, waves.as<int>(), h_n[1],
+                                   F->rp.as<int>(), F->ci.as<int>(), F->v64.as<double>(), dagg.as<int>(), M.amg_omega,
+                                   lc.as<int>(), lvals.as<double>(), len.as<int>(), cnt.as<int>(), longs.as<int>(),
+                                   n_lists.as<int>());
+                HSPS_CHECK_LAUNCH();
+                HSPS_HIP(hipMemcpyAsync(h_n, n_lists.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                HSPS_HIP(hipStreamSynchronize(s));
+            }
+            const int n_long = h_n[0];
             const int Wp = std::min(std::max(nc, 1), kPWindow);
             const size_t long_lds = (size_t)Wp * sizeof(double) + 64 * (sizeof(double) + sizeof(int));
             if (n_long)
