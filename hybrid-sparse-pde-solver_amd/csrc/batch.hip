@@ -123,7 +123,10 @@ __device__ __forceinline__ BHot load_hot(const BState* S) {
 // threads; vec = 0 sends the tile down the general paths, which read the
 // tables themselves.  The iteration kernels read the records of the tiles
 // whose systems are still iterating from a compacted copy (BArgs::tva*),
-// which the host rebuilds as systems halt.
+// which the host rebuilds as systems halt.  vec = 3: a CSR system none of
+// whose tiles exceeds the LDS product stage, run by k_b_k1c / k_b_k2c (every
+// tile of a system takes the same kernels, so its fused finish finds all of
+// them in one launch).
 template <class T>
 struct alignas(128) TileVec {
     int r0, r1, sys, vec;  // rows [r0, r1) of internal system sys
@@ -131,6 +134,7 @@ struct alignas(128) TileVec {
     int off[5];            // diagonal offsets, ascending (padding: offset 0 with value 0, never present)
     int tile;              // the tile's index (its partials slot)
     int fin;               // in an active list: the system's workgroup the dispatcher starts last (runs the finish)
+    int cdok;              // D^-1 is the scalar cd (else the dinv vector)
     T uv[5];               // the diagonals' values
     T cd;                  // D^-1
 };
@@ -622,6 +626,155 @@ __device__ __forceinline__ void tile_rows64(const BArgs& B, int s, int r0, int r
     }
 }
 
+// K1 rows [r0, r1) of a CSR system whose tile holds at most kLdsNnz nonzeros
+// (k0 = rp[r0], k1 = rp[r1]):
+// p_new = D^-1 r + beta p_old for the tile's own rows (stored to pn) and for
+// every column its nonzeros touch (the same expression on the same inputs as
+// the column's owner row, so bit-identical), q = A p_new (stored) and the
+// p_new . q partial.  Every column and value load of the tile is issued, then
+// every gather of r, p_old (and D^-1) at those columns, then the own rows'
+// loads, and only then is anything stored or summed: a wave keeps its loads in
+// flight together instead of one dependent gather chain per product.  The
+// products val * p_new go to LDS and each row is summed from there exactly as
+// tile_spmv sums it (same order, same rounding).  The trip count nu is
+// uniform over the workgroup; lanes past the last nonzero re-read it and park
+// a product in a slot no row reads.
+constexpr int kCsrOwn = 4;  // own rows per thread held in registers (tiles of up to 1024 rows)
+template <class T>
+__device__ __forceinline__ void k1_csr(int r0, int r1, const int* __restrict__ rp, const int* __restrict__ ci,
+                                       const T* __restrict__ val, const T* __restrict__ r, const T* __restrict__ po,
+                                       const T* __restrict__ dinv, bool cdok, T cdv, double beta, lds_ptr<T> prod,
+                                       T* __restrict__ pn, T* __restrict__ q, int k0, int k1, double& pq) {
+    // products per thread, in rounds of UB whose loads are all in flight
+    // together (FP64: two rounds of four, which keeps the kernel's registers
+    // at the occupancy of its other paths)
+    constexpr int U = kLdsNnz / kBlock, UB = sizeof(T) == 8 ? 4 : U;
+    const int tid = threadIdx.x;
+    const int nu = (k1 - k0 + kBlock - 1) / kBlock;
+    auto pnew = [&](T d, T rj, T pj) -> T { return (T)((double)(T)(d * rj) + beta * (double)pj); };
+    T ro[kCsrOwn], pw[kCsrOwn], dw[kCsrOwn];
+#pragma unroll
+    for (int ub = 0; ub < U; ub += UB) {
+        unsigned col[UB];
+        T vv[UB], rg[UB], pg[UB], dg[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            col[u] = 0;
+            vv[u] = rg[u] = pg[u] = T(0);
+            dg[u] = cdv;
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u)
+            if (ub + u < nu) {
+                const int k = min(k0 + tid + (ub + u) * kBlock, k1 - 1);
+                col[u] = (unsigned)ci[k];
+                vv[u] = val[k];
+            }
+#pragma unroll
+        for (int u = 0; u < UB; ++u)
+            if (ub + u < nu) {
+                rg[u] = r[col[u]];
+                pg[u] = po[col[u]];
+                if (!cdok) dg[u] = dinv[col[u]];
+            }
+        if (ub == 0) {
+#pragma unroll
+            for (int j = 0; j < kCsrOwn; ++j) {
+                const int i = min(r0 + tid + j * kBlock, r1 - 1);
+                ro[j] = r[i];
+                pw[j] = po[i];
+                dw[j] = cdok ? cdv : dinv[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u)
+            if (ub + u < nu) prod[tid + (ub + u) * kBlock] = vv[u] * pnew(dg[u], rg[u], pg[u]);
+    }
+    T own[kCsrOwn];
+#pragma unroll
+    for (int j = 0; j < kCsrOwn; ++j) {
+        const int i = r0 + tid + j * kBlock;
+        own[j] = pnew(dw[j], ro[j], pw[j]);
+        if (i < r1) pn[i] = own[j];
+    }
+    auto pnew_at = [&](int i) -> T { return pnew(cdok ? cdv : dinv[i], r[i], po[i]); };
+    for (int i = r0 + tid + kCsrOwn * kBlock; i < r1; i += kBlock) pn[i] = pnew_at(i);
+    __syncthreads();
+    const int nrows = r1 - r0;
+    int tpr = 1;
+    while (tpr < 64 && tpr * 2 * nrows <= kBlock) tpr *= 2;
+    auto row_sum = [&](int row, int l) -> T {
+        const int a = rp[row] - k0, b = rp[row + 1] - k0;
+        T s = 0;
+        for (int k = a + l; k < b; k += tpr) s += prod[k];
+        return s;
+    };
+    if (tpr == 1) {  // one row per thread: rows r0 + tid + j kBlock, p_new from registers
+#pragma unroll
+        for (int j = 0; j < kCsrOwn; ++j) {
+            const int row = r0 + tid + j * kBlock;
+            if (row < r1) {
+                const T sum = row_sum(row, 0);
+                q[row] = sum;
+                pq += (double)own[j] * (double)sum;
+            }
+        }
+        for (int row = r0 + tid + kCsrOwn * kBlock; row < r1; row += kBlock) {
+            const T sum = row_sum(row, 0);
+            q[row] = sum;
+            pq += (double)pnew_at(row) * (double)sum;
+        }
+    } else {
+        const int g = tid / tpr, l = tid - g * tpr, groups = kBlock / tpr;
+        for (int row = r0 + g; row < r1; row += groups) {
+            T s = row_sum(row, l);
+            for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (l == 0) {
+                q[row] = s;
+                pq += (double)pnew_at(row) * (double)s;
+            }
+        }
+    }
+}
+
+// K2 rows [r0, r1) of a system whose q is stored (CSR, DIA): x += alpha p
+// (x_lo in mixed mode), r -= alpha q, r.r and r.z partials.  kCsrOwn rows per
+// thread load together before any store (vmcnt counts loads and stores in one
+// queue), each row in the same arithmetic as k_b_k2's row body.
+template <class T, bool MIXED>
+__device__ __forceinline__ void k2_rows(int r0, int r1, const T* __restrict__ p, const T* __restrict__ q,
+                                        T* __restrict__ r, double* __restrict__ x64, float* __restrict__ xlo,
+                                        const T* __restrict__ dinv, bool cdok, T cdv, double alpha,
+                                        double (&red)[2]) {
+    using X = std::conditional_t<MIXED, float, double>;
+    X* __restrict__ xs = MIXED ? (X*)xlo : (X*)x64;
+    for (int i0 = r0 + threadIdx.x; i0 < r1; i0 += kCsrOwn * kBlock) {
+        T pv[kCsrOwn], qv[kCsrOwn], rv[kCsrOwn], dv[kCsrOwn];
+        X xv[kCsrOwn];
+#pragma unroll
+        for (int j = 0; j < kCsrOwn; ++j) {
+            const int i = min(i0 + j * kBlock, r1 - 1);
+            pv[j] = p[i];
+            qv[j] = q[i];
+            rv[j] = r[i];
+            xv[j] = xs[i];
+            dv[j] = cdok ? cdv : dinv[i];
+        }
+#pragma unroll
+        for (int j = 0; j < kCsrOwn; ++j) {
+            const int i = i0 + j * kBlock;
+            if (i < r1) {
+                xs[i] = (X)((double)xv[j] + alpha * (double)pv[j]);
+                const T ri = (T)((double)rv[j] - alpha * (double)qv[j]);
+                r[i] = ri;
+                const double rd = (double)ri;
+                red[0] += rd * rd;
+                red[1] += rd * (double)(T)(dv[j] * ri);
+            }
+        }
+    }
+}
+
 // Per-system finish: one wave per system of [k0, k0+gridDim.x), or per entry
 // of sys when given (the active systems of an iteration).  The
 // system's state and tile range are loaded first (one round trip), then its
@@ -999,12 +1152,12 @@ k_b_k1(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __
     const TileVec<T>& D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
     if (D.vec) return;  // k_b_k1v runs it
     const int tile = D.tile;
-    const int s = B.tile_sys[tile];
+    const int s = B.tile_sys[tile];  // (D.sys, the same, puts a stack frame on this kernel)
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
     const int flag = S0->flag;
     const unsigned tag = (unsigned)S0->seq + 1u;
-    const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
+    const int r0 = D.r0, r1 = D.r1;
     const bool cdok = B.cd_ok[s] != 0;
     const T cdv = Prec<T>::cd(B)[s];
     const T* __restrict__ dinv = Prec<T>::dinv(B);
@@ -1049,11 +1202,17 @@ k_b_k1(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __
             }
         }
     } else {
-        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) pn[i] = pnew(i);
-        tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), pnew, (lds_ptr<T>)prod64, [&](int i, T sum) {
-            q[i] = sum;
-            red[0] += (double)pnew(i) * (double)sum;
-        });
+        const int k0 = B.rp[r0], k1 = B.rp[r1];
+        if (k1 - k0 <= kLdsNnz) {
+            k1_csr<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), r, po, dinv, cdok, cdv, beta, (lds_ptr<T>)prod64, pn, q,
+                      k0, k1, red[0]);
+        } else {  // one row longer than the product stage
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) pn[i] = pnew(i);
+            tile_spmv<T>(r0, r1, B.rp, B.ci, Prec<T>::val(B), pnew, (lds_ptr<T>)prod64, [&](int i, T sum) {
+                q[i] = sum;
+                red[0] += (double)pnew(i) * (double)sum;
+            });
+        }
     }
     tile_finish<2, kBlock>(red, B, tile, s, 0, tag, D.fin, OpF1{});
 }
@@ -1069,12 +1228,12 @@ k_b_k2(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __r
     const TileVec<T>& D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
     if (D.vec) return;  // k_b_k2v runs it
     const int tile = D.tile;
-    const int s = B.tile_sys[tile];
+    const int s = D.sys;
     const BState* S0 = &B.st[s];
     if (S0->halt) return;
     const int flag = S0->flag;
     const unsigned tag = (unsigned)S0->seq + 1u;
-    const int r0 = B.tile_r0[tile], r1 = B.tile_r1[tile];
+    const int r0 = D.r0, r1 = D.r1;
     if (flag == BS_FLUSHED) {
         if (MIXED)
             for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
@@ -1110,7 +1269,7 @@ k_b_k2(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __r
             for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], A.template row<0>(i, pv));
         }
     } else {
-        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) body(i, p[i], q[i]);
+        k2_rows<T, MIXED>(r0, r1, p, q, r, x64, xlo, dinv, cdok, cdv, alpha, red);
     }
     tile_finish<2, kBlock>(red, B, tile, s, 1, tag, D.fin, OpF2<MIXED>{});
 }
@@ -1177,6 +1336,103 @@ k_b_k2v(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, T* __restri
     k2_stencil_vec<T, MIXED, BS>(D, B.umask, (lds_ptr<T>)(lds_ptr<double>)stage_lds, h.pcur ? p1 : p0, r, x64, xlo,
                              h.alpha, red);
     tile_finish<2, BS>(red, B, tile, D.sys, 1, (unsigned)h.seq + 1u, D.fin, OpF2<MIXED>{});
+}
+
+// ---- K1 / K2 of CSR systems (TileVec::vec = 3) ---------------------------------------------
+// The same steps and arithmetic as k_b_k1 / k_b_k2's CSR paths, with every
+// value a workgroup needs before its first vector load in the tile record and
+// the loads that do not depend on the system's state (the tile's nonzero
+// range; K2's own rows of q, r and x) issued together with the state: a
+// workgroup waits for the record, then one round trip, then its column and
+// value loads, where the general kernel waited for four dependent lookups.
+template <class T, bool MIXED>
+__global__ void __launch_bounds__(kBlock)
+k_b_k1c(BArgs B, T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q, T* __restrict__ r,
+        const double* __restrict__ b, const double* __restrict__ x64, const float* __restrict__ xlo) {
+    extern __shared__ __attribute__((aligned(16))) double stage_lds[];  // B.stage_doubles >= kLdsNnz
+    const TileVec<T> D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    if (D.vec != 3) return;
+    const int k0 = B.rp[D.r0], k1 = B.rp[D.r1];
+    const BHot h = load_hot(&B.st[D.sys]);
+    const int mode = h.halt ? -1 : h.flag;
+    if (mode < 0) return;
+    const T* __restrict__ dinv = Prec<T>::dinv(B);
+    const bool cdok = D.cdok != 0;
+    double red[2] = {0.0, 0.0};
+    if (mode == BS_FLUSH) {  // FP64 true residual, as k_b_k1
+        auto xv = [&](int j) { return MIXED ? x64[j] + (double)xlo[j] : x64[j]; };
+        tile_spmv<double>(D.r0, D.r1, B.rp, B.ci, B.v64, xv, (lds_ptr<double>)stage_lds, [&](int i, double sum) {
+            const double rh = b[i] - sum;
+            const T ri = (T)rh;
+            r[i] = ri;
+            red[0] += rh * rh;
+            red[1] += (double)ri * (double)(T)((cdok ? D.cd : dinv[i]) * ri);
+        });
+    } else {
+        k1_csr<T>(D.r0, D.r1, B.rp, B.ci, Prec<T>::val(B), r, h.pcur ? p1 : p0, dinv, cdok, D.cd, h.beta,
+                  (lds_ptr<T>)(lds_ptr<double>)stage_lds, h.pcur ? p0 : p1, q, k0, k1, red[0]);
+    }
+    tile_finish<2, kBlock>(red, B, D.tile, D.sys, 0, (unsigned)h.seq + 1u, D.fin, OpF1{});
+}
+
+template <class T, bool MIXED>
+__global__ void __launch_bounds__(kBlock)
+k_b_k2c(BArgs B, const T* __restrict__ p0, const T* __restrict__ p1, const T* __restrict__ q,
+        T* __restrict__ r, double* __restrict__ x64, float* __restrict__ xlo) {
+    using X = std::conditional_t<MIXED, float, double>;
+    const TileVec<T> D = Prec<T>::tva(B)[xcd_tile(blockIdx.x, gridDim.x)];
+    if (D.vec != 3) return;
+    X* __restrict__ xs = MIXED ? (X*)xlo : (X*)x64;
+    const int tid = threadIdx.x, r0 = D.r0, r1 = D.r1;
+    // the first kCsrOwn rows per thread of q, r and x, loaded with the state
+    T qv[kCsrOwn], rv[kCsrOwn];
+    X xv[kCsrOwn];
+#pragma unroll
+    for (int j = 0; j < kCsrOwn; ++j) {
+        const int i = min(r0 + tid + j * kBlock, r1 - 1);
+        qv[j] = q[i];
+        rv[j] = r[i];
+        xv[j] = xs[i];
+    }
+    const BHot h = load_hot(&B.st[D.sys]);
+    const int mode = h.halt ? -1 : h.flag;
+    if (mode != BS_NORMAL && mode != BS_FLUSHED) return;
+    if (mode == BS_FLUSHED) {
+        if (MIXED)
+            for (int i = r0 + tid; i < r1; i += kBlock) {
+                x64[i] += (double)xlo[i];
+                xlo[i] = 0.0f;
+            }
+        return;
+    }
+    const T* __restrict__ p = h.pcur ? p1 : p0;
+    const T* __restrict__ dinv = Prec<T>::dinv(B);
+    const bool cdok = D.cdok != 0;
+    const double alpha = h.alpha;
+    double red[2] = {0.0, 0.0};
+    T pv[kCsrOwn], dv[kCsrOwn];
+#pragma unroll
+    for (int j = 0; j < kCsrOwn; ++j) {
+        const int i = min(r0 + tid + j * kBlock, r1 - 1);
+        pv[j] = p[i];
+        dv[j] = cdok ? D.cd : dinv[i];
+    }
+    // k2_rows' row body, rows r0 + tid + j kBlock in the same order
+#pragma unroll
+    for (int j = 0; j < kCsrOwn; ++j) {
+        const int i = r0 + tid + j * kBlock;
+        if (i < r1) {
+            xs[i] = (X)((double)xv[j] + alpha * (double)pv[j]);
+            const T ri = (T)((double)rv[j] - alpha * (double)qv[j]);
+            r[i] = ri;
+            const double rd = (double)ri;
+            red[0] += rd * rd;
+            red[1] += rd * (double)(T)(dv[j] * ri);
+        }
+    }
+    if (r1 - r0 > kCsrOwn * kBlock)
+        k2_rows<T, MIXED>(r0 + kCsrOwn * kBlock, r1, p, q, r, x64, xlo, dinv, cdok, D.cd, alpha, red);
+    tile_finish<2, kBlock>(red, B, D.tile, D.sys, 1, (unsigned)h.seq + 1u, D.fin, OpF2<MIXED>{});
 }
 
 // ---- final: x64 += x_lo and the FP64 true residual per (large) system ----------------------
@@ -1471,6 +1727,10 @@ static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::
     const int es = (int)sizeof(T), cap = B.stage_doubles[es == 4 ? 1 : 0] * 8 / es;
     std::vector<TileVec<T>> tv(std::max<size_t>(r0.size(), 1));
     std::memset(tv.data(), 0, tv.size() * sizeof(TileVec<T>));
+    // CSR systems all of whose tiles fit the product stage (no row longer than it)
+    std::vector<char> csr_fits(B.S, 1);
+    for (size_t t = 0; t < r0.size(); ++t)
+        if (B.h_rp[r1[t]] - B.h_rp[r0[t]] > kLdsNnz) csr_fits[sys[t]] = 0;
     for (size_t t = 0; t < r0.size(); ++t) {
         const int k = sys[t], s = B.order[k];
         TileVec<T>& d = tv[t];
@@ -1490,10 +1750,11 @@ static void build_tile_records(Batch& B, const std::vector<int>& r0, const std::
             d.uv[j] = j < nd ? (T)B.h_uv[(size_t)k * kMaxDiags + j] : T(0);
         }
         d.cd = (T)B.h_cd[k];
+        d.cdok = B.h_cdok[k];
         const int need = stage_need(halo, es);
         d.vec = B.h_kind[k] == SK_STENCIL && nd <= 5 && B.h_cdok[k] && need <= cap
                     ? ((long long)need * es > kStageMinDoubles * 8LL ? 2 : 1)
-                    : 0;
+                    : B.h_kind[k] == SK_CSR && csr_fits[k] ? 3 : 0;
     }
     out = DevBuf(B.ctx, tv.size() * sizeof(TileVec<T>));
     upload_on(B.ctx->stream, out.p, tv.data(), tv.size() * sizeof(TileVec<T>));
@@ -1792,10 +2053,13 @@ static BArgs args_of(const Batch& B, const hsps_params& p) {
 
 // The active set of a wave: the tiles K1 / K2 run (their records, compacted)
 // and the number of their systems.  kinds: bit 0 / bit 2 = some tile takes the
-// narrow / wide five-point kernels, bit 1 = some tile takes the general ones.
+// narrow / wide five-point kernels, bit 3 the CSR ones, bit 1 the general ones.
 struct Active {
     int nt = 0, ns = 0, kinds = 0;
 };
+// bit of Active::kinds for a tile record's kernel family: 1 narrow and 4 wide
+// five-point, 8 CSR (vec = 3), 2 the general kernels
+static int kind_bit(int vec) { return vec == 1 ? 1 : vec == 2 ? 4 : vec == 3 ? 8 : 2; }
 
 template <class T>
 static const std::vector<TileVec<T>>& host_records(const Batch& B) {
@@ -1839,7 +2103,7 @@ static Active upload_active(Batch& B, int k0, int k1, const std::vector<char>* l
         ++A.ns;
         for (int t = B.h_sys_t0[k]; t < B.h_sys_t1[k]; ++t) {
             rec[A.nt++] = tv[t];
-            A.kinds |= tv[t].vec == 1 ? 1 : tv[t].vec == 2 ? 4 : 2;
+            A.kinds |= kind_bit(tv[t].vec);
         }
     }
     mark_finishers(rec, A.nt, B.S);
@@ -1859,7 +2123,7 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
     T* q = B.q.as<T>();
     float* xl = B.xlo.as<float>();
     const int nt = act.nt;
-    const bool narrow = act.kinds & 1, any_gen = act.kinds & 2, wide = act.kinds & 4;
+    const bool narrow = act.kinds & 1, any_gen = act.kinds & 2, wide = act.kinds & 4, csr = act.kinds & 8;
     for (int it = 0; it < iters; ++it) {
         const size_t lds = sizeof(double) * (size_t)a.stage_doubles;
         if (narrow)
@@ -1871,6 +2135,9 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
         if (any_gen)
             hipLaunchKernelGGL((k_b_k1<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, q, r, b,
                                (const double*)x, (const float*)xl);
+        if (csr)
+            hipLaunchKernelGGL((k_b_k1c<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, p0, p1, q, r, b,
+                               (const double*)x, (const float*)xl);
         if (narrow)
             hipLaunchKernelGGL((k_b_k2v<T, MIXED, kVecNarrow>), dim3(nt), dim3(kVecNarrow), lds, s, a, (const T*)p0,
                                (const T*)p1, r, x, xl);
@@ -1879,6 +2146,9 @@ static void enqueue_iters(const Batch& B, const BArgs& a, const Active& act, int
                                (const T*)p1, r, x, xl);
         if (any_gen)
             hipLaunchKernelGGL((k_b_k2<T, MIXED>), dim3(nt), dim3(kBlock), lds, s, a, (const T*)p0, (const T*)p1,
+                               (const T*)q, r, x, xl);
+        if (csr)
+            hipLaunchKernelGGL((k_b_k2c<T, MIXED>), dim3(nt), dim3(kBlock), 0, s, a, (const T*)p0, (const T*)p1,
                                (const T*)q, r, x, xl);
     }
     HSPS_CHECK_LAUNCH();
@@ -2476,7 +2746,8 @@ static void session_alloc(Session& S, Ctx* ctx, int max_systems, long long max_r
 }
 
 template <class T>
-static TileVec<T> session_record(const SysStorage& st, int slot, int t, int r0, int r1, int row0, int n) {
+static TileVec<T> session_record(const SysStorage& st, int slot, int t, int r0, int r1, int row0, int n,
+                                 bool csr_fits) {
     constexpr int es = (int)sizeof(T);
     TileVec<T> d;
     std::memset(&d, 0, sizeof d);
@@ -2495,12 +2766,13 @@ static TileVec<T> session_record(const SysStorage& st, int slot, int t, int r0, 
         d.uv[j] = j < st.nd ? (T)st.uv[j] : T(0);
     }
     d.cd = (T)st.cd;
+    d.cdok = st.cdok;
     // the five-point path when the system's own window fits the largest one a
     // launch provides (a batch of this system alone decides the same way)
     const int need = stage_need(halo, es), cap = kStageMaxDoubles * 8 / es;
     d.vec = st.kind == SK_STENCIL && st.nd <= 5 && st.cdok && need <= cap
                 ? ((long long)need * es > kStageMinDoubles * 8LL ? 2 : 1)
-                : 0;
+                : st.kind == SK_CSR && csr_fits ? 3 : 0;
     return d;
 }
 
@@ -2546,6 +2818,8 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
         i = j;
     }
     const int nt = (int)tr0.size();
+    bool csr_fits = true;
+    for (int t = 0; t < nt; ++t) csr_fits = csr_fits && rp[tr1[t]] - rp[tr0[t]] <= kLdsNnz;
     const long long row0 = S.rows.take(row_len);
     if (row0 < 0) return -1;
     const long long nz0 = S.nzs.take(nnz);
@@ -2605,8 +2879,8 @@ static int session_add(Session& S, int n, const int* rp, const int* ci, const do
     for (int t = 0; t < nt; ++t) {
         g0[t] = r0i + tr0[t];
         g1[t] = r0i + tr1[t];
-        r32[t] = session_record<float>(st, slot, t0i + t, g0[t], g1[t], r0i, n);
-        r64[t] = session_record<double>(st, slot, t0i + t, g0[t], g1[t], r0i, n);
+        r32[t] = session_record<float>(st, slot, t0i + t, g0[t], g1[t], r0i, n, csr_fits);
+        r64[t] = session_record<double>(st, slot, t0i + t, g0[t], g1[t], r0i, n, csr_fits);
         B.h_tv32[t0i + t] = r32[t];
         B.h_tv64[t0i + t] = r64[t];
     }
@@ -2675,7 +2949,7 @@ static void session_upload_active(Session& S) {
         need = std::max(need, sl.need[sizeof(T) == 4 ? 1 : 0]);
         for (int t = sl.t0; t < sl.t1; ++t) {
             rec[A.nt++] = tv[t];
-            A.kinds |= tv[t].vec == 1 ? 1 : tv[t].vec == 2 ? 4 : 2;
+            A.kinds |= kind_bit(tv[t].vec);
         }
     }
     mark_finishers(rec, A.nt, (int)S.slots.size());
