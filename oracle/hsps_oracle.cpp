@@ -10,6 +10,9 @@
 #include <limits>
 #include <numeric>
 #include <stdexcept>
+#include <vector>
+
+#include <omp.h>
 
 namespace orc {
 
@@ -127,10 +130,26 @@ void spmv(const Csr& A, const Real* x, Real* y) {
 }
 
 // ---- dot / norm2: vector.cpp:39-53 (OpenMP static reduction) -------------------
+// The same static partition of the rows over the threads as the reference's
+// reduction(+); the per-thread partial sums are then added in thread order
+// (libgomp combines a reduction's partials in arrival order, which made the
+// last bits, and the iteration counts of ill-conditioned solves, vary from
+// run to run).
 Real dot(Index n, const Real* a, const Real* b) {
+    const int nt = omp_get_max_threads();
+    std::vector<Real> part((size_t)nt, 0.0);
+    int used = 1;
+#pragma omp parallel
+    {
+        Real s = 0.0;
+#pragma omp for schedule(static)
+        for (Index i = 0; i < n; ++i) s += a[i] * b[i];
+        part[(size_t)omp_get_thread_num()] = s;
+#pragma omp single
+        used = omp_get_num_threads();
+    }
     Real s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (Index i = 0; i < n; ++i) s += a[i] * b[i];
+    for (int t = 0; t < used; ++t) s += part[(size_t)t];
     return s;
 }
 Real norm2(Index n, const Real* a) { return std::sqrt(dot(n, a, a)); }
