@@ -386,8 +386,11 @@ def bench_batched(H, ctx, ev, args):
         order = np.argsort([len(sy[2]) for sy in systems], kind="stable")
         sample_p, sample_1 = [int(s) for s in order[8::16]], [int(s) for s in order[32::64]]
         P, n_cpus = physical_cores()
-        rp_, xo = cpu_oracle_runs(specs, b, row_off, sample_p, P, 1, 3, "all physical cores", want_x=True)
-        r1_, _ = cpu_oracle_runs(specs, b, row_off, sample_1, 1, 1, 3, "1 thread")
+        # protocol: 3 warm-up passes, then the median of 5 timed ones (about 25 s of CPU work in all)
+        CPU_WARMUP, CPU_REPS = 3, 5
+        rp_, xo = cpu_oracle_runs(specs, b, row_off, sample_p, P, CPU_WARMUP, CPU_REPS, "all physical cores",
+                                  want_x=True)
+        r1_, _ = cpu_oracle_runs(specs, b, row_off, sample_1, 1, CPU_WARMUP, CPU_REPS, "1 thread")
         work = np.array([len(sy[2]) for sy in systems], np.float64) * it_mx.astype(np.float64)
 
         def ratio(sample):  # the sample's mean work per solve over the batch's
@@ -395,13 +398,15 @@ def bench_batched(H, ctx, ev, args):
         cpu = {"value": len(sample_p) / rp_["median_s"], "unit": "solves/s", "cores": P, "kind": "port",
                "sample": f"{len(sample_p)} of the {len(systems)} systems (every 16th by nonzeros), solved by the FP64 "
                          f"OpenMP oracle at {P} threads (OMP_PROC_BIND=close, OMP_PLACES=cores; {P} physical cores of "
-                         f"the {n_cpus} CPUs this process may use): 1 warm-up pass, median of 3 timed passes "
+                         f"the {n_cpus} CPUs this process may use): {CPU_WARMUP} warm-up passes, median of {CPU_REPS} "
+                         f"timed passes "
                          f"({rp_['median_s']:.2f} s); the sample's mean work per solve (nonzeros x iterations) is "
                          f"{ratio(sample_p):.3f} x the batch's, and value is the sample's own solves/s (no extrapolation)",
                "one_thread": {"value": len(sample_1) / r1_["median_s"], "unit": "solves/s", "threads": 1,
                               "systems": len(sample_1), "median_s": r1_["median_s"], "pass_seconds": r1_["pass_seconds"],
                               "work_ratio_vs_batch": ratio(sample_1),
-                              "sample": "every 64th system by nonzeros, 1 warm-up + median of 3 passes"},
+                              "sample": f"every 64th system by nonzeros, {CPU_WARMUP} warm-ups + median of {CPU_REPS} "
+                                        "passes"},
                "pass_seconds": rp_["pass_seconds"], "work_ratio_vs_batch": ratio(sample_p)}
         errs, dits = [], []
         for i, s in enumerate(sample_p):

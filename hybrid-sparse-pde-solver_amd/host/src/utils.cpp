@@ -69,12 +69,12 @@ SparseMatrix MatrixMarketIO::load_matrix(const std::string& path) {
     std::string line;
     std::getline(f, line);
     if (line.size() < 2 || line[0] != '%' || line[1] != '%')
-        throw std::runtime_error("MatrixMarketIO: missing %% banner in " + path);
+        throw std::runtime_error(path + ": first line is not a MatrixMarket header (expected %%MatrixMarket)");
     MatrixMarketHeader h = parse_header_line(line);
     if (h.format != MatrixMarketHeader::Format::COORDINATE)
-        throw std::runtime_error("MatrixMarketIO: load_matrix requires coordinate format, got array in " + path);
+        throw std::runtime_error(path + ": dense (array) MatrixMarket file; a sparse matrix needs the coordinate format");
     if (h.field == MatrixMarketHeader::Field::COMPLEX)
-        throw std::runtime_error("MatrixMarketIO: complex matrices not supported (use real part only)");
+        throw std::runtime_error(path + ": complex-valued MatrixMarket matrix; only real, integer and pattern fields are read");
     while (std::getline(f, line)) {
         if (line.empty() || line[0] == '%') continue;
         std::istringstream iss(line);
