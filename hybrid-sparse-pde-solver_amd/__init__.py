@@ -634,7 +634,7 @@ class ServeResult(C.Structure):
     _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("state", C.c_int), ("batched", C.c_int),
                 ("escalations", C.c_int), ("true_residual", C.c_double), ("queue_s", C.c_double),
                 ("latency_s", C.c_double), ("submit_s", C.c_double), ("done_s", C.c_double),
-                ("setup_s", C.c_double), ("solve_s", C.c_double)]
+                ("setup_s", C.c_double), ("solve_s", C.c_double), ("classify_s", C.c_double)]
 
 
 _llp = C.POINTER(C.c_longlong)

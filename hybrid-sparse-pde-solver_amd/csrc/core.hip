@@ -9,7 +9,12 @@
 // =============================================================================
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 
 #include "device_common.hpp"
 
@@ -40,16 +45,158 @@ void csr_release(hsps_csr_s* A) {
     ctx_release(c);
 }
 
+// ---- device memory cache -------------------------------------------------------
+// hipFree waits for the work of every stream on the device, so a solve that
+// frees its buffers while other contexts' streams are busy (a request server
+// has one context per worker) would wait for all of their kernels.  Freed
+// blocks go to a per-device cache instead, each with an event recorded on the
+// freeing context's stream, and are handed out again (same size class) once
+// that event has completed, i.e. once everything the context enqueued before
+// freeing the block has run.  Work on any other stream that used the block
+// must be complete, or ordered before that point, when it is freed (the
+// library's second streams are joined into the context's stream before their
+// buffers are released; hsps_ctx_set_stream drains the previous stream).
+// Sizes are rounded up to classes 2^k and 3 * 2^(k-1) (at most a third
+// wasted).  The cache keeps at most HSPS_MEM_CACHE_MB (default: a quarter of
+// the device's memory; 0 turns the cache off) and gives its blocks back with
+// hipFree when an allocation fails.
+namespace {
+struct CachedBlock {
+    void* p;
+    hipEvent_t ev;
+};
+struct MemCache {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::deque<CachedBlock>> blocks;  // (device, class) -> blocks, oldest first
+    std::map<int, std::vector<hipEvent_t>> events;                      // completed events, per device
+    std::unordered_map<void*, size_t> user;                             // hsps_malloc'd pointer -> class
+    std::map<int, size_t> bytes, cap;                                   // cached bytes and limit, per device
+};
+MemCache& mem_cache() {
+    static MemCache* m = new MemCache;  // never destroyed: cached blocks live until the process ends
+    return *m;
+}
+size_t size_class(size_t b) {
+    size_t c = 256;
+    while (c < b) {
+        const size_t h = c + c / 2;
+        if (h >= b) return h;
+        c *= 2;
+    }
+    return c;
+}
+size_t cache_cap(MemCache& M, int dev) {  // M.mu held, dev current
+    auto it = M.cap.find(dev);
+    if (it != M.cap.end()) return it->second;
+    size_t cap = 0;
+    if (const char* e = std::getenv("HSPS_MEM_CACHE_MB")) {
+        cap = (size_t)std::max(0LL, std::atoll(e)) << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) cap = tot / 4;
+        else (void)hipGetLastError();
+    }
+    M.cap[dev] = cap;
+    return cap;
+}
+// Gives the device's cached blocks back (hipFree), oldest first, until at most
+// keep bytes stay cached; only blocks whose event has completed unless all.
+void cache_trim(MemCache& M, int dev, size_t keep, bool all) {  // M.mu held, dev current
+    for (auto it = M.blocks.begin(); it != M.blocks.end() && M.bytes[dev] > keep;) {
+        if (it->first.first != dev) { ++it; continue; }
+        auto& q = it->second;
+        while (!q.empty() && M.bytes[dev] > keep) {
+            CachedBlock b = q.front();
+            if (all) {
+                (void)hipEventSynchronize(b.ev);
+            } else if (hipEventQuery(b.ev) != hipSuccess) {
+                (void)hipGetLastError();  // not ready: keep it
+                break;
+            }
+            q.pop_front();
+            M.events[dev].push_back(b.ev);
+            (void)hipFree(b.p);
+            M.bytes[dev] -= it->first.second;
+        }
+        it = q.empty() ? M.blocks.erase(it) : std::next(it);
+    }
+}
+}  // namespace
+
+// A block of at least b bytes on c's device: a cached one whose last user
+// has finished, else a new one.
+void* dev_alloc(Ctx* c, size_t b) {
+    c->set_current();
+    MemCache& M = mem_cache();
+    const size_t cls = size_class(b);
+    {
+        std::lock_guard<std::mutex> lk(M.mu);
+        auto it = M.blocks.find({c->device, cls});
+        if (it != M.blocks.end()) {
+            auto& q = it->second;
+            for (size_t k = 0; k < q.size() && k < 8; ++k) {  // oldest first: the likeliest done
+                if (hipEventQuery(q[k].ev) != hipSuccess) {
+                    (void)hipGetLastError();  // not ready
+                    continue;
+                }
+                void* p = q[k].p;
+                M.events[c->device].push_back(q[k].ev);
+                q.erase(q.begin() + (long)k);
+                M.bytes[c->device] -= cls;
+                return p;
+            }
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(M.mu);
+            cache_trim(M, c->device, 0, true);
+        }
+        e = hipMalloc(&p, cls);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(HSPS_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e));
+        }
+    }
+    return p;
+}
+
+// Returns a block of b bytes from dev_alloc to the cache, behind the work
+// already enqueued on c's stream.
+void dev_release(Ctx* c, void* p, size_t b) {
+    if (!p) return;
+    c->set_current();
+    MemCache& M = mem_cache();
+    const size_t cls = size_class(b);
+    std::lock_guard<std::mutex> lk(M.mu);
+    const size_t cap = cache_cap(M, c->device);
+    hipEvent_t ev = nullptr;
+    auto& evs = M.events[c->device];
+    if (!evs.empty()) {
+        ev = evs.back();
+        evs.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        ev = nullptr;
+    }
+    if (cap < cls || !ev || hipEventRecord(ev, c->stream) != hipSuccess) {  // no caching: the plain free
+        (void)hipGetLastError();
+        if (ev) evs.push_back(ev);
+        (void)hipFree(p);
+        return;
+    }
+    M.blocks[{c->device, cls}].push_back({p, ev});
+    M.bytes[c->device] += cls;
+    if (M.bytes[c->device] > cap) cache_trim(M, c->device, cap, false);
+}
+
 // ---- DevBuf ------------------------------------------------------------------
 DevBuf::DevBuf(Ctx* c, size_t b) : ctx(c), bytes(b) {
     if (b == 0) return;
-    c->set_current();
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-        p = nullptr;
-        (void)hipGetLastError();
-        throw Error(HSPS_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e));
-    }
+    p = dev_alloc(c, b);
     c->bytes_in_use += b;
 }
 DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
@@ -62,8 +209,12 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
 }
 void DevBuf::reset() {
     if (p) {
-        (void)hipFree(p);
-        if (ctx) ctx->bytes_in_use -= bytes;
+        if (ctx) {
+            dev_release(ctx, p, bytes);
+            ctx->bytes_in_use -= bytes;
+        } else {
+            (void)hipFree(p);
+        }
     }
     p = nullptr;
     bytes = 0;
@@ -398,6 +549,10 @@ int hsps_ctx_destroy(hsps_ctx ctx) {
 int hsps_ctx_set_stream(hsps_ctx ctx, void* stream) {
     return guarded([&] {
         if (!ctx) throw Error(HSPS_ERR_INVALID, "null context");
+        ctx->set_current();
+        // blocks freed from now on are fenced on the new stream only: the work
+        // enqueued on the previous one completes first
+        HSPS_HIP(hipStreamSynchronize(ctx->stream));
         ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
         return HSPS_OK;
     });
@@ -418,11 +573,13 @@ int hsps_malloc(hsps_ctx ctx, size_t bytes, void** d_ptr) {
         ctx->set_current();
         *d_ptr = nullptr;
         if (bytes == 0) return HSPS_OK;
-        hipError_t e = hipMalloc(d_ptr, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(HSPS_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        void* p = dev_alloc(ctx, bytes);
+        {
+            MemCache& M = mem_cache();
+            std::lock_guard<std::mutex> lk(M.mu);
+            M.user[p] = bytes;
         }
+        *d_ptr = p;
         ctx_retain(ctx);
         return HSPS_OK;
     });
@@ -433,7 +590,18 @@ int hsps_free(hsps_ctx ctx, void* d_ptr) {
         if (d_ptr) {
             ctx->set_current();
             HSPS_HIP(hipStreamSynchronize(ctx->stream));
-            HSPS_HIP(hipFree(d_ptr));
+            size_t bytes = 0;
+            {
+                MemCache& M = mem_cache();
+                std::lock_guard<std::mutex> lk(M.mu);
+                auto it = M.user.find(d_ptr);
+                if (it != M.user.end()) {
+                    bytes = it->second;
+                    M.user.erase(it);
+                }
+            }
+            if (bytes) dev_release(ctx, d_ptr, bytes);
+            else HSPS_HIP(hipFree(d_ptr));
             ctx_release(ctx);
         }
         return HSPS_OK;

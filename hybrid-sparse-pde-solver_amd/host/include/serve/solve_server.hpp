@@ -58,6 +58,7 @@ struct ServeResult {
     double done_s = 0;       ///< completion time, CLOCK_MONOTONIC seconds
     double setup_s = 0;      ///< preconditioner setups of all attempts (host and device)
     double solve_s = 0;      ///< Krylov solves of all attempts (a session attempt: join to end)
+    double classify_s = 0;   ///< admission -> ladder state chosen (features: device upload of A, Lanczos)
 };
 
 class SolveServer {

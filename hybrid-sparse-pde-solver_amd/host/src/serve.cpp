@@ -185,6 +185,7 @@ void SolveServer::start(const Req& r) {
     try {
         AdaptiveSelector sel(r->p);
         state = sel.select_initial_state(sel.extract_features(r->A));
+        r->res.classify_s = std::chrono::duration<double>(Clock::now() - r->t_admit).count();
     } catch (const std::exception& e) {
         HSPS_LOG_WARN("SolveServer: request ", r->id, " features failed: ", e.what());
         r->res.converged = false;
@@ -547,6 +548,7 @@ int hsps_server_wait(hsps_server S, long long ticket, double* x, hsps_serve_resu
             res->done_s = r.done_s;
             res->setup_s = r.setup_s;
             res->solve_s = r.solve_s;
+            res->classify_s = r.classify_s;
         }
         return r.converged ? 1 : 0;
     });

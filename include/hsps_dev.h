@@ -111,7 +111,8 @@ void hsps_params_default(hsps_params* p);
  *      parallel_context.cpp:31-50; one per device + stream) --------------- */
 int hsps_ctx_create(int device, hsps_ctx* out);
 int hsps_ctx_destroy(hsps_ctx ctx);
-/* Use an existing hipStream_t (NULL = the context's own stream). */
+/* Use an existing hipStream_t (NULL = the context's own stream); the work
+ * enqueued on the previous stream is drained first. */
 int hsps_ctx_set_stream(hsps_ctx ctx, void* hip_stream);
 void* hsps_ctx_stream(hsps_ctx ctx);
 int hsps_ctx_sync(hsps_ctx ctx);
@@ -119,6 +120,13 @@ int hsps_ctx_sync(hsps_ctx ctx);
 size_t hsps_ctx_bytes_in_use(hsps_ctx ctx);
 
 /* ---- device memory (replaces DeviceVector, cuda_backend.cpp:26-83) ------ */
+/* Device memory comes from a per-device cache of freed blocks (size classes
+ * 2^k and 3 * 2^(k-1)), because hipFree waits for the work of every stream on
+ * the device.  hsps_free drains the context's stream and returns the block to
+ * the cache; work on other streams that uses it must be complete by then.
+ * The library's own buffers go through the same cache, fenced by an event on
+ * the context's stream.  HSPS_MEM_CACHE_MB bounds the cached bytes per device
+ * (default: a quarter of its memory; 0 = plain hipMalloc / hipFree). */
 int hsps_malloc(hsps_ctx ctx, size_t bytes, void** d_ptr);
 int hsps_free(hsps_ctx ctx, void* d_ptr);
 int hsps_memcpy_h2d(hsps_ctx ctx, void* d_dst, const void* h_src, size_t bytes);

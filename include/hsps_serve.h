@@ -35,6 +35,7 @@ typedef struct {
     double done_s;        /* completion time, CLOCK_MONOTONIC seconds */
     double setup_s;       /* preconditioner setups of all attempts */
     double solve_s;       /* Krylov solves of all attempts (a session attempt: join to end) */
+    double classify_s;    /* admission -> ladder state chosen: the features (device upload of A, Lanczos) */
 } hsps_serve_result;
 
 /* workers: concurrent ladder solves; hbm_budget_bytes: admitted requests'
