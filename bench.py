@@ -36,6 +36,13 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# The serving leg runs one context (a stream of its own) per worker plus the
+# batch thread's.  HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware
+# queues (4 by default), and kernels of streams that share a queue run one
+# after another, so a small request would wait behind a large one: give the
+# streams queues of their own.  Read when HIP initialises, i.e. before the
+# first device call below; a value set by the caller is kept.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch of the FP64 SpMV on the 2048^2 Poisson matrix, from
 # two rocprofv3 --pmc runs of `bench.py --spmv-only` (FETCH_SIZE 154,831 KiB

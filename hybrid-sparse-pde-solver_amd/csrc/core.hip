@@ -63,7 +63,7 @@ void csr_release(hsps_csr_s* A) {
 namespace {
 struct CachedBlock {
     void* p;
-    hipEvent_t ev;
+    hipEvent_t ev;  // null: idle when cached
 };
 struct MemCache {
     std::mutex mu;
@@ -107,14 +107,15 @@ void cache_trim(MemCache& M, int dev, size_t keep, bool all) {  // M.mu held, de
         auto& q = it->second;
         while (!q.empty() && M.bytes[dev] > keep) {
             CachedBlock b = q.front();
-            if (all) {
+            if (!b.ev) {
+            } else if (all) {
                 (void)hipEventSynchronize(b.ev);
             } else if (hipEventQuery(b.ev) != hipSuccess) {
                 (void)hipGetLastError();  // not ready: keep it
                 break;
             }
             q.pop_front();
-            M.events[dev].push_back(b.ev);
+            if (b.ev) M.events[dev].push_back(b.ev);
             (void)hipFree(b.p);
             M.bytes[dev] -= it->first.second;
         }
@@ -135,12 +136,12 @@ void* dev_alloc(Ctx* c, size_t b) {
         if (it != M.blocks.end()) {
             auto& q = it->second;
             for (size_t k = 0; k < q.size() && k < 8; ++k) {  // oldest first: the likeliest done
-                if (hipEventQuery(q[k].ev) != hipSuccess) {
+                if (q[k].ev && hipEventQuery(q[k].ev) != hipSuccess) {
                     (void)hipGetLastError();  // not ready
                     continue;
                 }
                 void* p = q[k].p;
-                M.events[c->device].push_back(q[k].ev);
+                if (q[k].ev) M.events[c->device].push_back(q[k].ev);
                 q.erase(q.begin() + (long)k);
                 M.bytes[c->device] -= cls;
                 return p;
@@ -165,14 +166,20 @@ void* dev_alloc(Ctx* c, size_t b) {
 }
 
 // Returns a block of b bytes from dev_alloc to the cache, behind the work
-// already enqueued on c's stream.
-void dev_release(Ctx* c, void* p, size_t b) {
+// already enqueued on c's stream (idle: that work is known to be done).
+void dev_release(Ctx* c, void* p, size_t b, bool idle) {
     if (!p) return;
     c->set_current();
     MemCache& M = mem_cache();
     const size_t cls = size_class(b);
     std::lock_guard<std::mutex> lk(M.mu);
     const size_t cap = cache_cap(M, c->device);
+    if (idle && cap >= cls) {
+        M.blocks[{c->device, cls}].push_back({p, nullptr});
+        M.bytes[c->device] += cls;
+        if (M.bytes[c->device] > cap) cache_trim(M, c->device, cap, false);
+        return;
+    }
     hipEvent_t ev = nullptr;
     auto& evs = M.events[c->device];
     if (!evs.empty()) {
@@ -210,7 +217,7 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
 void DevBuf::reset() {
     if (p) {
         if (ctx) {
-            dev_release(ctx, p, bytes);
+            dev_release(ctx, p, bytes, false);
             ctx->bytes_in_use -= bytes;
         } else {
             (void)hipFree(p);
@@ -600,7 +607,7 @@ int hsps_free(hsps_ctx ctx, void* d_ptr) {
                     M.user.erase(it);
                 }
             }
-            if (bytes) dev_release(ctx, d_ptr, bytes);
+            if (bytes) dev_release(ctx, d_ptr, bytes, true);  // the stream is drained
             else HSPS_HIP(hipFree(d_ptr));
             ctx_release(ctx);
         }
