@@ -853,6 +853,13 @@ def main():
         return
     log("[bench] spmv roofline")
     roof, rn, rnnz = spmv_roofline(H, ctx, ev)
+    # achievable HBM streaming rate on this device (hsps_stream_probe: two FP32 reads and one write per
+    # value, nontemporal 16-byte accesses, 3 x 338 MB, well above the 256 MB Infinity Cache)
+    ach = C.c_double()
+    H.check(H.lib().hsps_stream_probe(ctx.h, 84434944, 20, C.byref(ach)))
+    achievable = {"GBs": ach.value, "kernel": "hsps_stream_probe: y = a r + b p over 84,434,944 FP32 values "
+                                               "(12 B/value, nontemporal 16-byte loads and stores), median of 20"}
+    log(f"[bench] achievable streaming rate {ach.value:.0f} GB/s")
     if args.spmv_only:
         nbytes = 256 << 20  # calibration copy: reads and writes exactly 256 MiB
         src, dst = ctx.array(n=nbytes // 8), ctx.array(n=nbytes // 8)
@@ -885,7 +892,8 @@ def main():
                  "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per dispatch, "
                                    "calibrated on a 256 MiB copy (profiles/spmv_poisson2048_fp64_fp32.md)",
                  "algorithmic_bytes": r64["algorithmic_bytes"], "avg_launch_us": r64["avg_launch_us"],
-                 "fp32_spmv": roof["fp32"]}
+                 "fp32_spmv": roof["fp32"],
+                 "achievable": dict(achievable, frac=r64["achieved_GBs"] / achievable["GBs"])}
     bi = res.get("batched_iteration")
     if bi and bi.get("full_load_achieved_GBs"):
         alg = bi["bytes_per_row_iteration"] * bi["tiled_rows"]
@@ -900,7 +908,8 @@ def main():
                                       f"({BATCHED_HBM_BYTES_PER_ROW_ITER:.1f} B vs {bi['bytes_per_row_iteration']} B "
                                       "algorithmic; profiles/batched512_pcg_jacobi_kernels.md)",
                     "algorithmic_bytes": alg, "avg_launch_us": bi["full_load_iteration_us"],
-                    "per": "one iteration (one launch each of K1 and K2)"}
+                    "per": "one iteration (one launch each of K1 and K2)",
+                    "achievable": dict(achievable, frac=bi["full_load_achieved_GBs"] / achievable["GBs"])}
         res["spmv_roofline"] = spmv_roof
     else:
         roofline = spmv_roof

@@ -112,6 +112,7 @@ SIGNATURES = {
     "hsps_batch_destroy": (_i, [_vp]),
     "hsps_batch_solve": (_i, [_vp, _vp, _vp, _PP, _ip, _ip, _dp]),
     "hsps_batch_storage": (_i, [_vp, _ip, _ip, _ip]),
+    "hsps_stream_probe": (_i, [_vp, C.c_longlong, _i, _dp]),
     "hsps_session_create": (_i, [_vp, _i, C.c_longlong, C.c_longlong, _PP, C.POINTER(_vp)]),
     "hsps_session_destroy": (_i, [_vp]),
     "hsps_session_add": (_i, [_vp, _i, _ip, _ip, _dp, _dp, _vp, _ip]),

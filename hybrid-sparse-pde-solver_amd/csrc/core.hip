@@ -653,6 +653,73 @@ int hsps_memset(hsps_ctx ctx, void* dst, int value, size_t bytes) {
     });
 }
 
+}  // extern "C"
+
+// Achievable HBM streaming rate: pn = 0.5 r + 0.25 p over FP32 vectors, two
+// reads and one write per value in 16-byte nontemporal accesses, four groups
+// per lane in flight (the fastest of the read-2 / write-1 forms measured on
+// MI355X beyond the Infinity Cache).
+constexpr int kProbeBlock = 256, kProbeU = 4, kProbeTile = kProbeBlock * kProbeU * 4;  // values per workgroup
+__global__ void __launch_bounds__(kProbeBlock) k_stream_probe(const float* __restrict__ r, const float* __restrict__ p,
+                                                              float* __restrict__ pn) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const long long base = (long long)blockIdx.x * kProbeTile;
+    f4 rv[kProbeU], pv[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) {
+        const long long g = base + (threadIdx.x + u * kProbeBlock) * 4;
+        rv[u] = __builtin_nontemporal_load(reinterpret_cast<const f4*>(r + g));
+        pv[u] = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p + g));
+    }
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) {
+        const long long g = base + (threadIdx.x + u * kProbeBlock) * 4;
+        __builtin_nontemporal_store(0.5f * rv[u] + 0.25f * pv[u], reinterpret_cast<f4*>(pn + g));
+    }
+}
+
+extern "C" {
+
+int hsps_stream_probe(hsps_ctx ctx, long long n, int reps, double* gbs) {
+    return guarded([&] {
+        if (!ctx || !gbs || reps < 1) throw Error(HSPS_ERR_INVALID, "hsps_stream_probe: bad argument");
+        const long long tiles = n / kProbeTile;
+        if (tiles < 1 || tiles > (1LL << 30)) throw Error(HSPS_ERR_INVALID, "hsps_stream_probe: n out of range");
+        const size_t bytes = (size_t)tiles * kProbeTile * sizeof(float);
+        ctx->set_current();
+        hipStream_t s = ctx->stream;
+        DevBuf r(ctx, bytes), p(ctx, bytes), pn(ctx, bytes);
+        HSPS_HIP(hipMemsetAsync(r.p, 0, bytes, s));
+        HSPS_HIP(hipMemsetAsync(p.p, 0, bytes, s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        std::vector<float> ms;
+        try {
+            HSPS_HIP(hipEventCreate(&e0));
+            HSPS_HIP(hipEventCreate(&e1));
+            for (int k = -3; k < reps; ++k) {  // three untimed launches first
+                HSPS_HIP(hipEventRecord(e0, s));
+                hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)tiles), dim3(kProbeBlock), 0, s, r.as<float>(),
+                                   p.as<float>(), pn.as<float>());
+                HSPS_CHECK_LAUNCH();
+                HSPS_HIP(hipEventRecord(e1, s));
+                HSPS_HIP(hipEventSynchronize(e1));
+                float t = 0.0f;
+                HSPS_HIP(hipEventElapsedTime(&t, e0, e1));
+                if (k >= 0) ms.push_back(t);
+            }
+        } catch (...) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::sort(ms.begin(), ms.end());
+        *gbs = 3.0 * (double)bytes / (ms[ms.size() / 2] * 1e-3) * 1e-9;
+        return HSPS_OK;
+    });
+}
+
 int hsps_csr_create(hsps_ctx ctx, int rows, int cols, int nnz, const int* rp, const int* ci, const double* v,
                     hsps_csr* out) {
     return guarded([&] {

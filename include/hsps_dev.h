@@ -133,6 +133,13 @@ int hsps_memcpy_h2d(hsps_ctx ctx, void* d_dst, const void* h_src, size_t bytes);
 int hsps_memcpy_d2h(hsps_ctx ctx, void* h_dst, const void* d_src, size_t bytes);
 int hsps_memcpy_d2d(hsps_ctx ctx, void* d_dst, const void* d_src, size_t bytes);
 int hsps_memset(hsps_ctx ctx, void* d_dst, int value, size_t bytes);
+/* Diagnostic (no reference counterpart): the achievable HBM streaming rate
+ * against which the HBM-bound kernels are judged.  Times reps launches of
+ * y = a r + b p over n FP32 values (two reads and one write per value,
+ * 16-byte nontemporal accesses; n rounded down to a multiple of 4096) on the
+ * context's stream and returns the median rate in GB/s of moved bytes.  With
+ * 12 n bytes well above the 256 MB Infinity Cache it measures HBM. */
+int hsps_stream_probe(hsps_ctx ctx, long long n, int reps, double* gbs);
 
 /* ---- CSR matrix (SparseMatrix, sparse_matrix.hpp:34-170; int32 indices,
  *      sorted columns).  Uploads FP64 values, an FP32 copy, the diagonal and
