@@ -330,7 +330,8 @@ class Precond:
         return check(lib().hsps_amg_levels(self.h))
 
     def ilu_sweeps(self):
-        """((forward kind, W), (backward kind, W)): kind 1 = band wavefront, 0 = levels."""
+        """((forward kind, W), (backward kind, W)): kind 1 = band wavefront, 0 = levels,
+        2 = nothing to sweep (forward, no entry below the diagonal: y = r)."""
         v = [C.c_int() for _ in range(4)]
         check(lib().hsps_ilu_sweep_kinds(self.h, *[C.byref(x) for x in v]))
         return (v[0].value, v[2].value), (v[1].value, v[3].value)

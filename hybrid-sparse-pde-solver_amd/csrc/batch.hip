@@ -2210,6 +2210,15 @@ static void launch_chunk(Batch& B, const BArgs& a, int w, int chunk, const Activ
         if (graph) (void)hipGraphDestroy(graph);
         if (ei != hipSuccess) {
             (void)hipGetLastError();
+            // an invalidated capture may leave the stream capturing: end it
+            // before launching directly
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(s, &g2);
+                if (g2) (void)hipGraphDestroy(g2);
+            }
+            (void)hipGetLastError();
             ++B.capture_failures;
             direct();
             return;

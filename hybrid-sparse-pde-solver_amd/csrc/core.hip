@@ -24,12 +24,14 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void ctx_retain(Ctx* c) { c->refs.fetch_add(1); }
+void dev_release_all(Ctx* c);
 void ctx_release(Ctx* c) {
     if (c->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->red_partials.reset();
     c->red_out.reset();
+    dev_release_all(c);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete static_cast<hsps_ctx_s*>(c);
@@ -49,28 +51,37 @@ void csr_release(hsps_csr_s* A) {
 // hipFree waits for the work of every stream on the device, so a solve that
 // frees its buffers while other contexts' streams are busy (a request server
 // has one context per worker) would wait for all of their kernels.  Freed
-// blocks go to a per-device cache instead, each with an event recorded on the
-// freeing context's stream, and are handed out again (same size class) once
-// that event has completed, i.e. once everything the context enqueued before
-// freeing the block has run.  Work on any other stream that used the block
-// must be complete, or ordered before that point, when it is freed (the
-// library's second streams are joined into the context's stream before their
-// buffers are released; hsps_ctx_set_stream drains the previous stream).
-// Sizes are rounded up to classes 2^k and 3 * 2^(k-1) (at most a third
-// wasted).  The cache keeps at most HSPS_MEM_CACHE_MB (default: a quarter of
-// the device's memory; 0 turns the cache off) and gives its blocks back with
-// hipFree when an allocation fails.
+// blocks are cached instead:
+//  * a block a context frees stays with that context, tagged with an event
+//    recorded on its stream, and is handed out again (same size class) to
+//    that context once the event has completed, i.e. once everything the
+//    context enqueued before freeing it has run.  Only the owning context
+//    queries its blocks' events, so no thread ever touches another thread's
+//    stream (a query of an event whose stream is being captured into a graph
+//    would disturb the capture);
+//  * blocks known to be idle (hsps_free drains the stream first; a destroyed
+//    context's blocks once its stream is drained) go to a per-device pool
+//    any context may take from.
+// Work on any other stream that used a block must be complete, or ordered
+// before that point, when it is freed (the library's second streams are
+// joined into the context's stream before their buffers are released;
+// hsps_ctx_set_stream drains the previous stream).  Sizes are rounded up to
+// classes 2^k and 3 * 2^(k-1) (at most a third wasted).  The cache keeps at
+// most HSPS_MEM_CACHE_MB per device (default: a quarter of the device's
+// memory; 0 turns it off) and gives blocks back with hipFree when an
+// allocation fails.
 namespace {
 struct CachedBlock {
     void* p;
-    hipEvent_t ev;  // null: idle when cached
+    hipEvent_t ev;
 };
 struct MemCache {
     std::mutex mu;
-    std::map<std::pair<int, size_t>, std::deque<CachedBlock>> blocks;  // (device, class) -> blocks, oldest first
-    std::map<int, std::vector<hipEvent_t>> events;                      // completed events, per device
-    std::unordered_map<void*, size_t> user;                             // hsps_malloc'd pointer -> class
-    std::map<int, size_t> bytes, cap;                                   // cached bytes and limit, per device
+    std::map<std::pair<const Ctx*, size_t>, std::deque<CachedBlock>> owned;  // (context, class), oldest first
+    std::map<std::pair<int, size_t>, std::vector<void*>> idle;               // (device, class)
+    std::map<int, std::vector<hipEvent_t>> events;                           // completed events, per device
+    std::unordered_map<void*, size_t> user;                                  // hsps_malloc'd pointer -> bytes
+    std::map<int, size_t> bytes, cap;                                        // cached bytes and limit, per device
 };
 MemCache& mem_cache() {
     static MemCache* m = new MemCache;  // never destroyed: cached blocks live until the process ends
@@ -99,53 +110,71 @@ size_t cache_cap(MemCache& M, int dev) {  // M.mu held, dev current
     M.cap[dev] = cap;
     return cap;
 }
-// Gives the device's cached blocks back (hipFree), oldest first, until at most
-// keep bytes stay cached; only blocks whose event has completed unless all.
-void cache_trim(MemCache& M, int dev, size_t keep, bool all) {  // M.mu held, dev current
-    for (auto it = M.blocks.begin(); it != M.blocks.end() && M.bytes[dev] > keep;) {
+// Gives idle blocks of the device back (hipFree) until at most keep bytes
+// stay cached; with c, also c's own blocks whose event has completed (all of
+// them, waiting for their events, when all is set).
+void cache_trim(MemCache& M, int dev, const Ctx* c, size_t keep, bool all) {  // M.mu held, dev current
+    for (auto it = M.idle.begin(); it != M.idle.end() && M.bytes[dev] > keep;) {
         if (it->first.first != dev) { ++it; continue; }
+        auto& v = it->second;
+        while (!v.empty() && M.bytes[dev] > keep) {
+            (void)hipFree(v.back());
+            v.pop_back();
+            M.bytes[dev] -= it->first.second;
+        }
+        it = v.empty() ? M.idle.erase(it) : std::next(it);
+    }
+    if (!c) return;
+    for (auto it = M.owned.begin(); it != M.owned.end() && M.bytes[dev] > keep;) {
+        if (it->first.first != c) { ++it; continue; }
         auto& q = it->second;
         while (!q.empty() && M.bytes[dev] > keep) {
             CachedBlock b = q.front();
-            if (!b.ev) {
-            } else if (all) {
+            if (all) {
                 (void)hipEventSynchronize(b.ev);
             } else if (hipEventQuery(b.ev) != hipSuccess) {
                 (void)hipGetLastError();  // not ready: keep it
                 break;
             }
             q.pop_front();
-            if (b.ev) M.events[dev].push_back(b.ev);
+            M.events[dev].push_back(b.ev);
             (void)hipFree(b.p);
             M.bytes[dev] -= it->first.second;
         }
-        it = q.empty() ? M.blocks.erase(it) : std::next(it);
+        it = q.empty() ? M.owned.erase(it) : std::next(it);
     }
 }
 }  // namespace
 
-// A block of at least b bytes on c's device: a cached one whose last user
-// has finished, else a new one.
+// A block of at least b bytes for c: one of c's cached blocks whose last
+// user has finished, else an idle one, else a new one.
 void* dev_alloc(Ctx* c, size_t b) {
     c->set_current();
     MemCache& M = mem_cache();
     const size_t cls = size_class(b);
     {
         std::lock_guard<std::mutex> lk(M.mu);
-        auto it = M.blocks.find({c->device, cls});
-        if (it != M.blocks.end()) {
+        auto it = M.owned.find({c, cls});
+        if (it != M.owned.end()) {
             auto& q = it->second;
             for (size_t k = 0; k < q.size() && k < 8; ++k) {  // oldest first: the likeliest done
-                if (q[k].ev && hipEventQuery(q[k].ev) != hipSuccess) {
+                if (hipEventQuery(q[k].ev) != hipSuccess) {
                     (void)hipGetLastError();  // not ready
                     continue;
                 }
                 void* p = q[k].p;
-                if (q[k].ev) M.events[c->device].push_back(q[k].ev);
+                M.events[c->device].push_back(q[k].ev);
                 q.erase(q.begin() + (long)k);
                 M.bytes[c->device] -= cls;
                 return p;
             }
+        }
+        auto jt = M.idle.find({c->device, cls});
+        if (jt != M.idle.end() && !jt->second.empty()) {
+            void* p = jt->second.back();
+            jt->second.pop_back();
+            M.bytes[c->device] -= cls;
+            return p;
         }
     }
     void* p = nullptr;
@@ -154,7 +183,7 @@ void* dev_alloc(Ctx* c, size_t b) {
         (void)hipGetLastError();
         {
             std::lock_guard<std::mutex> lk(M.mu);
-            cache_trim(M, c->device, 0, true);
+            cache_trim(M, c->device, c, 0, true);
         }
         e = hipMalloc(&p, cls);
         if (e != hipSuccess) {
@@ -165,8 +194,9 @@ void* dev_alloc(Ctx* c, size_t b) {
     return p;
 }
 
-// Returns a block of b bytes from dev_alloc to the cache, behind the work
-// already enqueued on c's stream (idle: that work is known to be done).
+// Returns a block of b bytes from dev_alloc to the cache: c's own, behind the
+// work already enqueued on c's stream, or the idle pool when that work is
+// known to be done.
 void dev_release(Ctx* c, void* p, size_t b, bool idle) {
     if (!p) return;
     c->set_current();
@@ -174,30 +204,48 @@ void dev_release(Ctx* c, void* p, size_t b, bool idle) {
     const size_t cls = size_class(b);
     std::lock_guard<std::mutex> lk(M.mu);
     const size_t cap = cache_cap(M, c->device);
-    if (idle && cap >= cls) {
-        M.blocks[{c->device, cls}].push_back({p, nullptr});
-        M.bytes[c->device] += cls;
-        if (M.bytes[c->device] > cap) cache_trim(M, c->device, cap, false);
-        return;
-    }
-    hipEvent_t ev = nullptr;
-    auto& evs = M.events[c->device];
-    if (!evs.empty()) {
-        ev = evs.back();
-        evs.pop_back();
-    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-        (void)hipGetLastError();
-        ev = nullptr;
-    }
-    if (cap < cls || !ev || hipEventRecord(ev, c->stream) != hipSuccess) {  // no caching: the plain free
-        (void)hipGetLastError();
-        if (ev) evs.push_back(ev);
+    if (cap < cls) {  // no caching: the plain free
         (void)hipFree(p);
         return;
     }
-    M.blocks[{c->device, cls}].push_back({p, ev});
+    if (idle) {
+        M.idle[{c->device, cls}].push_back(p);
+    } else {
+        hipEvent_t ev = nullptr;
+        auto& evs = M.events[c->device];
+        if (!evs.empty()) {
+            ev = evs.back();
+            evs.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            ev = nullptr;
+        }
+        if (!ev || hipEventRecord(ev, c->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            if (ev) evs.push_back(ev);
+            (void)hipFree(p);
+            return;
+        }
+        M.owned[{c, cls}].push_back({p, ev});
+    }
     M.bytes[c->device] += cls;
-    if (M.bytes[c->device] > cap) cache_trim(M, c->device, cap, false);
+    if (M.bytes[c->device] > cap) cache_trim(M, c->device, c, cap, false);
+}
+
+// A context going away: its stream is drained, so its cached blocks become
+// idle for every other context.
+void dev_release_all(Ctx* c) {
+    MemCache& M = mem_cache();
+    std::lock_guard<std::mutex> lk(M.mu);
+    for (auto it = M.owned.begin(); it != M.owned.end();) {
+        if (it->first.first != c) { ++it; continue; }
+        for (const CachedBlock& b : it->second) {
+            (void)hipEventSynchronize(b.ev);
+            M.events[c->device].push_back(b.ev);
+            M.idle[{c->device, it->first.second}].push_back(b.p);
+        }
+        it = M.owned.erase(it);
+    }
 }
 
 // ---- DevBuf ------------------------------------------------------------------

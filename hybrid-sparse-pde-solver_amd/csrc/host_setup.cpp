@@ -61,12 +61,12 @@ HostCsr ilu0(const HostCsr& A) {
 
 // ILUT(tau, p), ilu.cpp:94-220.  The reference scans every column k < i of
 // a dense work row; the same arithmetic in the same order visits only the
-// active columns, found as the set bits of a bitmap in increasing order (an
-// elimination of k only activates columns j > k, so a forward scan of the
-// bitmap meets every pivot once, in order, including those activated during
-// the scan), and resets only the columns the row touched.  The kept upper
-// entries come out of the same bitmap already sorted.  Same factors, bit for
-// bit; O(nnz(L\U) * row length + band / 64) per row instead of O(n).
+// row's active columns: the lower ones in a small sorted list, walked in
+// increasing order while an elimination of k inserts the columns it
+// activates (always > k, so behind the walk's position), and the upper ones
+// collected and sorted at the end.  A bitmap answers "active?" in O(1), and
+// only the columns the row touched are reset.  Same factors, bit for bit;
+// O(nnz(L\U) * row length) per row instead of O(n).
 HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     const int n = A.rows;
     // finished rows, flat: L strictly lower, U from the diagonal on
@@ -80,7 +80,7 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     std::vector<double> w(n, 0.0), udiag(n, 0.0);
     std::vector<unsigned long long> act((size_t)n / 64 + 1, 0);  // active columns of the work row
     std::vector<char> seen(n, 0);
-    std::vector<int> touched;
+    std::vector<int> touched, lower, upper;
     std::vector<std::pair<int, double>> Lr, Ur;
     auto on = [&](int j) { return (act[(size_t)j >> 6] >> (j & 63)) & 1ull; };
     auto set = [&](int j) { act[(size_t)j >> 6] |= 1ull << (j & 63); };
@@ -90,31 +90,27 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     };
     for (int i = 0; i < n; ++i) {
         touched.clear();
+        lower.clear();
+        upper.clear();
         Lr.clear();
         Ur.clear();
         double rn = 0.0;
-        int hi = i;  // largest column touched
         for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
             const int j = A.ci[p];
             w[j] = A.v[p];
-            set(j);
+            if (!on(j)) {
+                set(j);
+                if (j < i) lower.push_back(j);
+                else if (j > i) upper.push_back(j);
+            }
             touch(j);
             rn += A.v[p] * A.v[p];
-            hi = std::max(hi, j);
         }
+        std::sort(lower.begin(), lower.end());
         const double tau = tau_rel * std::sqrt(rn);
         // pivot columns k < i in increasing order (ilu.cpp:121-146)
-        int pos = A.rp[i] < A.rp[i + 1] ? std::min(A.ci[A.rp[i]], i) : i;
-        while (pos < i) {
-            const size_t wi = (size_t)pos >> 6;
-            const unsigned long long word = act[wi] & (~0ull << (pos & 63));
-            if (!word) {
-                pos = (int)((wi + 1) << 6);
-                continue;
-            }
-            const int k = (int)(wi << 6) + __builtin_ctzll(word);
-            if (k >= i) break;
-            pos = k + 1;
+        for (size_t t = 0; t < lower.size(); ++t) {
+            const int k = lower[t];
             if (std::abs(w[k]) < tau) { w[k] = 0.0; clr(k); continue; }
             const double ukk = udiag[k];
             if (std::abs(ukk) < kEps) { clr(k); continue; }
@@ -123,25 +119,22 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
             for (long long q = up[k]; q < up[k + 1]; ++q) {
                 const int j = uc[q];
                 if (j <= k) continue;
-                if (!on(j) && std::abs(lik * uv[q]) >= tau) set(j);
+                if (!on(j) && std::abs(lik * uv[q]) >= tau) {
+                    set(j);
+                    if (j < i) lower.insert(std::upper_bound(lower.begin() + (long)t + 1, lower.end(), j), j);
+                    else if (j > i) upper.push_back(j);
+                }
                 w[j] -= lik * uv[q];
                 touch(j);
-                hi = std::max(hi, j);
             }
             if (std::abs(w[k]) >= tau) Lr.emplace_back(k, w[k]);  // ilu.cpp:151-155, ascending k
         }
         double d = w[i];
         if (std::abs(d) < kEps) d = (d >= 0 ? 1.0 : -1.0) * kEps * 1e6;
         Ur.emplace_back(i, d);
-        for (size_t wi = (size_t)(i + 1) >> 6; i + 1 <= hi && wi <= (size_t)hi >> 6; ++wi) {
-            unsigned long long word = act[wi];
-            if (wi == (size_t)(i + 1) >> 6) word &= ~0ull << ((i + 1) & 63);
-            while (word) {
-                const int j = (int)(wi << 6) + __builtin_ctzll(word);
-                word &= word - 1;
-                if (std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
-            }
-        }
+        std::sort(upper.begin(), upper.end());
+        for (int j : upper)
+            if (std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
         for (int j : touched) {
             w[j] = 0.0;
             clr(j);

@@ -457,26 +457,38 @@ static void setup_ilu(Precond& M, const Csr& A, double drop_tol, int fill_p) {
             throw Error(HSPS_ERR_PIVOT, "ILU backward sub: near-zero U diagonal at row " + std::to_string(i));
         invd[i] = 1.0 / d;
     }
-    int nl_lo = 0, nl_up = 0;
-    auto lo = hsps_host::tri_levels(F, true, &nl_lo);
-    auto up = hsps_host::tri_levels(F, false, &nl_up);
-    f->fwd = make_schedule(M.ctx, lo, nl_lo);
-    f->bwd = make_schedule(M.ctx, up, nl_up);
-    f->fwd_ell = make_tri_ell(M.ctx, F, lo, f->fwd, true, invd);
-    f->bwd_ell = make_tri_ell(M.ctx, F, up, f->bwd, false, invd);
-    f->inv_diag64 = upload(M.ctx, invd.data(), invd.size() * sizeof(double));
-    f->inv_diag32 = upload_f32(M.ctx, invd);
     f->tmp64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
     f->tmp32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
-    f->pos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
-    f->pos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
-    f->opos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
-    f->opos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
     f->fwd_band = make_tri_band(M.ctx, F, true, invd);
     f->bwd_band = make_tri_band(M.ctx, F, false, invd);
     f->band_err = DevBuf(M.ctx, sizeof(int));
     fill_on(M.ctx->stream, f->band_err.p, 0, sizeof(int));
-    f->lu = to_dev(M.ctx, std::move(F));
+    f->l_empty = 1;
+    for (int i = 0; i < F.rows && f->l_empty; ++i)
+        for (int k = F.rp[i]; k < F.rp[i + 1]; ++k)
+            if (F.ci[k] < i) {
+                f->l_empty = 0;
+                break;
+            }
+    // the level-scheduled sweeps (and the factor on the device) only when a
+    // sweep with work is not taken by a band layout, which holds everything
+    // its sweep reads
+    if (!((f->fwd_band.ok || f->l_empty) && f->bwd_band.ok)) {
+        int nl_lo = 0, nl_up = 0;
+        auto lo = hsps_host::tri_levels(F, true, &nl_lo);
+        auto up = hsps_host::tri_levels(F, false, &nl_up);
+        f->fwd = make_schedule(M.ctx, lo, nl_lo);
+        f->bwd = make_schedule(M.ctx, up, nl_up);
+        f->fwd_ell = make_tri_ell(M.ctx, F, lo, f->fwd, true, invd);
+        f->bwd_ell = make_tri_ell(M.ctx, F, up, f->bwd, false, invd);
+        f->inv_diag64 = upload(M.ctx, invd.data(), invd.size() * sizeof(double));
+        f->inv_diag32 = upload_f32(M.ctx, invd);
+        f->pos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
+        f->pos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
+        f->opos64 = DevBuf(M.ctx, (size_t)F.rows * sizeof(double));
+        f->opos32 = DevBuf(M.ctx, (size_t)F.rows * sizeof(float));
+        f->lu = to_dev(M.ctx, std::move(F));
+    }
     M.ilu = std::move(f);
 }
 
@@ -699,7 +711,15 @@ static void apply_t(const Precond& M, const T* r, T* z, hipStream_t s, const int
             const bool dbl = std::is_same<T, double>::value;
             T* y = dbl ? (T*)f.tmp64.p : (T*)f.tmp32.p;
             const T* invd = dbl ? (const T*)f.inv_diag64.p : (const T*)f.inv_diag32.p;
-            if (f.fwd_ell.K && f.bwd_ell.K) {
+            if ((f.fwd_band.ok || f.l_empty) && f.bwd_band.ok) {
+                int* err = f.band_err.as<int>();
+                const T* yin = r;  // unit L with nothing below the diagonal: y = r exactly
+                if (!f.l_empty) {
+                    tri_band_sweep<T>(f.fwd_band, false, n, r, y, err, s, skip);
+                    yin = y;
+                }
+                tri_band_sweep<T>(f.bwd_band, true, n, yin, z, err, s, skip);
+            } else if (f.fwd_ell.K && f.bwd_ell.K) {
                 T* ip = dbl ? (T*)f.pos64.p : (T*)f.pos32.p;
                 T* op = dbl ? (T*)f.opos64.p : (T*)f.opos32.p;
                 int* err = f.band_err.as<int>();
@@ -917,7 +937,7 @@ int hsps_amg_aggregates(hsps_precond M, int lv, int* out) {
 int hsps_ilu_sweep_kinds(hsps_precond M, int* fwd, int* bwd, int* w_fwd, int* w_bwd) {
     if (!M || M->kind != HSPS_PRECOND_ILU || !M->ilu) return HSPS_ERR_INVALID;
     const IluFactor& f = *M->ilu;
-    if (fwd) *fwd = f.fwd_band.ok;
+    if (fwd) *fwd = f.fwd_band.ok ? 1 : f.l_empty && f.bwd_band.ok ? 2 : 0;  // as the apply chooses
     if (bwd) *bwd = f.bwd_band.ok;
     if (w_fwd) *w_fwd = f.fwd_band.ok ? f.fwd_band.W : 0;
     if (w_bwd) *w_bwd = f.bwd_band.ok ? f.bwd_band.W : 0;

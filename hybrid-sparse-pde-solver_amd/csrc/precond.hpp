@@ -45,6 +45,7 @@ struct IluFactor {
     DevBuf pos64, pos32;      // a sweep's input in level order
     DevBuf opos64, opos32;    // a narrow sweep's output in level order
     TriBand fwd_band, bwd_band;  // wavefront sweeps, when the factor is banded (else ok = 0)
+    int l_empty = 0;             // no entry below the diagonal: the forward sweep is y = r
     DevBuf band_err;             // int: set if a band sweep's wait gave up
 };
 

@@ -213,9 +213,10 @@ int hsps_amg_level_matrix(hsps_precond M, int lv, int which, int* h_row_ptr, int
 int hsps_amg_setup_profile(hsps_precond M, double* h_out6);
 /* How an ILU preconditioner's two sweeps run (no reference counterpart;
  * diagnostic): 0 = level-scheduled, 1 = wavefront over a banded grid layout
- * (the factor's dependencies within a band of W rows, short histories);
- * *w_fwd / *w_bwd get the band width W (0 for level sweeps).  Both give the
- * same bits. */
+ * (the factor's dependencies within a band of W rows, short histories),
+ * 2 = nothing to sweep (*fwd only: no entry below the diagonal, y = r);
+ * *w_fwd / *w_bwd get the band width W (0 otherwise).  All give the same
+ * bits. */
 int hsps_ilu_sweep_kinds(hsps_precond M, int* fwd, int* bwd, int* w_fwd, int* w_bwd);
 
 /* ---- Krylov solves (SolverBase::solve, solver_base.hpp:32-35;

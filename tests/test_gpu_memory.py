@@ -19,15 +19,23 @@ def _malloc(H, ctx, nbytes):
     return p
 
 
-def test_freed_block_is_reused_by_its_size_class(H, ctx):
-    p1 = _malloc(H, ctx, 1 << 20)
-    H.check(H.lib().hsps_free(ctx.h, p1))
-    p2 = _malloc(H, ctx, (1 << 20) - 4096)  # same class (2^20): the drained block comes back
-    assert p2.value == p1.value
-    p3 = _malloc(H, ctx, 1 << 22)  # another class
-    assert p3.value != p2.value
-    H.check(H.lib().hsps_free(ctx.h, p2))
-    H.check(H.lib().hsps_free(ctx.h, p3))
+def test_freed_block_is_reused_by_its_size_class(H):
+    """hsps_free drains the context's stream and parks the block in the idle
+    pool; a context that owns no cached block of that size class takes the
+    last block parked there (a fresh context: the earlier tests' contexts own
+    blocks of their own)."""
+    c = H.Context(0)
+    try:
+        p1 = _malloc(H, c, 3 << 23)
+        H.check(H.lib().hsps_free(c.h, p1))
+        p2 = _malloc(H, c, (3 << 23) - 4096)  # same class (3 * 2^23): the drained block comes back
+        assert p2.value == p1.value
+        p3 = _malloc(H, c, 1 << 25)  # another class
+        assert p3.value != p2.value
+        H.check(H.lib().hsps_free(c.h, p2))
+        H.check(H.lib().hsps_free(c.h, p3))
+    finally:
+        c.close()
 
 
 def test_free_does_not_wait_for_another_contexts_stream(H, ctx):
