@@ -4,8 +4,9 @@ Bars: every system's true FP64 rel-res <= tol; its solution within 1e-6
 relative of the oracle's single-system PCG+Jacobi solution; iteration counts
 within 5% (FP64) / 10% (mixed, reliable-update refinement) of the oracle's;
 and a system's result is bit-identical alone and inside a mixed batch (same
-tile size), since tiles never straddle systems and each system's reductions
-run over its own tiles in a fixed order.
+tile size, and for a tiled five-point system the same first row modulo 4),
+since tiles never straddle systems and each system's reductions run over its
+own tiles in a fixed order.
 """
 import numpy as np
 import pytest
@@ -148,6 +149,17 @@ def wide_stencil(n, w):
     import scipy.sparse as sp
     return csr_of(sp.diags([-np.ones(n - w), -np.ones(n - 1), 4 * np.ones(n), -np.ones(n - 1), -np.ones(n - w)],
                            [-w, -1, 0, 1, w]))
+
+
+def grid_2d(W, Hg, eps):
+    """Anisotropic five-point operator on a grid of Hg rows of W points:
+    -eps to the west and east, -1 to the north and south, 2 + 2 eps on the
+    diagonal, no coupling across a grid-row end (stencil storage; 2-D tiles
+    when W > 512)."""
+    import scipy.sparse as sp
+    Tx = sp.diags([-eps * np.ones(W - 1), 2 * eps * np.ones(W), -eps * np.ones(W - 1)], [-1, 0, 1])
+    Ty = sp.diags([-np.ones(Hg - 1), 2 * np.ones(Hg), -np.ones(Hg - 1)], [-1, 0, 1])
+    return csr_of(sp.kron(sp.identity(Hg), Tx) + sp.kron(Ty, sp.identity(W)))
 
 
 def tiled_systems(H):
@@ -297,3 +309,50 @@ def test_batch_grid_shrinks_to_live_systems(H, ctx, precision):
         ok1, it1, _, _ = Bs.solve(bs, x1, p)
         assert ok1 and it1[0] == it_mix[s], (s, it1[0], it_mix[s])
         np.testing.assert_array_equal(x1.download(), xs)
+
+
+def grid_systems(H):
+    """Five-point systems on full grids 516, 1,028 and 600 points wide (the
+    wide-halo five-point kernels), grid heights that are not multiples of 8,
+    and small systems between them so that they start at rows 1, 2 and 3
+    modulo 4 (16-byte groups that straddle a system's first row); then a
+    narrow grid and a wide stencil that is not a grid."""
+    arrs = [H.poisson_2d(31), grid_2d(516, 41, 0.3), H.poisson_2d(5), grid_2d(1028, 21, 1.0), H.poisson_2d(3),
+            grid_2d(600, 50, 0.05), H.poisson_2d(150), wide_stencil(30000, 2000)]
+    oracles = [O.Csr.from_csr(len(rp) - 1, len(rp) - 1, rp, ci, v) for rp, ci, v in arrs]
+    bs = [A.spmv(O.uniform(700 + s, A.rows)) for s, A in enumerate(oracles)]
+    return arrs, oracles, bs
+
+
+GRIDS = [1, 3, 5]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_wide_grids_at_unaligned_rows(H, ctx, precision):
+    """Wide full grids starting at rows 1, 2, 3 mod 4 against the oracle; the
+    same system alone (first row 0) converges alike.  (Its bits may differ:
+    the five-point kernels deal rows to lanes by 16-byte groups, so a
+    system's dot products keep their order at a fixed start modulo 4.)"""
+    arrs, oracles, bs = grid_systems(H)
+    p = H.default_params(tol=1e-8, max_iter=5000, precision=precision, batch_tile_rows=512)
+    B = H.Batch.from_systems(ctx, arrs)
+    assert B.storage()["stencil"] == len(arrs)
+    assert [B.row_off[k] % 4 for k in GRIDS] == [1, 2, 3]
+    bd, xd = ctx.array(np.concatenate(bs)), ctx.array(n=B.n)
+    ok, iters, conv, tr = B.solve(bd, xd, p)
+    x = xd.download()
+    assert ok and conv.all(), (iters, conv, tr)
+    for s, A in enumerate(oracles):
+        xs_dev = x[B.row_off[s]:B.row_off[s + 1]]
+        rel = np.linalg.norm(bs[s] - A.spmv(xs_dev)) / np.linalg.norm(bs[s])
+        assert rel < 1e-8 and abs(rel - tr[s]) <= 1e-3 * rel + 1e-16, (s, rel, tr[s])
+        xo, st = O.solve(A, bs[s], "cg", "jacobi", tol=1e-8, max_iter=5000)
+        assert np.linalg.norm(xs_dev - xo) <= 1e-6 * np.linalg.norm(xo), s
+        assert abs(iters[s] - st.iterations) <= max(1, 0.05 * st.iterations), (s, precision, iters[s], st.iterations)
+    for s in GRIDS:
+        Bs = H.Batch.from_systems(ctx, [arrs[s]])
+        b1, x1 = ctx.array(bs[s]), ctx.array(n=len(bs[s]))
+        ok1, it1, _, _ = Bs.solve(b1, x1, p)
+        assert ok1 and abs(int(it1[0]) - int(iters[s])) <= 1, (s, it1[0], iters[s])
+        xa = x[B.row_off[s]:B.row_off[s + 1]]
+        assert np.linalg.norm(x1.download() - xa) <= 1e-6 * np.linalg.norm(xa), s

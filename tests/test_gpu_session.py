@@ -12,14 +12,14 @@ import numpy as np
 import pytest
 
 import oracle as O
-from test_gpu_batch import graph_spd, varcoef_2d, wide_stencil
+from test_gpu_batch import graph_spd, grid_2d, varcoef_2d, wide_stencil
 
 pytestmark = pytest.mark.gpu
 
 
 def systems(H):
     arrs = [H.poisson_2d(160), graph_spd(24000, 1), varcoef_2d(H, 150, 5), H.poisson_2d(30), graph_spd(500, 2),
-            wide_stencil(30000, 2000), H.anisotropic_2d(120, 0.02)]
+            wide_stencil(30000, 2000), H.anisotropic_2d(120, 0.02), grid_2d(600, 50, 0.05)]
     oracles = [O.Csr.from_csr(len(rp) - 1, len(rp) - 1, rp, ci, v) for rp, ci, v in arrs]
     bs = [A.spmv(O.uniform(500 + s, A.rows)) for s, A in enumerate(oracles)]
     return arrs, oracles, bs
@@ -49,10 +49,10 @@ def test_session_join_mid_solve_equals_alone(H, ctx, precision):
     res, pending = {}, {}
     # system 0 starts; the others join one or a few chunks later, mid-solve
     pending[S.add(*arrs[0], bs[0])] = 0
-    order = [1, 2, 3, 4, 5, 6]
+    order = [1, 2, 3, 4, 5, 6, 7]
     step = 0
     while pending or order:
-        if order and step in (2, 3, 5, 6, 9, 12):
+        if order and step in (2, 3, 5, 6, 9, 12, 14):
             k = order.pop(0)
             pending[S.add(*arrs[k], bs[k])] = k
         for slot in S.step():
@@ -86,7 +86,7 @@ def test_session_equals_batch_solve(H, ctx, precision):
     bits of hsps_batch_solve of that system alone at the same tile rows."""
     arrs, oracles, bs = systems(H)
     p = params(H, precision)
-    for k in [0, 1, 2, 5]:
+    for k in [0, 1, 2, 5, 7]:
         n = oracles[k].rows
         S = H.Session(ctx, 2, n, len(arrs[k][1]), p)
         out = {}
