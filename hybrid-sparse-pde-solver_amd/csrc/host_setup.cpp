@@ -3,12 +3,18 @@
 // =============================================================================
 #include "host_setup.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <functional>
 #include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 
 namespace hsps_host {
@@ -67,110 +73,202 @@ HostCsr ilu0(const HostCsr& A) {
 // collected and sorted at the end.  A bitmap answers "active?" in O(1), and
 // only the columns the row touched are reset.  Same factors, bit for bit;
 // O(nnz(L\U) * row length) per row instead of O(n).
+namespace {
+// Factor rows of one thread: chunks that never move, so other threads can
+// read a finished row through its pointers while this one appends.
+struct RowStore {
+    std::vector<std::unique_ptr<int[]>> ci;
+    std::vector<std::unique_ptr<double[]>> v;
+    size_t cap = 0, used = 0;
+    std::pair<int*, double*> take(size_t m) {
+        if (ci.empty() || used + m > cap) {
+            cap = std::max<size_t>(m, size_t(1) << 20);
+            ci.emplace_back(new int[cap]);
+            v.emplace_back(new double[cap]);
+            used = 0;
+        }
+        std::pair<int*, double*> r(ci.back().get() + used, v.back().get() + used);
+        used += m;
+        return r;
+    }
+};
+
+constexpr int kIlutBlock = 512;        // rows per block at least (and the factor's lower bandwidth)
+constexpr int kIlutPublish = 16;       // a block's progress is published every this many rows
+struct alignas(64) Progress {          // one cache line per block's count (no false sharing)
+    std::atomic<int> rows{0};
+};
+constexpr int kIlutMinRows = 65536;    // below this one thread
+constexpr int kIlutMaxThreads = 8;
+
+int ilut_threads(int n, int nb) {
+    int t = 1;
+    if (const char* e = std::getenv("HSPS_ILUT_THREADS")) {
+        t = std::max(1, std::atoi(e));
+    } else if (n >= kIlutMinRows) {
+        t = std::min(kIlutMaxThreads, std::max(1, omp_get_max_threads()));
+    }
+    return std::max(1, std::min(t, nb));
+}
+}  // namespace
+
+// Rows run in blocks of S >= the lower bandwidth bw of A, dealt round-robin to
+// threads.  Row i reads the finished U rows k of the columns it eliminates,
+// all in [i - bw, i) (a fill column lies between a pivot row and i), so in
+// its own block (done by its own thread, in order) or the previous block:
+// for those it waits until that block's thread has published row k (a
+// per-block count of finished rows, release / acquire).  The lowest
+// unfinished block never waits, so every wait ends.  Each row's arithmetic
+// is the sequential scan's, so the factors are the same bits for any number
+// of threads (HSPS_ILUT_THREADS sets it; by default up to 8 from 65,536 rows).
 HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     const int n = A.rows;
-    // finished rows, flat: L strictly lower, U from the diagonal on
-    std::vector<long long> lp(n + 1, 0), up(n + 1, 0);
-    std::vector<int> lc, uc;
-    std::vector<double> lv, uv;
-    lc.reserve(A.ci.size());
-    lv.reserve(A.ci.size());
-    uc.reserve(A.ci.size());
-    uv.reserve(A.ci.size());
-    std::vector<double> w(n, 0.0), udiag(n, 0.0);
-    std::vector<unsigned long long> act((size_t)n / 64 + 1, 0);  // active columns of the work row
-    std::vector<char> seen(n, 0);
-    std::vector<int> touched, lower, upper;
-    std::vector<std::pair<int, double>> Lr, Ur;
-    auto on = [&](int j) { return (act[(size_t)j >> 6] >> (j & 63)) & 1ull; };
-    auto set = [&](int j) { act[(size_t)j >> 6] |= 1ull << (j & 63); };
-    auto clr = [&](int j) { act[(size_t)j >> 6] &= ~(1ull << (j & 63)); };
-    auto touch = [&](int j) {
-        if (!seen[j]) { seen[j] = 1; touched.push_back(j); }
-    };
-    for (int i = 0; i < n; ++i) {
-        touched.clear();
-        lower.clear();
-        upper.clear();
-        Lr.clear();
-        Ur.clear();
-        double rn = 0.0;
-        for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
-            const int j = A.ci[p];
-            w[j] = A.v[p];
-            if (!on(j)) {
-                set(j);
-                if (j < i) lower.push_back(j);
-                else if (j > i) upper.push_back(j);
-            }
-            touch(j);
-            rn += A.v[p] * A.v[p];
-        }
-        std::sort(lower.begin(), lower.end());
-        const double tau = tau_rel * std::sqrt(rn);
-        // pivot columns k < i in increasing order (ilu.cpp:121-146)
-        for (size_t t = 0; t < lower.size(); ++t) {
-            const int k = lower[t];
-            if (std::abs(w[k]) < tau) { w[k] = 0.0; clr(k); continue; }
-            const double ukk = udiag[k];
-            if (std::abs(ukk) < kEps) { clr(k); continue; }
-            const double lik = w[k] / ukk;
-            w[k] = lik;
-            for (long long q = up[k]; q < up[k + 1]; ++q) {
-                const int j = uc[q];
-                if (j <= k) continue;
-                if (!on(j) && std::abs(lik * uv[q]) >= tau) {
-                    set(j);
-                    if (j < i) lower.insert(std::upper_bound(lower.begin() + (long)t + 1, lower.end(), j), j);
-                    else if (j > i) upper.push_back(j);
-                }
-                w[j] -= lik * uv[q];
-                touch(j);
-            }
-            if (std::abs(w[k]) >= tau) Lr.emplace_back(k, w[k]);  // ilu.cpp:151-155, ascending k
-        }
-        double d = w[i];
-        if (std::abs(d) < kEps) d = (d >= 0 ? 1.0 : -1.0) * kEps * 1e6;
-        Ur.emplace_back(i, d);
-        std::sort(upper.begin(), upper.end());
-        for (int j : upper)
-            if (std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
-        for (int j : touched) {
-            w[j] = 0.0;
-            clr(j);
-            seen[j] = 0;
-        }
-        w[i] = 0.0;
-        clr(i);
-        const int keep = A.rp[i + 1] - A.rp[i] + fill_p;
-        auto limit = [keep](std::vector<std::pair<int, double>>& row) {
-            if (keep <= 0 || (int)row.size() <= keep) return;
-            std::partial_sort(row.begin(), row.begin() + keep, row.end(),
-                              [](const auto& a, const auto& b) { return std::abs(a.second) > std::abs(b.second); });
-            row.resize(keep);
-            std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    int bw = 0;
+    for (int i = 0; i < n; ++i)
+        for (int p = A.rp[i]; p < A.rp[i + 1]; ++p)
+            if (A.ci[p] < i) bw = std::max(bw, i - A.ci[p]);
+    const int S = std::max(bw, kIlutBlock);
+    const int nb = (n + S - 1) / S;
+    const int T = ilut_threads(n, nb);
+    // finished rows: L strictly lower, U from the diagonal on
+    std::vector<const int*> lcp(n), ucp(n);
+    std::vector<const double*> lvp(n), uvp(n);
+    std::vector<int> llen(n, 0), ulen(n, 0);
+    std::vector<double> udiag(n, 0.0);
+    std::unique_ptr<Progress[]> done(new Progress[std::max(nb, 1)]);
+    std::vector<RowStore> stores(T);
+    auto worker = [&](int t) {
+        RowStore& st = stores[t];
+        std::vector<double> w(n, 0.0);
+        std::vector<unsigned long long> act((size_t)n / 64 + 1, 0);  // active columns of the work row
+        std::vector<char> seen(n, 0);
+        std::vector<int> touched, lower, upper;
+        std::vector<std::pair<int, double>> Lr, Ur;
+        auto on = [&](int j) { return (act[(size_t)j >> 6] >> (j & 63)) & 1ull; };
+        auto set = [&](int j) { act[(size_t)j >> 6] |= 1ull << (j & 63); };
+        auto clr = [&](int j) { act[(size_t)j >> 6] &= ~(1ull << (j & 63)); };
+        auto touch = [&](int j) {
+            if (!seen[j]) { seen[j] = 1; touched.push_back(j); }
         };
-        limit(Lr);
-        limit(Ur);
-        udiag[i] = 0.0;
-        for (const auto& e : Ur)
-            if (e.first == i) { udiag[i] = e.second; break; }
-        for (const auto& e : Lr) { lc.push_back(e.first); lv.push_back(e.second); }
-        for (const auto& e : Ur) { uc.push_back(e.first); uv.push_back(e.second); }
-        lp[i + 1] = (long long)lc.size();
-        up[i + 1] = (long long)uc.size();
+        for (int b = t; b < nb; b += T) {
+            const int r0 = b * S, r1 = std::min(n, r0 + S);
+            int prev = 0;  // rows of block b - 1 seen published
+            for (int i = r0; i < r1; ++i) {
+                touched.clear();
+                lower.clear();
+                upper.clear();
+                Lr.clear();
+                Ur.clear();
+                double rn = 0.0;
+                for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                    const int j = A.ci[p];
+                    w[j] = A.v[p];
+                    if (!on(j)) {
+                        set(j);
+                        if (j < i) lower.push_back(j);
+                        else if (j > i) upper.push_back(j);
+                    }
+                    touch(j);
+                    rn += A.v[p] * A.v[p];
+                }
+                std::sort(lower.begin(), lower.end());
+                const double tau = tau_rel * std::sqrt(rn);
+                // pivot columns k < i in increasing order (ilu.cpp:121-146)
+                for (size_t q0 = 0; q0 < lower.size(); ++q0) {
+                    const int k = lower[q0];
+                    if (std::abs(w[k]) < tau) { w[k] = 0.0; clr(k); continue; }
+                    if (k < r0 && k - (b - 1) * S >= prev) {  // row k of the previous block: wait until published
+                        const int need = k - (b - 1) * S + 1;
+                        for (int spin = 0; (prev = done[b - 1].rows.load(std::memory_order_acquire)) < need; ++spin) {
+                            if (spin < 1024) __builtin_ia32_pause();
+                            else std::this_thread::yield();
+                        }
+                    }
+                    const double ukk = udiag[k];
+                    if (std::abs(ukk) < kEps) { clr(k); continue; }
+                    const double lik = w[k] / ukk;
+                    w[k] = lik;
+                    const int* uck = ucp[k];
+                    const double* uvk = uvp[k];
+                    for (int q = 0; q < ulen[k]; ++q) {
+                        const int j = uck[q];
+                        if (j <= k) continue;
+                        if (!on(j) && std::abs(lik * uvk[q]) >= tau) {
+                            set(j);
+                            if (j < i) lower.insert(std::upper_bound(lower.begin() + (long)q0 + 1, lower.end(), j), j);
+                            else if (j > i) upper.push_back(j);
+                        }
+                        w[j] -= lik * uvk[q];
+                        touch(j);
+                    }
+                    if (std::abs(w[k]) >= tau) Lr.emplace_back(k, w[k]);  // ilu.cpp:151-155, ascending k
+                }
+                double d = w[i];
+                if (std::abs(d) < kEps) d = (d >= 0 ? 1.0 : -1.0) * kEps * 1e6;
+                Ur.emplace_back(i, d);
+                std::sort(upper.begin(), upper.end());
+                for (int j : upper)
+                    if (std::abs(w[j]) >= tau) Ur.emplace_back(j, w[j]);
+                for (int j : touched) {
+                    w[j] = 0.0;
+                    clr(j);
+                    seen[j] = 0;
+                }
+                w[i] = 0.0;
+                clr(i);
+                const int keep = A.rp[i + 1] - A.rp[i] + fill_p;
+                auto limit = [keep](std::vector<std::pair<int, double>>& row) {
+                    if (keep <= 0 || (int)row.size() <= keep) return;
+                    std::partial_sort(row.begin(), row.begin() + keep, row.end(),
+                                      [](const auto& a, const auto& b) { return std::abs(a.second) > std::abs(b.second); });
+                    row.resize(keep);
+                    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+                };
+                limit(Lr);
+                limit(Ur);
+                double di = 0.0;
+                for (const auto& e : Ur)
+                    if (e.first == i) { di = e.second; break; }
+                udiag[i] = di;
+                auto put = [&](const std::vector<std::pair<int, double>>& row, const int** cp, const double** vp, int* len) {
+                    auto m = st.take(row.size());
+                    for (size_t e = 0; e < row.size(); ++e) {
+                        m.first[e] = row[e].first;
+                        m.second[e] = row[e].second;
+                    }
+                    *cp = m.first;
+                    *vp = m.second;
+                    *len = (int)row.size();
+                };
+                put(Lr, &lcp[i], &lvp[i], &llen[i]);
+                put(Ur, &ucp[i], &uvp[i], &ulen[i]);
+                if ((i - r0 + 1) % kIlutPublish == 0 || i + 1 == r1)
+                    done[b].rows.store(i - r0 + 1, std::memory_order_release);
+            }
+        }
+    };
+    if (T == 1) {
+        worker(0);
+    } else {
+        std::vector<std::thread> th;
+        th.reserve(T - 1);
+        for (int t = 1; t < T; ++t) th.emplace_back(worker, t);
+        worker(0);
+        for (auto& x : th) x.join();
     }
     HostCsr F;
     F.rows = F.cols = n;
     F.rp.assign(n + 1, 0);
-    F.ci.reserve(lc.size() + uc.size());
-    F.v.reserve(lc.size() + uc.size());
+    for (int i = 0; i < n; ++i) F.rp[i + 1] = F.rp[i] + llen[i] + ulen[i];
+    F.ci.resize(F.rp[n]);
+    F.v.resize(F.rp[n]);
+#pragma omp parallel for schedule(static) if (n >= kIlutMinRows)
     for (int i = 0; i < n; ++i) {
-        F.ci.insert(F.ci.end(), lc.begin() + lp[i], lc.begin() + lp[i + 1]);
-        F.v.insert(F.v.end(), lv.begin() + lp[i], lv.begin() + lp[i + 1]);
-        F.ci.insert(F.ci.end(), uc.begin() + up[i], uc.begin() + up[i + 1]);
-        F.v.insert(F.v.end(), uv.begin() + up[i], uv.begin() + up[i + 1]);
-        F.rp[i + 1] = (int)F.ci.size();
+        const int o = F.rp[i];
+        std::copy(lcp[i], lcp[i] + llen[i], F.ci.begin() + o);
+        std::copy(lvp[i], lvp[i] + llen[i], F.v.begin() + o);
+        std::copy(ucp[i], ucp[i] + ulen[i], F.ci.begin() + o + llen[i]);
+        std::copy(uvp[i], uvp[i] + ulen[i], F.v.begin() + o + llen[i]);
     }
     return F;
 }
