@@ -65,6 +65,11 @@ public:
         static int feature_dim() { return 24; }
     };
     MatrixFeatures extract_features(const SparseMatrix& A) const;
+    // The fields select_initial_state reads (n, nnz, density and the device
+    // scan: diagonal dominance, symmetry, norms, the Lanczos condition
+    // estimate, SPD), without the host scans of the extended fields: what a
+    // server needs to pick a request's first ladder state.
+    MatrixFeatures extract_state_features(const SparseMatrix& A) const;
     AdaptiveState select_initial_state(const MatrixFeatures& f) const;
     bool should_escalate(AdaptiveState current_state, int iteration, Real residual_now,
                          const std::vector<Real>& residual_history) const;

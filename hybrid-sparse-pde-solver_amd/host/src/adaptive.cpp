@@ -17,7 +17,7 @@ namespace hsps {
 
 AdaptiveSelector::AdaptiveSelector(const SolverParams& params) : params_(params) {}
 
-AdaptiveSelector::MatrixFeatures AdaptiveSelector::extract_features(const SparseMatrix& A) const {
+AdaptiveSelector::MatrixFeatures AdaptiveSelector::extract_state_features(const SparseMatrix& A) const {
     MatrixFeatures f;
     double dev[8];
     dev_check(hsps_features(A.device(), dev));  // Lanczos on the device + structural scans
@@ -32,6 +32,11 @@ AdaptiveSelector::MatrixFeatures AdaptiveSelector::extract_features(const Sparse
     f.spectral_radius_est = dev[6];
     f.lambda_min_est = dev[6] / dev[5];
     f.is_spd = dev[7] != 0.0;
+    return f;
+}
+
+AdaptiveSelector::MatrixFeatures AdaptiveSelector::extract_features(const SparseMatrix& A) const {
+    MatrixFeatures f = extract_state_features(A);
     // extended fields (adaptive_selector.cpp:130-196)
     f.bandwidth = A.bandwidth();
     f.avg_nnz_per_row = f.n > 0 ? static_cast<double>(f.nnz) / f.n : 0.0;

@@ -184,7 +184,7 @@ void SolveServer::start(const Req& r) {
     AdaptiveState state;
     try {
         AdaptiveSelector sel(r->p);
-        state = sel.select_initial_state(sel.extract_features(r->A));
+        state = sel.select_initial_state(sel.extract_state_features(r->A));
         r->res.classify_s = std::chrono::duration<double>(Clock::now() - r->t_admit).count();
     } catch (const std::exception& e) {
         HSPS_LOG_WARN("SolveServer: request ", r->id, " features failed: ", e.what());
