@@ -253,8 +253,11 @@ k_tri_band_mc(int NB, int T_band, int W, int S, int NWT, const V* __restrict__ v
     static_assert(H >= 1 && H <= kBandHistMax, "history depth");
     constexpr int P = kBandMcChunk;
     constexpr int SW = SlotWords<T>::N;
-    // steps of loads in flight: one wave per SIMD leaves registers for a deep ring
-    constexpr int kAhead = (K <= 2 ? 16 : K <= 4 ? 8 : 4) * (sizeof(T) == 4 ? 2 : 1);
+    // steps of loads in flight: one wave per SIMD leaves registers for a deep
+    // ring; at most 16 (FP32 five-point factors: 32 steps ahead made the
+    // conv-diff 1024^2 mixed FGMRES + ILU(0) solve 14.1 ms against 13.3 ms at
+    // 16; 8 is slower again)
+    constexpr int kAhead = std::min(16, (K <= 2 ? 16 : K <= 4 ? 8 : 4) * (sizeof(T) == 4 ? 2 : 1));
     static_assert(P % kAhead == 0, "ring rounds fit a chunk");
     static_assert(P <= 64, "a chunk's handoff fits the lanes");
     if (skip && *skip) return;
