@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <exception>
 #include <functional>
 #include <limits>
 #include <memory>
@@ -137,6 +138,10 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
     std::vector<double> udiag(n, 0.0);
     std::unique_ptr<Progress[]> done(new Progress[std::max(nb, 1)]);
     std::vector<RowStore> stores(T);
+    // a thread that throws (allocation) stops the others' waits; the first
+    // exception is rethrown once every thread has returned
+    std::atomic<bool> failed{false};
+    std::vector<std::exception_ptr> errs(T);
     auto worker = [&](int t) {
         RowStore& st = stores[t];
         std::vector<double> w(n, 0.0);
@@ -180,6 +185,7 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
                     if (k < r0 && k - (b - 1) * S >= prev) {  // row k of the previous block: wait until published
                         const int need = k - (b - 1) * S + 1;
                         for (int spin = 0; (prev = done[b - 1].rows.load(std::memory_order_acquire)) < need; ++spin) {
+                            if (failed.load(std::memory_order_relaxed)) return;
                             if (spin < 1024) __builtin_ia32_pause();
                             else std::this_thread::yield();
                         }
@@ -247,14 +253,24 @@ HostCsr ilut(const HostCsr& A, double tau_rel, int fill_p) {
             }
         }
     };
+    auto guarded_worker = [&](int t) {
+        try {
+            worker(t);
+        } catch (...) {
+            errs[t] = std::current_exception();
+            failed.store(true);
+        }
+    };
     if (T == 1) {
         worker(0);
     } else {
         std::vector<std::thread> th;
         th.reserve(T - 1);
-        for (int t = 1; t < T; ++t) th.emplace_back(worker, t);
-        worker(0);
+        for (int t = 1; t < T; ++t) th.emplace_back(guarded_worker, t);
+        guarded_worker(0);
         for (auto& x : th) x.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
     }
     HostCsr F;
     F.rows = F.cols = n;
