@@ -68,4 +68,17 @@ TEST(dataset_results_replay) {
         expect_less(s.true_residual, 1e-8, std::string(r.name) + " true residual");
     }
 }
+TEST(state_features_pick_the_same_state) {
+    // the server's classification (extract_state_features) reads every field
+    // select_initial_state reads, with the same values as extract_features
+    const SparseMatrix mats[] = {SparseMatrix::poisson_2d(40), SparseMatrix::convection_diffusion_2d(30, 0.01, 1.0, 0.3),
+                                 SparseMatrix::anisotropic_2d(36, 1e-3), SparseMatrix::convection_diffusion_2d(24, 1.0, 0.0, 0.0)};
+    AdaptiveSelector sel;
+    for (const auto& A : mats) {
+        const auto f = sel.extract_features(A), g = sel.extract_state_features(A);
+        expect_true(f.is_spd == g.is_spd && f.diag_dominance == g.diag_dominance, "state fields equal");
+        expect_less(std::abs(f.estimated_cond - g.estimated_cond), 1e-12 * f.estimated_cond, "condition estimate");
+        expect_true(sel.select_initial_state(f) == sel.select_initial_state(g), "same initial state");
+    }
+}
 int main() { return run_all(); }
