@@ -824,7 +824,7 @@ def main():
     ap.add_argument("--workers", type=int, default=8, help="serve: concurrent ladder solves")
     ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: admission budget (bytes)")
     ap.add_argument("--serve-nmax", type=float, default=4e6, help="serve: largest request (rows)")
-    ap.add_argument("--serve-batch-rows", type=float, default=262144,
+    ap.add_argument("--serve-batch-rows", type=float, default=1048576,
                     help="serve: largest EASY request that joins the session (rows)")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--grid", type=int, default=2048)
